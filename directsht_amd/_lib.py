@@ -1,0 +1,64 @@
+"""ctypes binding of libdsht.so (declared in include/dsht.h).
+
+torch is imported before the library is opened: torch ships its own HIP
+runtime (libamdhip64.so, SONAME libamdhip64.so.7) and libdsht.so must bind to
+that same runtime so torch's device pointers and streams are valid in it.
+Loading fails loudly when the library has not been built -- there is no CPU
+fallback.
+"""
+import ctypes as ct
+import os
+import threading
+
+import torch  # noqa: F401  (must precede the CDLL: shared HIP runtime)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdsht.so")
+
+DSHT_FP64, DSHT_FP32 = 0, 1
+DSHT_MODE_CALL, DSHT_MODE_BASIC = 0, 1
+DSHT_STATUS_EQU_RANGE, DSHT_STATUS_POLAR_RANGE = 1, 2
+
+# every entry point of include/dsht.h: name -> (restype, argtypes)
+_dp = ct.POINTER(ct.c_double)
+_vp = ct.c_void_p
+SIGNATURES = {
+    "make_table": (ct.c_int, [ct.c_int, ct.c_int, ct.c_double, _dp, _dp]),
+    "do_transform": (ct.c_int, [ct.c_int, ct.c_int, ct.c_double, _dp, _dp, ct.c_int,
+                                _dp, _dp, _dp, _dp, _dp]),
+    "dsht_plan_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_double, ct.c_int, ct.POINTER(_vp)]),
+    "dsht_plan_destroy": (ct.c_int, [_vp]),
+    "dsht_plan_footprint": (ct.c_int, [_vp, ct.POINTER(ct.c_int64)]),
+    "dsht_plan_estimate": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int64,
+                                      ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
+    "dsht_plan_get_tables": (ct.c_int, [_vp, _vp, _vp]),
+    "dsht_transform": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "dsht_last_error": (ct.c_char_p, []),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load():
+    """Open libdsht.so once and declare its signatures."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built; run `python -m directsht_amd.build` "
+                    "(the direct SHT has no CPU fallback)")
+            lib = ct.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().dsht_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (code {rc}): {msg}")

@@ -1,0 +1,460 @@
+// C ABI of libdsht.so: plan management, transform orchestration and the
+// reference-compatible make_table / do_transform entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace dsht {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups, std::vector<int64_t>& mp_off,
+                  int64_t& tab_vec4) {
+  groups.clear();
+  mp_off.assign(2 * (size_t)Nl, 0);
+  int64_t off = 0;
+  for (int m = 0; m < Nl; ++m) {
+    for (int P = 0; P < 2; ++P) {
+      mp_off[2 * m + P] = off;
+      const int R = rows_mp(Nl, m, P);
+      for (int i0 = 0; i0 < R; i0 += kGroupRows) {
+        GroupDesc g;
+        g.m = m;
+        g.P = P;
+        g.i0 = i0;
+        g.nrows = std::min(kGroupRows, R - i0);
+        g.off = off;
+        groups.push_back(g);
+        off += (int64_t)g.nrows * Npair;
+      }
+    }
+  }
+  tab_vec4 = off;
+}
+
+static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+struct WsSizes {
+  int64_t points, chunks, runs;
+};
+
+static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
+  WsSizes z;
+  z.points = std::max<int64_t>(Np, 1);
+  z.chunks = (Np + chunk - 1) / chunk;
+  z.runs = z.chunks + num_segments(Nx) + 1;
+  return z;
+}
+
+int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
+  const WsSizes z = ws_sizes(Nx, Np, chunk);
+  const int Npair = (Nx + 1) / 2;
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  const int nseg = num_segments(Nx);
+  int64_t b = 0;
+  b += align256(z.points * (6 * sizeof(double) + sizeof(int)));
+  b += 2 * align256((z.chunks + 1) * sizeof(int));
+  b += align256(z.runs * sizeof(int));
+  b += align256(z.runs * (int64_t)Nl * 8 * sizeof(double));
+  b += 3 * align256((nseg + 2) * sizeof(int));
+  b += align256((int64_t)Nl * 2 * Npair * 16 * (precision == DSHT_FP32 ? 4 : 8));
+  b += 2 * align256(Nlm * 2 * sizeof(double));
+  b += 256;
+  return b;
+}
+
+static void free_workspace(Workspace& ws) {
+  SortedPoints& sp = ws.sp;
+  for (void* p : {(void*)sp.ere, (void*)sp.eim, (void*)sp.h0, (void*)sp.h1, (void*)sp.h2,
+                  (void*)sp.h3, (void*)sp.seg, (void*)ws.run_count, (void*)ws.run_base,
+                  (void*)ws.run_seg, (void*)ws.part, (void*)ws.seg_count, (void*)ws.seg_start,
+                  (void*)ws.seg_cursor, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol,
+                  (void*)ws.flags})
+    if (p) (void)hipFree(p);
+  ws = Workspace{};
+}
+
+#define DSHT_ALLOC(ptr, nbytes_)                                                        \
+  do {                                                                                  \
+    const size_t n_ = (size_t)(nbytes_);                                                \
+    hipError_t e_ = hipMalloc((void**)&(ptr), std::max<size_t>(n_, 16));                \
+    if (e_ != hipSuccess) {                                                             \
+      set_error(std::string("hipMalloc(") + std::to_string(n_) + " bytes): " +         \
+                hipGetErrorString(e_));                                                 \
+      return DSHT_ENOMEM;                                                               \
+    }                                                                                   \
+    ws.bytes += (int64_t)n_;                                                            \
+  } while (0)
+
+// Grow the workspace to hold a transform of Np points.  Growth waits for the
+// device so buffers still in use by queued work are never freed under it.
+static int ensure_workspace(Plan& pl, int64_t Np) {
+  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
+  Workspace& ws = pl.ws;
+  if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs)
+    return DSHT_OK;
+  DSHT_HIP(hipDeviceSynchronize());
+  WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
+               std::max(z.runs, ws.cap_runs)};
+  free_workspace(ws);
+  SortedPoints& sp = ws.sp;
+  const int64_t P = want.points;
+  DSHT_ALLOC(sp.ere, P * sizeof(double));
+  DSHT_ALLOC(sp.eim, P * sizeof(double));
+  DSHT_ALLOC(sp.h0, P * sizeof(double));
+  DSHT_ALLOC(sp.h1, P * sizeof(double));
+  DSHT_ALLOC(sp.h2, P * sizeof(double));
+  DSHT_ALLOC(sp.h3, P * sizeof(double));
+  DSHT_ALLOC(sp.seg, P * sizeof(int));
+  DSHT_ALLOC(ws.run_count, (want.chunks + 1) * sizeof(int));
+  DSHT_ALLOC(ws.run_base, (want.chunks + 1) * sizeof(int));
+  DSHT_ALLOC(ws.run_seg, want.runs * sizeof(int));
+  DSHT_ALLOC(ws.part, want.runs * (int64_t)pl.Nl * 8 * sizeof(double));
+  DSHT_ALLOC(ws.seg_count, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8));
+  DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sizeof(double));
+  DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sizeof(double));
+  DSHT_ALLOC(ws.flags, 4 * sizeof(int));
+  ws.cap_points = want.points;
+  ws.cap_chunks = want.chunks;
+  ws.cap_runs = want.runs;
+  return DSHT_OK;
+}
+
+// d^l_{l,m}(pi/2) = (-1)^{l-m} sqrt(C(2l, l+m)) / 2^l as mantissa * 2^exp, and
+// the recursion coefficients A(m'), 1/B(m'), for every l < Nl.
+static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int>& ex,
+                               std::vector<double>& A, std::vector<double>& iB) {
+  const int64_t n = (int64_t)Nl * (Nl + 1) / 2;
+  top.assign(n, 0.0);
+  ex.assign(n, 0);
+  A.assign(n, 0.0);
+  iB.assign(n, 0.0);
+  double d0 = 1.0;  // d^l_{l,0}
+  for (int l = 0; l < Nl; ++l) {
+    if (l > 0) d0 *= -std::sqrt((2.0 * l - 1.0) / (2.0 * l));
+    const int64_t tri = (int64_t)l * (l + 1) / 2;
+    double v = d0;
+    int e = 0;
+    for (int m = 0; m <= l; ++m) {
+      top[tri + m] = v;
+      ex[tri + m] = e;
+      if (m < l) {
+        v *= -std::sqrt((double)(l - m) / (double)(l + m + 1));
+        if (std::fabs(v) < 0x1p-400) {
+          v *= 0x1p400;
+          e -= 400;
+        }
+      }
+    }
+    for (int mp = 1; mp <= l; ++mp) {
+      A[tri + mp] = std::sqrt((double)(l - mp) * (double)(l + mp + 1));
+      iB[tri + mp] = 1.0 / std::sqrt((double)(l + mp) * (double)(l - mp + 1));
+    }
+  }
+}
+
+static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool alloc_tables,
+                     bool build) {
+  if (Nl < 2 || Nx < 2 || !(xmax > 0.0 && xmax < 1.0)) {
+    set_error("dsht: need Nl >= 2, Nx >= 2 and 0 < xmax < 1");
+    return DSHT_EINVAL;
+  }
+  if (Nl > 2048) {
+    set_error("dsht: Nl > 2048 is not supported");
+    return DSHT_EINVAL;
+  }
+  if (precision != DSHT_FP64 && precision != DSHT_FP32) {
+    set_error("dsht: precision must be DSHT_FP64 or DSHT_FP32");
+    return DSHT_EINVAL;
+  }
+  DSHT_HIP(hipGetDevice(&pl.device));
+  pl.Nl = Nl;
+  pl.Nx = Nx;
+  pl.Npair = (Nx + 1) / 2;
+  pl.nseg = num_segments(Nx);
+  pl.Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  pl.xmax = xmax;
+  pl.dx = xmax / (double)(Nx - 1);
+  pl.acos_xmax = std::acos(xmax);
+  pl.acos_mxmax = std::acos(-xmax);
+  pl.precision = precision;
+  std::vector<GroupDesc> groups;
+  std::vector<int64_t> mp_off;
+  build_layout(Nl, pl.Npair, groups, mp_off, pl.tab_vec4);
+  pl.ngroups = (int)groups.size();
+  Workspace& ws = pl.ws;  // DSHT_ALLOC counts bytes into ws.bytes; moved below
+  DSHT_ALLOC(pl.groups, groups.size() * sizeof(GroupDesc));
+  DSHT_ALLOC(pl.mp_off, mp_off.size() * sizeof(int64_t));
+  DSHT_HIP(hipMemcpy(pl.groups, groups.data(), groups.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
+  DSHT_HIP(hipMemcpy(pl.mp_off, mp_off.data(), mp_off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  if (alloc_tables && precision == DSHT_FP64) DSHT_ALLOC(pl.tab64, pl.tab_vec4 * 4 * sizeof(double));
+  if (alloc_tables && precision == DSHT_FP32) DSHT_ALLOC(pl.tab32, pl.tab_vec4 * 4 * sizeof(float));
+  std::vector<double> top, A, iB;
+  std::vector<int> ex;
+  rotation_constants(Nl, top, ex, A, iB);
+  DSHT_ALLOC(pl.rot_top, top.size() * sizeof(double));
+  DSHT_ALLOC(pl.rot_exp, ex.size() * sizeof(int));
+  DSHT_ALLOC(pl.rot_A, A.size() * sizeof(double));
+  DSHT_ALLOC(pl.rot_iB, iB.size() * sizeof(double));
+  DSHT_HIP(hipMemcpy(pl.rot_top, top.data(), top.size() * sizeof(double), hipMemcpyHostToDevice));
+  DSHT_HIP(hipMemcpy(pl.rot_exp, ex.data(), ex.size() * sizeof(int), hipMemcpyHostToDevice));
+  DSHT_HIP(hipMemcpy(pl.rot_A, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
+  DSHT_HIP(hipMemcpy(pl.rot_iB, iB.data(), iB.size() * sizeof(double), hipMemcpyHostToDevice));
+  pl.table_bytes = ws.bytes;
+  ws.bytes = 0;
+  if (alloc_tables && build) {
+    int rc = launch_tables(pl, precision == DSHT_FP64, precision == DSHT_FP32, nullptr, nullptr, 0);
+    if (rc) return rc;
+    DSHT_HIP(hipDeviceSynchronize());
+  }
+  return DSHT_OK;
+}
+
+static void plan_free(Plan& pl) {
+  free_workspace(pl.ws);
+  for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64, (void*)pl.tab32,
+                  (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_A, (void*)pl.rot_iB})
+    if (p) (void)hipFree(p);
+  pl = Plan{};
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    else prev = -1;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+static int run_transform(Plan& pl, int64_t Np, const double* theta, const double* phi,
+                         const double* wt, double* alm, int* status, int mode, hipStream_t s) {
+  if (Np < 0 || Np >= (int64_t)1 << 31) {
+    set_error("dsht_transform: need 0 <= Np < 2^31");
+    return DSHT_EINVAL;
+  }
+  if (Np > 0 && (!theta || !phi || !wt)) {
+    set_error("dsht_transform: null input pointer");
+    return DSHT_EINVAL;
+  }
+  if (!alm) {
+    set_error("dsht_transform: null output pointer");
+    return DSHT_EINVAL;
+  }
+  int rc = ensure_workspace(pl, Np);
+  if (rc) return rc;
+  pl.ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+  if ((rc = launch_points(pl, Np, theta, phi, wt, mode, s))) return rc;
+  if ((rc = launch_scatter(pl, Np, s))) return rc;
+  if ((rc = launch_fold(pl, s))) return rc;
+  if ((rc = launch_contract(pl, s))) return rc;
+  if ((rc = launch_rotate_combine(pl, alm, status, mode, s))) return rc;
+  return DSHT_OK;
+}
+
+}  // namespace dsht
+
+struct dsht_plan {
+  dsht::Plan pl;
+};
+
+using namespace dsht;
+
+extern "C" {
+
+const char* dsht_last_error(void) { return g_err.c_str(); }
+
+int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** out) {
+  if (!out) {
+    set_error("dsht_plan_create: null output");
+    return DSHT_EINVAL;
+  }
+  *out = nullptr;
+  dsht_plan* p = new dsht_plan();
+  int rc = plan_init(p->pl, Nl, Nx, xmax, precision, true, true);
+  if (rc) {
+    plan_free(p->pl);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return DSHT_OK;
+}
+
+int dsht_plan_destroy(dsht_plan* p) {
+  if (!p) return DSHT_OK;
+  DeviceGuard g(p->pl.device);
+  (void)hipDeviceSynchronize();
+  plan_free(p->pl);
+  delete p;
+  return DSHT_OK;
+}
+
+int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {
+  if (!p || !bytes) {
+    set_error("dsht_plan_footprint: null argument");
+    return DSHT_EINVAL;
+  }
+  *bytes = p->pl.table_bytes + p->pl.ws.bytes;
+  return DSHT_OK;
+}
+
+int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table_bytes,
+                       int64_t* work_bytes) {
+  if (Nl < 2 || Nx < 2) {
+    set_error("dsht_plan_estimate: need Nl >= 2, Nx >= 2");
+    return DSHT_EINVAL;
+  }
+  std::vector<GroupDesc> groups;
+  std::vector<int64_t> mp_off;
+  int64_t vec4 = 0;
+  const int Npair = (Nx + 1) / 2;
+  build_layout(Nl, Npair, groups, mp_off, vec4);
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  if (table_bytes)
+    *table_bytes = vec4 * 4 * (precision == DSHT_FP32 ? 4 : 8) + Nlm * (3 * 8 + 4) +
+                   (int64_t)groups.size() * sizeof(GroupDesc) + (int64_t)mp_off.size() * 8;
+  if (work_bytes) *work_bytes = workspace_bytes(Nl, Nx, precision, Np, 2048);
+  return DSHT_OK;
+}
+
+int dsht_plan_get_tables(dsht_plan* p, double* Yv, double* Yd) {
+  if (!p) {
+    set_error("dsht_plan_get_tables: null plan");
+    return DSHT_EINVAL;
+  }
+  Plan& pl = p->pl;
+  DeviceGuard g(pl.device);
+  const size_t bytes = (size_t)pl.Nlm * pl.Nx * sizeof(double);
+  double *dv = nullptr, *dd = nullptr;
+  if (hipMalloc((void**)&dv, bytes) != hipSuccess || hipMalloc((void**)&dd, bytes) != hipSuccess) {
+    if (dv) (void)hipFree(dv);
+    set_error("dsht_plan_get_tables: out of device memory");
+    return DSHT_ENOMEM;
+  }
+  int rc = launch_tables(pl, false, false, dv, dd, 0);
+  if (rc == DSHT_OK) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess && Yv) e = hipMemcpy(Yv, dv, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && Yd) e = hipMemcpy(Yd, dd, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_error(std::string("dsht_plan_get_tables: ") + hipGetErrorString(e));
+      rc = DSHT_EHIP;
+    }
+  }
+  (void)hipFree(dv);
+  (void)hipFree(dd);
+  return rc;
+}
+
+int dsht_transform(dsht_plan* p, int64_t Np, const double* theta, const double* phi,
+                   const double* wt, double* alm, int* status, int mode, void* stream) {
+  if (!p) {
+    set_error("dsht_transform: null plan");
+    return DSHT_EINVAL;
+  }
+  if (mode != DSHT_MODE_CALL && mode != DSHT_MODE_BASIC) {
+    set_error("dsht_transform: bad mode");
+    return DSHT_EINVAL;
+  }
+  DeviceGuard g(p->pl.device);
+  return run_transform(p->pl, Np, theta, phi, wt, alm, status, mode, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Reference (csht) C ABI
+// ---------------------------------------------------------------------------
+
+int make_table(int Nl, int Nx, double xmax, double Yv[], double Yd[]) {
+  Plan pl;
+  int rc = plan_init(pl, Nl, Nx, xmax, DSHT_FP64, false, false);
+  if (rc == DSHT_OK) {
+    const size_t bytes = (size_t)pl.Nlm * Nx * sizeof(double);
+    double *dv = nullptr, *dd = nullptr;
+    if (hipMalloc((void**)&dv, bytes) != hipSuccess || hipMalloc((void**)&dd, bytes) != hipSuccess) {
+      set_error("make_table: out of device memory");
+      rc = DSHT_ENOMEM;
+    } else {
+      rc = launch_tables(pl, false, false, dv, dd, 0);
+      if (rc == DSHT_OK) {
+        if (hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(Yv, dv, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(Yd, dd, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+          set_error("make_table: device copy failed");
+          rc = DSHT_EHIP;
+        }
+      }
+    }
+    if (dv) (void)hipFree(dv);
+    if (dd) (void)hipFree(dd);
+  }
+  plan_free(pl);
+  return rc == DSHT_OK ? 0 : 1;
+}
+
+int do_transform(int Nl, int Nx, double xmax, double Yv[], double Yd[], int Np, double cost[],
+                 double phi[], double wt[], double carr[], double sarr[]) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  Plan pl;
+  int rc = plan_init(pl, Nl, Nx, xmax, DSHT_FP64, true, false);
+  const size_t tb = (size_t)pl.Nlm * Nx * sizeof(double);
+  double *dv = nullptr, *dd = nullptr, *dc = nullptr, *dp = nullptr, *dw = nullptr, *da = nullptr;
+  std::vector<double> alm;
+  if (rc == DSHT_OK) {
+    const size_t pb = std::max<size_t>(1, (size_t)Np) * sizeof(double);
+    if (hipMalloc((void**)&dv, tb) != hipSuccess || hipMalloc((void**)&dd, tb) != hipSuccess ||
+        hipMalloc((void**)&dc, pb) != hipSuccess || hipMalloc((void**)&dp, pb) != hipSuccess ||
+        hipMalloc((void**)&dw, pb) != hipSuccess ||
+        hipMalloc((void**)&da, pl.Nlm * 2 * sizeof(double)) != hipSuccess) {
+      set_error("do_transform: out of device memory");
+      rc = DSHT_ENOMEM;
+    }
+  }
+  if (rc == DSHT_OK) {
+    // the caller's tables, not a rebuild: same semantics as csht
+    if (hipMemcpy(dv, Yv, tb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dd, Yd, tb, hipMemcpyHostToDevice) != hipSuccess ||
+        (Np > 0 && (hipMemcpy(dc, cost, Np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(dp, phi, Np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(dw, wt, Np * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))) {
+      set_error("do_transform: host to device copy failed");
+      rc = DSHT_EHIP;
+    }
+  }
+  if (rc == DSHT_OK) rc = launch_relayout(pl, dv, dd, 0);
+  if (rc == DSHT_OK) rc = run_transform(pl, Np, dc, dp, dw, da, nullptr, DSHT_MODE_CSHT, 0);
+  if (rc == DSHT_OK) {
+    alm.resize(pl.Nlm * 2);
+    if (hipMemcpy(alm.data(), da, alm.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+      set_error("do_transform: device to host copy failed");
+      rc = DSHT_EHIP;
+    }
+  }
+  if (rc == DSHT_OK) {
+    for (int64_t i = 0; i < pl.Nlm; ++i) {
+      carr[i] = alm[2 * i];
+      sarr[i] = -alm[2 * i + 1];
+    }
+  }
+  for (double* q : {dv, dd, dc, dp, dw, da})
+    if (q) (void)hipFree(q);
+  plan_free(pl);
+  return rc == DSHT_OK ? 0 : 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal definitions shared by the HIP sources of libdsht.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/dsht.h"
+
+namespace dsht {
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+#define DSHT_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::dsht::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return DSHT_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+#define DSHT_LAUNCH_CHECK()                                                         \
+  do {                                                                              \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess) {                                                         \
+      ::dsht::set_error(std::string("kernel launch: ") + hipGetErrorString(e_));    \
+      return DSHT_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+// Internal transform mode of the csht-compatible do_transform: the "theta"
+// input holds cos(theta) of points of one sign, only |cos(theta)| is used and
+// no parity or polar split is applied (csht/sht_helper.c:176-286).
+constexpr int DSHT_MODE_CSHT = 2;
+
+// ---------------------------------------------------------------------------
+// Layout of the contraction tables
+// ---------------------------------------------------------------------------
+// For every m and parity P = (l - m) & 1 the rows l = m + P + 2i,
+// i = 0 .. R(m,P)-1, are cut into groups of up to 64 rows (one wavefront
+// each).  A group with nrows rows stores, for every node pair np
+// (nodes 2np, 2np+1), nrows consecutive 4-vectors
+//     { Y(x_2np), Y(x_2np+1), Y'(x_2np), Y'(x_2np+1) }
+// so one wavefront load at fixed np is one contiguous nrows*16-byte (FP32) or
+// nrows*32-byte (FP64) block, and a group is one contiguous stream.
+constexpr int kGroupRows = 64;
+
+struct GroupDesc {
+  int m;        // order
+  int P;        // parity of l - m
+  int i0;       // first row index i within (m, P)
+  int nrows;    // rows in this group (<= 64)
+  int64_t off;  // offset of the group, in 4-vectors
+};
+
+__host__ __device__ inline int rows_mp(int Nl, int m, int P) {
+  int r = Nl - m - P;
+  return r > 0 ? (r + 1) / 2 : 0;
+}
+
+__host__ __device__ inline int64_t hp_index(int ell, int m, int Nl) {
+  return ((int64_t)m * (2 * Nl - 1 - m)) / 2 + ell;
+}
+
+// ---------------------------------------------------------------------------
+// Segments of the sorted point array
+// ---------------------------------------------------------------------------
+// seg = (branch*2 + hemisphere) * (Nx-1) + bin, branch 0 = equatorial,
+// 1 = polar (rotated); hemisphere 0: x >= 0, 1: x < 0; bin j in [0, Nx-2].
+// Points that the reference drops (NaN theta) get seg = nseg (sorted last,
+// never scattered).
+__host__ __device__ inline int num_segments(int Nx) { return 4 * (Nx - 1); }
+
+// Per-point record after sorting (structure of arrays).
+struct SortedPoints {
+  double* ere;   // cos(phi)
+  double* eim;   // -sin(phi)      (e1 = e^{-i phi})
+  double* h0;    // w * h0(t)
+  double* h1;    // w * h1(t)      (dx applied at the fold)
+  double* h2;    // w * h2(t)
+  double* h3;    // w * h3(t)
+  int* seg;      // segment id
+};
+
+}  // namespace dsht

@@ -1,0 +1,70 @@
+// Plan: resident tables + per-transform workspace of one DirectSHT(Nl, Nx, xmax).
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+namespace dsht {
+
+struct Workspace {
+  int64_t cap_points = 0;  // capacity of the sorted-point arrays
+  int64_t cap_chunks = 0;  // capacity of per-chunk arrays
+  int64_t cap_runs = 0;    // capacity of the run partials
+  int cur_chunks = 0;      // scatter chunks of the transform being enqueued
+  SortedPoints sp{};
+  int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
+  int* run_base = nullptr;    // [cap_chunks + 1]: exclusive scan of run_count
+  int* run_seg = nullptr;     // [cap_runs]: segment of each run
+  double* part = nullptr;     // [cap_runs][Nl][8]: per-run v0..v3 (complex)
+  int* seg_count = nullptr;   // [nseg + 2]
+  int* seg_start = nullptr;   // [nseg + 2]
+  int* seg_cursor = nullptr;  // [nseg + 2]
+  void* rhs = nullptr;        // [Nl][2][Npair][16] float or double
+  double* a_equ = nullptr;    // [Nlm][2]
+  double* a_pol = nullptr;    // [Nlm][2]
+  int* flags = nullptr;       // [4]: status bits, n_polar, n_runs, unused
+  int64_t bytes = 0;
+};
+
+struct Plan {
+  int device = 0;
+  int Nl = 0, Nx = 0, Npair = 0, nseg = 0;
+  int64_t Nlm = 0;
+  double xmax = 0, dx = 0, acos_xmax = 0, acos_mxmax = 0;
+  int precision = DSHT_FP64;
+  // scatter decomposition
+  int chunk = 2048;  // sorted points per scatter work item
+  // contraction layout
+  int ngroups = 0;
+  GroupDesc* groups = nullptr;  // device [ngroups]
+  int64_t* mp_off = nullptr;    // device [2*Nl]
+  int64_t tab_vec4 = 0;         // 4-vectors in one table copy
+  double* tab64 = nullptr;
+  float* tab32 = nullptr;
+  int64_t table_bytes = 0;
+  // polar rotation constants, per l at offset l(l+1)/2: d^l_{l,m}(pi/2) as
+  // mantissa * 2^exp for m = 0..l, A(m') and 1/B(m') of the recursion
+  double* rot_top = nullptr;
+  int* rot_exp = nullptr;
+  double* rot_A = nullptr;
+  double* rot_iB = nullptr;
+  Workspace ws;
+};
+
+// Host-side layout computation shared by plan creation and estimates.
+void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups,
+                  std::vector<int64_t>& mp_off, int64_t& tab_vec4);
+int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk);
+
+// Kernel launchers (one per source file).
+int launch_tables(const Plan& pl, bool w64, bool w32, double* hpYv, double* hpYd, hipStream_t s);
+int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipStream_t s);
+int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
+                  int mode, hipStream_t s);
+int launch_scatter(Plan& pl, int64_t Np, hipStream_t s);
+int launch_fold(Plan& pl, hipStream_t s);
+int launch_contract(Plan& pl, hipStream_t s);
+int launch_rotate_combine(Plan& pl, double* alm, int* status, int mode, hipStream_t s);
+
+}  // namespace dsht

@@ -1,0 +1,242 @@
+// Stage (b), part 1: per-point classification, binning and counting sort.
+//
+// For every point (FP64 throughout, as the binning needs -- t = (x-x_j)/dx
+// amplifies the rounding of x by 1/dx):
+//   * polar/equatorial split |cos theta| >= xmax          sht/sht.py:128-130
+//   * polar points rotated by R_y(+90 deg)               sht/sht.py:134-135
+//     (n' = (z, y, -x); any R_z(a) R_y(+-90) gives the same final alm)
+//   * theta range checks of basic_sht                    sht/sht.py:171-172
+//   * hemisphere split x >= 0 / x < 0                     sht/sht.py:193
+//   * bin j = digitize(|x|, x_grid) - 1, t = (|x|-x_j)/dx sht/sht.py:212,221
+//   * Hermite weights w * {h0..h3}(t)                     sht/sht.py:239-241
+// The segment id (branch, hemisphere, bin) is counted into a histogram
+// (LDS-privatized, one global atomic per non-empty bucket per block), scanned,
+// and each point's record is placed into its segment's range of the sorted
+// arrays.  Points whose cos(theta) is NaN are dropped, as the reference's two
+// np.nonzero masks drop them.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "plan.h"
+
+namespace dsht {
+
+struct PointParams {
+  int Nx, nseg, mode;
+  double xmax, dx, acos_xmax, acos_mxmax;
+};
+
+struct Classified {
+  int seg;       // nseg = dropped
+  int polar;     // 1 if the point went through the rotated branch
+  int flag;      // DSHT_STATUS_* bits raised by this point
+  double ere, eim, h0, h1, h2, h3;
+};
+
+__device__ inline Classified classify(double theta, double phi, double w, const PointParams& p) {
+#pragma clang fp contract(off)
+  Classified c;
+  c.seg = p.nseg;
+  c.polar = 0;
+  c.flag = 0;
+  c.ere = 1.0; c.eim = 0.0;
+  c.h0 = c.h1 = c.h2 = c.h3 = 0.0;
+  double st, ct;
+  double xu, sp, cp;
+  int branch;
+  if (p.mode == DSHT_MODE_CSHT) {
+    ct = fabs(theta);  // the input is cos(theta); one hemisphere by contract
+    st = 0.0;
+  } else {
+    sincos(theta, &st, &ct);
+  }
+  const double xv = fabs(ct);
+  if (p.mode == DSHT_MODE_CSHT) {
+    sincos(phi, &sp, &cp);
+    xu = ct;
+    c.ere = cp;
+    c.eim = -sp;
+    branch = 0;
+  } else if (p.mode == DSHT_MODE_CALL && xv >= p.xmax) {
+    // polar: rotate, then basic_sht's checks on the rotated theta
+    sincos(phi, &sp, &cp);
+    const double X = ct, Y = st * sp, Z = -(st * cp);
+    const double rho = sqrt(X * X + Y * Y);
+    const double tr = atan2(rho, Z);
+    if (!(tr > p.acos_xmax && tr < p.acos_mxmax)) c.flag |= DSHT_STATUS_POLAR_RANGE;
+    xu = cos(tr);
+    if (rho > 0.0) {
+      c.ere = X / rho;
+      c.eim = -(Y / rho);
+    }
+    branch = 1;
+    c.polar = 1;
+  } else if (p.mode == DSHT_MODE_BASIC || xv < p.xmax) {
+    if (!(theta >= 0.0 && theta > p.acos_xmax && theta < p.acos_mxmax)) c.flag |= DSHT_STATUS_EQU_RANGE;
+    sincos(phi, &sp, &cp);
+    xu = ct;
+    c.ere = cp;
+    c.eim = -sp;
+    branch = 0;
+  } else {
+    return c;  // NaN cos(theta): dropped
+  }
+  if (xu != xu) return c;  // NaN after rotation: dropped
+  const int h = xu < 0.0 ? 1 : 0;
+  const double ax = fabs(xu);
+  // np.digitize(ax, x_grid) - 1 with x_grid[j] = j * dx (exact FP64 products)
+  int j = (int)fmin(ax / p.dx, (double)(p.Nx - 1));
+  if (j < 0) j = 0;
+  while (j + 1 <= p.Nx - 1 && (double)(j + 1) * p.dx <= ax) ++j;
+  while (j > 0 && (double)j * p.dx > ax) --j;
+  // |x| in [x_{Nx-1}, xmax): the reference would index past the table; the
+  // last interval is used (t slightly above 1).
+  if (j > p.Nx - 2) j = p.Nx - 2;
+  const double t = (ax - (double)j * p.dx) / p.dx;
+  const double omt = 1.0 - t;
+  c.h0 = w * ((2.0 * t + 1.0) * (omt * omt));
+  c.h1 = w * (t * (omt * omt));
+  c.h2 = w * ((t * t) * (3.0 - 2.0 * t));
+  c.h3 = w * ((t * t) * (t - 1.0));
+  c.seg = (branch * 2 + h) * (p.Nx - 1) + j;
+  return c;
+}
+
+// Histogram of segment ids (+1 bucket for dropped points) and the status /
+// polar-count reductions.  flags[0] |= status bits, flags[1] += polar points.
+template <bool kLds>
+__global__ __launch_bounds__(256) void k_hist(int64_t Np, const double* __restrict__ theta,
+                                              const double* __restrict__ phi,
+                                              const double* __restrict__ wt, PointParams p,
+                                              int* __restrict__ seg_count, int* __restrict__ flags) {
+  extern __shared__ int lds_hist[];
+  const int nb = p.nseg + 1;
+  if (kLds) {
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) lds_hist[i] = 0;
+    __syncthreads();
+  }
+  int flag = 0, npol = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    Classified c = classify(theta[i], phi[i], wt[i], p);
+    flag |= c.flag;
+    npol += c.polar;
+    if (kLds) atomicAdd(&lds_hist[c.seg], 1);
+    else atomicAdd(&seg_count[c.seg], 1);
+  }
+  if (flag) atomicOr(&flags[0], flag);
+  if (npol) atomicAdd(&flags[1], npol);
+  if (kLds) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+      int v = lds_hist[i];
+      if (v) atomicAdd(&seg_count[i], v);
+    }
+  }
+}
+
+// Exclusive scan of n ints by one workgroup of 1024 threads:
+// out[i] = sum_{k<i} in[k], out[n] = total; out2 (if given) gets a copy.
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n,
+                                               int* __restrict__ out, int* __restrict__ out2) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = tid * per, e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; ++i) s += in[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = tid ? part[tid - 1] : 0;
+  for (int i = b; i < e; ++i) {
+    out[i] = run;
+    if (out2) out2[i] = run;
+    run += in[i];
+  }
+  if (tid == 1023) {
+    out[n] = part[1023];
+    if (out2) out2[n] = part[1023];
+  }
+}
+
+// Place each point's record at its sorted position.  The order inside a
+// segment is that of the atomics (the sums over a segment do not depend on
+// it beyond rounding).
+__global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restrict__ theta,
+                                               const double* __restrict__ phi,
+                                               const double* __restrict__ wt, PointParams p,
+                                               int* __restrict__ cursor, SortedPoints sp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    Classified c = classify(theta[i], phi[i], wt[i], p);
+    const int pos = atomicAdd(&cursor[c.seg], 1);
+    sp.ere[pos] = c.ere;
+    sp.eim[pos] = c.eim;
+    sp.h0[pos] = c.h0;
+    sp.h1[pos] = c.h1;
+    sp.h2[pos] = c.h2;
+    sp.h3[pos] = c.h3;
+    sp.seg[pos] = c.seg;
+  }
+}
+
+// Number of runs (maximal same-segment stretches) in each chunk of the sorted
+// valid points; one wavefront per chunk.
+__global__ __launch_bounds__(64) void k_count_runs(const int* __restrict__ seg,
+                                                   const int* __restrict__ seg_start, int nseg,
+                                                   int chunk, int nchunks,
+                                                   int* __restrict__ run_count) {
+  const int c = blockIdx.x;
+  if (c >= nchunks) return;
+  const int nvalid = seg_start[nseg];
+  const int b = c * chunk, e = min(nvalid, b + chunk);
+  int cnt = 0;
+  for (int i = b + threadIdx.x; i < e; i += 64) {
+    const bool start = (i == b) || (seg[i] != seg[i - 1]);
+    cnt += __popcll(__ballot(start));
+  }
+  if (threadIdx.x == 0) run_count[c] = (b < e) ? cnt : 0;
+}
+
+int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
+                  int mode, hipStream_t s) {
+  Workspace& ws = pl.ws;
+  PointParams p{pl.Nx, pl.nseg, mode, pl.xmax, pl.dx, pl.acos_xmax, pl.acos_mxmax};
+  DSHT_HIP(hipMemsetAsync(ws.seg_count, 0, sizeof(int) * (pl.nseg + 2), s));
+  DSHT_HIP(hipMemsetAsync(ws.flags, 0, sizeof(int) * 4, s));
+  const int threads = 256;
+  const int64_t want = (Np + threads - 1) / threads;
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
+  const size_t lds = sizeof(int) * (pl.nseg + 1);
+  if (Np > 0) {
+    if (lds <= 64 * 1024) {
+      k_hist<true><<<blocks, threads, lds, s>>>(Np, theta, phi, wt, p, ws.seg_count, ws.flags);
+    } else {
+      k_hist<false><<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_count, ws.flags);
+    }
+    DSHT_LAUNCH_CHECK();
+  }
+  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg + 1, ws.seg_start, ws.seg_cursor);
+  DSHT_LAUNCH_CHECK();
+  if (Np > 0) {
+    k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.sp);
+    DSHT_LAUNCH_CHECK();
+  }
+  const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+  if (nchunks > 0) {
+    k_count_runs<<<nchunks, 64, 0, s>>>(ws.sp.seg, ws.seg_start, pl.nseg, pl.chunk, nchunks,
+                                        ws.run_count);
+    DSHT_LAUNCH_CHECK();
+  }
+  k_scan<<<1, 1024, 0, s>>>(ws.run_count, nchunks, ws.run_base, nullptr);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+}  // namespace dsht

@@ -1,0 +1,159 @@
+// Stage (a): Y_lm(x_j, 0) and dY_lm/dx spline-node tables, built by recurrence
+// in FP64 on the GPU.
+//
+// One thread per (m, node n) column runs the reference's recurrences over l:
+//   P_00 = 1, P_10 = x                                 sht/legendre_py.py:19-20
+//   P_mm = -sqrt(1 - 1/(2m)) sqrt(1-x^2) P_{m-1,m-1}   sht/legendre_py.py:28-30
+//   P_{m+1,m} = sqrt(2m+1) x P_mm                      sht/legendre_py.py:32-34
+//   P_lm = [(2l-1) x P_{l-1,m} - (l+m-1) P_{l-2,m} f2] f1/(l-m)
+//                                                      sht/legendre_py.py:91-100
+//   dP/dx (m = 0) = l/(1-x^2) (P_{l-1} - x P_l)        sht/legendre_py.py:54-56
+//   dP/dx (m > 0) = [(l+m) f P_{l-1,m} - l x P_lm]/(1-x^2)
+//                                                      sht/legendre_py.py:111-117
+//   then both times sqrt((2l+1)/4pi)                   sht/legendre_py.py:77-81
+// with floating-point contraction disabled and the reference's operation
+// order, so with correctly rounded division and sqrt the tables equal the
+// reference's numpy tables bit for bit.
+//
+// The per-column recurrence is inherently sequential in l; the parallelism is
+// over Nl * Nx columns.  Output goes either to the contraction layout
+// (GroupDesc in common.h; FP64 and/or FP32 copies) or, for DirectSHT.Yv/Yd,
+// to the healpy row layout [Nlm][Nx].
+#include "common.h"
+#include "plan.h"
+
+namespace dsht {
+
+template <bool kW64, bool kW32, bool kHP>
+__global__ __launch_bounds__(256) void k_tables(int Nl, int Nx, int Npair, double dx,
+                                                const int64_t* __restrict__ mp_off,
+                                                double* __restrict__ tab64,
+                                                float* __restrict__ tab32,
+                                                double* __restrict__ hpYv,
+                                                double* __restrict__ hpYd) {
+#pragma clang fp contract(off)
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (n >= 2 * Npair || m >= Nl) return;
+  const bool phantom = n >= Nx;  // padding node of an odd Nx: zeros
+  const double x = phantom ? 0.0 : (double)n * dx;
+  const double sx = sqrt(1.0 - x * x);
+  const double omx2 = 1.0 - x * x;
+  const int np = n >> 1, odd = n & 1;
+
+  auto emit = [&](int ell, double P, double D) {
+    const double norm = sqrt((double)(2 * ell + 1) / 4.0 / M_PI);
+    double y = phantom ? 0.0 : P * norm;
+    double yd = phantom ? 0.0 : D * norm;
+    if (kHP) {
+      if (!phantom) {
+        int64_t row = hp_index(ell, m, Nl);
+        hpYv[row * Nx + n] = y;
+        hpYd[row * Nx + n] = yd;
+      }
+    } else {
+      const int k = ell - m, P2 = k & 1, i = k >> 1;
+      const int g = i / kGroupRows, r = i % kGroupRows;
+      const int R = rows_mp(Nl, m, P2);
+      const int nrows = min(kGroupRows, R - g * kGroupRows);
+      const int64_t idx4 = mp_off[2 * m + P2] + (int64_t)g * kGroupRows * Npair + (int64_t)np * nrows + r;
+      if (kW64) {
+        tab64[idx4 * 4 + odd] = y;
+        tab64[idx4 * 4 + 2 + odd] = yd;
+      }
+      if (kW32) {
+        tab32[idx4 * 4 + odd] = (float)y;
+        tab32[idx4 * 4 + 2 + odd] = (float)yd;
+      }
+    }
+  };
+
+  // P_mm by the chain over m' = 1..m (each column recomputes its own diagonal)
+  double pmm = 1.0;
+  for (int k = 1; k <= m; ++k) pmm = (-sqrt(1.0 - 1.0 / (double)(2 * k)) * sx) * pmm;
+
+  double p2 = 0.0, p1 = 0.0;  // P_{l-2,m}, P_{l-1,m}
+  for (int ell = m; ell < Nl; ++ell) {
+    double p;
+    if (ell == m) {
+      p = pmm;
+    } else if (ell == m + 1) {
+      p = (m == 0) ? x : (sqrt((double)(2 * m) + 1.0) * x) * pmm;
+    } else {
+      const double f1 = sqrt((double)(ell - m) * 1.0 / (double)(ell + m));
+      const double f2 = sqrt(((double)(ell - m) - 1.0) / ((double)(ell + m) - 1.0));
+      const double a = ((double)(2 * ell - 1) * x) * p1;
+      const double b = ((double)(ell + m - 1) * p2) * f2;
+      p = (a - b) * (f1 / (double)(ell - m));
+    }
+    double d;
+    if (m == 0) {
+      if (ell == 0) d = 0.0;
+      else if (ell == 1) d = 1.0;
+      else d = ((double)ell / (1.0 - x * x)) * (p1 - x * p);
+    } else {
+      const double f = sqrt((double)(ell - m) / (double)(ell + m));
+      const double lhs = ((double)(ell + m) * f) * (ell == m ? 0.0 : p1);
+      d = (lhs - ((double)ell * x) * p) / omx2;
+    }
+    emit(ell, p, d);
+    p2 = p1;
+    p1 = p;
+  }
+}
+
+// Copy caller-provided healpy-row tables (csht's make_table layout) into the
+// contraction layout (FP64).  One block per healpy row.
+__global__ __launch_bounds__(256) void k_relayout(int Nl, int Nx, int Npair,
+                                                  const int64_t* __restrict__ mp_off,
+                                                  const double* __restrict__ hpYv,
+                                                  const double* __restrict__ hpYd,
+                                                  double* __restrict__ tab64) {
+  const int64_t row = blockIdx.x;
+  // invert row = m(2Nl-1-m)/2 + l (healpy index), as sht/shared_utils.py:93-98
+  const double b = 2.0 * Nl - 1.0;
+  int m = (int)ceil((b - sqrt(b * b - 8.0 * ((double)row - (Nl - 1)))) / 2.0);
+  if (m < 0) m = 0;
+  while (m > 0 && hp_index(m, m, Nl) > row) --m;
+  while (m + 1 < Nl && hp_index(m + 1, m + 1, Nl) <= row) ++m;
+  const int ell = (int)(row - hp_index(0, m, Nl));
+  const int k = ell - m, P = k & 1, i = k >> 1;
+  const int g = i / kGroupRows, r = i % kGroupRows;
+  const int nrows = min(kGroupRows, rows_mp(Nl, m, P) - g * kGroupRows);
+  const int64_t base = mp_off[2 * m + P] + (int64_t)g * kGroupRows * Npair + r;
+  for (int n = threadIdx.x; n < 2 * Npair; n += blockDim.x) {
+    const int64_t idx4 = base + (int64_t)(n >> 1) * nrows;
+    const double y = n < Nx ? hpYv[row * Nx + n] : 0.0;
+    const double yd = n < Nx ? hpYd[row * Nx + n] : 0.0;
+    tab64[idx4 * 4 + (n & 1)] = y;
+    tab64[idx4 * 4 + 2 + (n & 1)] = yd;
+  }
+}
+
+int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipStream_t s) {
+  k_relayout<<<(unsigned)pl.Nlm, 256, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.mp_off, hpYv, hpYd, pl.tab64);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_tables(const Plan& pl, bool w64, bool w32, double* hpYv, double* hpYd, hipStream_t s) {
+  dim3 block(256);
+  dim3 grid((2 * pl.Npair + 255) / 256, pl.Nl);
+  if (hpYv) {
+    k_tables<false, false, true><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
+                                                         nullptr, nullptr, hpYv, hpYd);
+  } else if (w64 && w32) {
+    k_tables<true, true, false><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
+                                                        pl.tab64, pl.tab32, nullptr, nullptr);
+  } else if (w64) {
+    k_tables<true, false, false><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
+                                                         pl.tab64, nullptr, nullptr, nullptr);
+  } else {
+    k_tables<false, true, false><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
+                                                         nullptr, pl.tab32, nullptr, nullptr);
+  }
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+}  // namespace dsht

@@ -1,0 +1,121 @@
+/*
+ * dsht.h -- C ABI of the MI355X direct spherical harmonic transform
+ * (libdsht.so, built from directsht_amd/csrc).
+ *
+ * Two families of entry points:
+ *
+ *  1. The reference's own native C ABI, kept bit-for-bit in signature and
+ *     meaning so a ctypes caller of the reference's csht/sht.py can switch
+ *     libraries without code changes (host pointers, caller-owned arrays):
+ *       make_table    replaces csht/sht_helper.c:18   (make_table)
+ *       do_transform  replaces csht/sht_helper.c:176  (do_transform)
+ *
+ *  2. A device-resident plan API behind the Python DirectSHT class, which
+ *     replaces sht/sht.py:30-284 (DirectSHT.__init__, __call__, basic_sht):
+ *     tables stay in HBM between calls, inputs and outputs are device
+ *     pointers, work is queued on the caller's HIP stream.
+ *
+ * All functions return 0 on success and a negative DSHT_E* code on failure;
+ * dsht_last_error() returns a message for the calling thread's last failure.
+ */
+#ifndef DSHT_H
+#define DSHT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define DSHT_OK 0
+#define DSHT_EINVAL (-1)   /* bad argument                                 */
+#define DSHT_EHIP (-2)     /* HIP runtime error                            */
+#define DSHT_ENOMEM (-3)   /* device allocation failed                     */
+
+/* ---- status bits written by a transform (dsht_transform *status) ------- */
+/* Some point kept by the equatorial branch has theta outside
+ * (arccos(xmax), arccos(-xmax)) or theta < 0: the reference raises
+ * AssertionError (sht/sht.py:171-172). */
+#define DSHT_STATUS_EQU_RANGE 1
+/* A polar point is still polar after the 90-degree rotation (possible only
+ * when xmax < sin(pi/4)): the reference catches the AssertionError and
+ * returns zeros for the whole transform (sht/sht.py:141-149).  The output
+ * alm is then all zeros. */
+#define DSHT_STATUS_POLAR_RANGE 2
+
+/* ---- precision of the table contraction (tables are always built in FP64) */
+#define DSHT_FP64 0
+#define DSHT_FP32 1
+
+/* ---- transform modes ---------------------------------------------------- */
+/* DirectSHT.__call__: polar/equatorial split, rotation, parity (sht/sht.py:112-150) */
+#define DSHT_MODE_CALL 0
+/* DirectSHT.basic_sht: no split, every point must be equatorial (sht/sht.py:153-283) */
+#define DSHT_MODE_BASIC 1
+
+/* ========================================================================= */
+/* 1. Reference C ABI (csht)                                                 */
+/* ========================================================================= */
+
+/* Fill caller-owned host arrays Yv, Yd (each Nl(Nl+1)/2 * Nx doubles, healpy
+ * row order, row length Nx) with Y_lm(x_j, 0) and dY_lm/dx at
+ * x_j = j*xmax/(Nx-1).  Built by the FP64 recurrence on the GPU.
+ * Replaces csht/sht_helper.c:18-116.  Returns 0. */
+int make_table(int Nl, int Nx, double xmax, double Yv[], double Yd[]);
+
+/* Direct SHT of Np points with cos(theta) = cost[i] (all of one sign,
+ * |cost| < xmax), phi[i], weight wt[i], using the caller's tables Yv, Yd (as
+ * filled by make_table).  Writes carr[idx] = sum Y cos(m phi) and
+ * sarr[idx] = sum Y sin(m phi) for every healpy index idx (alm = carr -
+ * i*sarr); the caller applies (-1)^(l-m) for cost < 0, as csht/sht.py:101-121
+ * does.  The reference requires cost sorted by |cost|; this implementation
+ * sorts on the device and accepts any order.  The tables are uploaded on every
+ * call (the plan API keeps them resident).  Replaces csht/sht_helper.c:176-286.
+ * Returns 0, or 1 if the device work failed (the reference returns 1 on
+ * allocation failure). */
+int do_transform(int Nl, int Nx, double xmax, double Yv[], double Yd[], int Np,
+                 double cost[], double phi[], double wt[], double carr[], double sarr[]);
+
+/* ========================================================================= */
+/* 2. Device-resident plan API                                               */
+/* ========================================================================= */
+
+typedef struct dsht_plan dsht_plan;
+
+/* Build the FP64 Ylm/dYlm spline tables of DirectSHT(Nl, Nx, xmax) on the
+ * current device (sht/sht.py:32-56) in the layout the contraction streams,
+ * keeping an FP64 copy (precision DSHT_FP64) or an FP32 copy (DSHT_FP32).
+ * Requires Nl >= 2, Nx >= 2, 0 < xmax < 1. */
+int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** plan);
+int dsht_plan_destroy(dsht_plan* plan);
+
+/* Bytes of HBM the plan holds (tables + current workspace). */
+int dsht_plan_footprint(const dsht_plan* plan, int64_t* bytes);
+
+/* HBM a plan would need for its tables and for a transform of Np points
+ * (what a server admits against). */
+int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table_bytes,
+                       int64_t* work_bytes);
+
+/* Copy the tables to caller-owned host arrays in healpy row order (the
+ * DirectSHT.Yv / .Yd attributes, sht/sht.py:56), FP64.  Either may be NULL. */
+int dsht_plan_get_tables(dsht_plan* plan, double* Yv, double* Yd);
+
+/* Transform Np points.  theta, phi, wt: device arrays of Np doubles.
+ * alm: device array of Nl(Nl+1)/2 complex doubles (interleaved re, im),
+ * healpy order, overwritten.  status: device int written with DSHT_STATUS_*
+ * bits (may be NULL).  mode: DSHT_MODE_CALL or DSHT_MODE_BASIC.  Work is
+ * enqueued on `stream` (a hipStream_t; 0 = default stream) and the call
+ * returns without synchronizing.  Concurrent transforms on one plan must be
+ * serialized by the caller (they share the plan's workspace). */
+int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
+                   const double* wt, double* alm, int* status, int mode, void* stream);
+
+/* Message describing the calling thread's last error ("" if none). */
+const char* dsht_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSHT_H */
