@@ -61,11 +61,11 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   const int nseg = num_segments(Nx);
   int64_t b = 0;
-  b += align256(z.points * (6 * sizeof(double) + sizeof(int)));
+  b += align256(z.points * sizeof(PointRec));
   b += 2 * align256((z.chunks + 1) * sizeof(int));
   b += align256(z.runs * sizeof(int));
-  b += align256(z.runs * (int64_t)Nl * 8 * sizeof(double));
-  b += 3 * align256((nseg + 2) * sizeof(int));
+  b += align256(z.runs * (int64_t)Nl * 8 * (precision == DSHT_FP32 ? 4 : 8));
+  b += 4 * align256((nseg + 2) * sizeof(int));
   b += align256((int64_t)Nl * 2 * Npair * 16 * (precision == DSHT_FP32 ? 4 : 8));
   b += 2 * align256(Nlm * 2 * sizeof(double));
   b += 256;
@@ -73,12 +73,9 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
 }
 
 static void free_workspace(Workspace& ws) {
-  SortedPoints& sp = ws.sp;
-  for (void* p : {(void*)sp.ere, (void*)sp.eim, (void*)sp.h0, (void*)sp.h1, (void*)sp.h2,
-                  (void*)sp.h3, (void*)sp.seg, (void*)ws.run_count, (void*)ws.run_base,
-                  (void*)ws.run_seg, (void*)ws.part, (void*)ws.seg_count, (void*)ws.seg_start,
-                  (void*)ws.seg_cursor, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol,
-                  (void*)ws.flags})
+  for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
+                  ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
+                  (void*)ws.ne_rank, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
 }
@@ -106,22 +103,15 @@ static int ensure_workspace(Plan& pl, int64_t Np) {
   WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
                std::max(z.runs, ws.cap_runs)};
   free_workspace(ws);
-  SortedPoints& sp = ws.sp;
-  const int64_t P = want.points;
-  DSHT_ALLOC(sp.ere, P * sizeof(double));
-  DSHT_ALLOC(sp.eim, P * sizeof(double));
-  DSHT_ALLOC(sp.h0, P * sizeof(double));
-  DSHT_ALLOC(sp.h1, P * sizeof(double));
-  DSHT_ALLOC(sp.h2, P * sizeof(double));
-  DSHT_ALLOC(sp.h3, P * sizeof(double));
-  DSHT_ALLOC(sp.seg, P * sizeof(int));
+  DSHT_ALLOC(ws.rec, want.points * sizeof(PointRec));
   DSHT_ALLOC(ws.run_count, (want.chunks + 1) * sizeof(int));
   DSHT_ALLOC(ws.run_base, (want.chunks + 1) * sizeof(int));
   DSHT_ALLOC(ws.run_seg, want.runs * sizeof(int));
-  DSHT_ALLOC(ws.part, want.runs * (int64_t)pl.Nl * 8 * sizeof(double));
+  DSHT_ALLOC(ws.part, want.runs * (int64_t)pl.Nl * 8 * (pl.precision == DSHT_FP32 ? 4 : 8));
   DSHT_ALLOC(ws.seg_count, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.ne_rank, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8));
   DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sizeof(double));
   DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sizeof(double));
@@ -133,7 +123,7 @@ static int ensure_workspace(Plan& pl, int64_t Np) {
 }
 
 // d^l_{l,m}(pi/2) = (-1)^{l-m} sqrt(C(2l, l+m)) / 2^l as mantissa * 2^exp, and
-// the recursion coefficients A(m'), 1/B(m'), for every l < Nl.
+// the recursion coefficients A(m')/B(m') and 1/B(m'), for every l < Nl.
 static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int>& ex,
                                std::vector<double>& A, std::vector<double>& iB) {
   const int64_t n = (int64_t)Nl * (Nl + 1) / 2;
@@ -159,8 +149,9 @@ static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int
       }
     }
     for (int mp = 1; mp <= l; ++mp) {
-      A[tri + mp] = std::sqrt((double)(l - mp) * (double)(l + mp + 1));
-      iB[tri + mp] = 1.0 / std::sqrt((double)(l + mp) * (double)(l - mp + 1));
+      const double b = std::sqrt((double)(l + mp) * (double)(l - mp + 1));
+      A[tri + mp] = std::sqrt((double)(l - mp) * (double)(l + mp + 1)) / b;
+      iB[tri + mp] = 1.0 / b;
     }
   }
 }
@@ -206,11 +197,11 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   rotation_constants(Nl, top, ex, A, iB);
   DSHT_ALLOC(pl.rot_top, top.size() * sizeof(double));
   DSHT_ALLOC(pl.rot_exp, ex.size() * sizeof(int));
-  DSHT_ALLOC(pl.rot_A, A.size() * sizeof(double));
+  DSHT_ALLOC(pl.rot_AB, A.size() * sizeof(double));
   DSHT_ALLOC(pl.rot_iB, iB.size() * sizeof(double));
   DSHT_HIP(hipMemcpy(pl.rot_top, top.data(), top.size() * sizeof(double), hipMemcpyHostToDevice));
   DSHT_HIP(hipMemcpy(pl.rot_exp, ex.data(), ex.size() * sizeof(int), hipMemcpyHostToDevice));
-  DSHT_HIP(hipMemcpy(pl.rot_A, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
+  DSHT_HIP(hipMemcpy(pl.rot_AB, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
   DSHT_HIP(hipMemcpy(pl.rot_iB, iB.data(), iB.size() * sizeof(double), hipMemcpyHostToDevice));
   pl.table_bytes = ws.bytes;
   ws.bytes = 0;
@@ -224,8 +215,9 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
 
 static void plan_free(Plan& pl) {
   free_workspace(pl.ws);
+  for (hipEvent_t e : pl.events) (void)hipEventDestroy(e);
   for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64, (void*)pl.tab32,
-                  (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_A, (void*)pl.rot_iB})
+                  (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB, (void*)pl.rot_iB})
     if (p) (void)hipFree(p);
   pl = Plan{};
 }
@@ -240,6 +232,20 @@ struct DeviceGuard {
     if (prev >= 0) (void)hipSetDevice(prev);
   }
 };
+
+static int harvest(Plan& pl, int set) {
+  if (!pl.pending[set]) return DSHT_OK;
+  hipEvent_t* ev = &pl.events[(size_t)set * (DSHT_NSTAGES + 1)];
+  DSHT_HIP(hipEventSynchronize(ev[DSHT_NSTAGES]));
+  for (int k = 0; k < DSHT_NSTAGES; ++k) {
+    float ms = 0.f;
+    DSHT_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+    pl.stage_ms[k] += ms;
+  }
+  pl.timed += 1;
+  pl.pending[set] = 0;
+  return DSHT_OK;
+}
 
 static int run_transform(Plan& pl, int64_t Np, const double* theta, const double* phi,
                          const double* wt, double* alm, int* status, int mode, hipStream_t s) {
@@ -258,11 +264,29 @@ static int run_transform(Plan& pl, int64_t Np, const double* theta, const double
   int rc = ensure_workspace(pl, Np);
   if (rc) return rc;
   pl.ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+  hipEvent_t* ev = nullptr;
+  if (pl.timing) {
+    const int set = pl.next_set;
+    pl.next_set = (pl.next_set + 1) % Plan::kTimingSets;
+    if ((rc = harvest(pl, set))) return rc;
+    ev = &pl.events[(size_t)set * (DSHT_NSTAGES + 1)];
+    pl.pending[set] = 1;
+  }
+  auto mark = [&](int k) -> int {
+    if (ev) DSHT_HIP(hipEventRecord(ev[k], s));
+    return DSHT_OK;
+  };
+  if ((rc = mark(0))) return rc;
   if ((rc = launch_points(pl, Np, theta, phi, wt, mode, s))) return rc;
+  if ((rc = mark(1))) return rc;
   if ((rc = launch_scatter(pl, Np, s))) return rc;
+  if ((rc = mark(2))) return rc;
   if ((rc = launch_fold(pl, s))) return rc;
+  if ((rc = mark(3))) return rc;
   if ((rc = launch_contract(pl, s))) return rc;
+  if ((rc = mark(4))) return rc;
   if ((rc = launch_rotate_combine(pl, alm, status, mode, s))) return rc;
+  if ((rc = mark(5))) return rc;
   return DSHT_OK;
 }
 
@@ -277,6 +301,44 @@ using namespace dsht;
 extern "C" {
 
 const char* dsht_last_error(void) { return g_err.c_str(); }
+
+int dsht_plan_set_timing(dsht_plan* p, int enable) {
+  if (!p) {
+    set_error("dsht_plan_set_timing: null plan");
+    return DSHT_EINVAL;
+  }
+  Plan& pl = p->pl;
+  DeviceGuard g(pl.device);
+  if (enable && pl.events.empty()) {
+    pl.events.resize((size_t)Plan::kTimingSets * (DSHT_NSTAGES + 1));
+    for (hipEvent_t& e : pl.events) DSHT_HIP(hipEventCreate(&e));
+    pl.pending.assign(Plan::kTimingSets, 0);
+  }
+  for (int k = 0; k < (int)pl.pending.size(); ++k) {
+    if (pl.pending[k]) DSHT_HIP(hipEventSynchronize(pl.events[(size_t)k * (DSHT_NSTAGES + 1) + DSHT_NSTAGES]));
+    pl.pending[k] = 0;
+  }
+  for (double& v : pl.stage_ms) v = 0.0;
+  pl.timed = 0;
+  pl.timing = enable != 0;
+  return DSHT_OK;
+}
+
+int dsht_plan_get_timing(dsht_plan* p, double stage_ms[DSHT_NSTAGES], int64_t* ntransforms) {
+  if (!p || !stage_ms) {
+    set_error("dsht_plan_get_timing: null argument");
+    return DSHT_EINVAL;
+  }
+  Plan& pl = p->pl;
+  DeviceGuard g(pl.device);
+  for (int k = 0; k < (int)pl.pending.size(); ++k) {
+    int rc = harvest(pl, k);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < DSHT_NSTAGES; ++k) stage_ms[k] = pl.stage_ms[k];
+  if (ntransforms) *ntransforms = pl.timed;
+  return DSHT_OK;
+}
 
 int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** out) {
   if (!out) {

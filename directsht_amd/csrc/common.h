@@ -76,15 +76,14 @@ __host__ __device__ inline int64_t hp_index(int ell, int m, int Nl) {
 // never scattered).
 __host__ __device__ inline int num_segments(int Nx) { return 4 * (Nx - 1); }
 
-// Per-point record after sorting (structure of arrays).
-struct SortedPoints {
-  double* ere;   // cos(phi)
-  double* eim;   // -sin(phi)      (e1 = e^{-i phi})
-  double* h0;    // w * h0(t)
-  double* h1;    // w * h1(t)      (dx applied at the fold)
-  double* h2;    // w * h2(t)
-  double* h3;    // w * h3(t)
-  int* seg;      // segment id
+// Per-point record after sorting: one 64-byte line per point, written once by
+// the placement pass and read by the scatter.
+struct alignas(64) PointRec {
+  double ere, eim;         // e1 = e^{-i phi} = (cos phi, -sin phi)
+  double h0, h1, h2, h3;   // w * Hermite basis at t (dx applied at the fold)
+  int seg;                 // segment id
+  int pad;
 };
+static_assert(sizeof(PointRec) == 64, "PointRec must be one 64-byte line");
 
 }  // namespace dsht

@@ -31,8 +31,8 @@ __device__ inline int lower_bound_int(const int* a, int n, int v) {
   return lo;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part,
+template <typename TP, typename T>
+__global__ __launch_bounds__(256) void k_fold(const TP* __restrict__ part,
                                               const int* __restrict__ run_seg,
                                               const int* __restrict__ run_total, int Nl, int Nx,
                                               int Npair, double dx, T* __restrict__ rhs) {
@@ -57,13 +57,13 @@ __global__ __launch_bounds__(256) void k_fold(const double* __restrict__ part,
     double c[2][2] = {{0, 0}, {0, 0}}, d[2][2] = {{0, 0}, {0, 0}};  // [hemisphere][re/im]
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int h = q >> 1, hi_half = q & 1;  // hi_half: v2,v3 of bin n-1
+      const int h = q >> 1, half = q & 1;  // half 1: v2,v3 of bin n-1
       for (int r = rr[q][0]; r < rr[q][1]; ++r) {
-        const double* v = part + ((int64_t)r * Nl + m) * 8 + 4 * hi_half;
-        c[h][0] += v[0];
-        c[h][1] += v[1];
-        d[h][0] += v[2];
-        d[h][1] += v[3];
+        const TP* v = part + (((int64_t)r * 2 + half) * Nl + m) * 4;
+        c[h][0] += (double)v[0];
+        c[h][1] += (double)v[1];
+        d[h][0] += (double)v[2];
+        d[h][1] += (double)v[3];
       }
     }
 #pragma unroll
@@ -82,11 +82,11 @@ int launch_fold(Plan& pl, hipStream_t s) {
   dim3 grid(2 * pl.Npair, 2);
   const int* run_total = pl.ws.run_base + pl.ws.cur_chunks;  // scan total = number of runs
   if (pl.precision == DSHT_FP32) {
-    k_fold<float><<<grid, 256, 0, s>>>(pl.ws.part, pl.ws.run_seg, run_total, pl.Nl, pl.Nx, pl.Npair,
-                                       pl.dx, (float*)pl.ws.rhs);
+    k_fold<float, float><<<grid, 256, 0, s>>>((const float*)pl.ws.part, pl.ws.run_seg, run_total,
+                                              pl.Nl, pl.Nx, pl.Npair, pl.dx, (float*)pl.ws.rhs);
   } else {
-    k_fold<double><<<grid, 256, 0, s>>>(pl.ws.part, pl.ws.run_seg, run_total, pl.Nl, pl.Nx, pl.Npair,
-                                        pl.dx, (double*)pl.ws.rhs);
+    k_fold<double, double><<<grid, 256, 0, s>>>((const double*)pl.ws.part, pl.ws.run_seg, run_total,
+                                                pl.Nl, pl.Nx, pl.Npair, pl.dx, (double*)pl.ws.rhs);
   }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

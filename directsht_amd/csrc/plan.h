@@ -12,14 +12,16 @@ struct Workspace {
   int64_t cap_chunks = 0;  // capacity of per-chunk arrays
   int64_t cap_runs = 0;    // capacity of the run partials
   int cur_chunks = 0;      // scatter chunks of the transform being enqueued
-  SortedPoints sp{};
+  PointRec* rec = nullptr;    // [cap_points] sorted point records
   int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
   int* run_base = nullptr;    // [cap_chunks + 1]: exclusive scan of run_count
   int* run_seg = nullptr;     // [cap_runs]: segment of each run
-  double* part = nullptr;     // [cap_runs][Nl][8]: per-run v0..v3 (complex)
+  void* part = nullptr;       // [cap_runs][2][Nl][4]: per-run (v0,v1 | v2,v3), complex;
+                              // double (FP64 scatter) or float (FP32 scatter)
   int* seg_count = nullptr;   // [nseg + 2]
   int* seg_start = nullptr;   // [nseg + 2]
   int* seg_cursor = nullptr;  // [nseg + 2]
+  int* ne_rank = nullptr;     // [nseg + 1]: non-empty segments before s
   void* rhs = nullptr;        // [Nl][2][Npair][16] float or double
   double* a_equ = nullptr;    // [Nlm][2]
   double* a_pol = nullptr;    // [Nlm][2]
@@ -44,12 +46,21 @@ struct Plan {
   float* tab32 = nullptr;
   int64_t table_bytes = 0;
   // polar rotation constants, per l at offset l(l+1)/2: d^l_{l,m}(pi/2) as
-  // mantissa * 2^exp for m = 0..l, A(m') and 1/B(m') of the recursion
+  // mantissa * 2^exp for m = 0..l, A(m')/B(m') and 1/B(m') of the recursion
   double* rot_top = nullptr;
   int* rot_exp = nullptr;
-  double* rot_A = nullptr;
+  double* rot_AB = nullptr;
   double* rot_iB = nullptr;
   Workspace ws;
+  // stage timing (dsht_plan_set_timing): a pool of event sets reused round
+  // robin; a set is harvested before reuse and by dsht_plan_get_timing
+  static constexpr int kTimingSets = 64;
+  bool timing = false;
+  std::vector<hipEvent_t> events;     // kTimingSets * (DSHT_NSTAGES + 1)
+  std::vector<char> pending;          // per set: recorded, not yet harvested
+  int next_set = 0;
+  double stage_ms[DSHT_NSTAGES] = {0, 0, 0, 0, 0};
+  int64_t timed = 0;
 };
 
 // Host-side layout computation shared by plan creation and estimates.

@@ -136,15 +136,16 @@ __global__ __launch_bounds__(256) void k_hist(int64_t Np, const double* __restri
 }
 
 // Exclusive scan of n ints by one workgroup of 1024 threads:
-// out[i] = sum_{k<i} in[k], out[n] = total; out2 (if given) gets a copy.
-__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n,
+// out[i] = sum_{k<i} v(in[k]), out[n] = total, with v(x) = x, or v(x) = (x > 0)
+// when `indicator` is set; out2 (if given) gets a copy.
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n, int indicator,
                                                int* __restrict__ out, int* __restrict__ out2) {
   __shared__ int part[1024];
   const int tid = threadIdx.x;
   const int per = (n + 1023) / 1024;
   const int b = tid * per, e = min(n, b + per);
   int s = 0;
-  for (int i = b; i < e; ++i) s += in[i];
+  for (int i = b; i < e; ++i) s += indicator ? (in[i] > 0) : in[i];
   part[tid] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n
   for (int i = b; i < e; ++i) {
     out[i] = run;
     if (out2) out2[i] = run;
-    run += in[i];
+    run += indicator ? (in[i] > 0) : in[i];
   }
   if (tid == 1023) {
     out[n] = part[1023];
@@ -165,43 +166,54 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n
   }
 }
 
-// Place each point's record at its sorted position.  The order inside a
-// segment is that of the atomics (the sums over a segment do not depend on
+// Place each point's 64-byte record at its sorted position.  The order inside
+// a segment is that of the atomics (the sums over a segment do not depend on
 // it beyond rounding).
 __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restrict__ theta,
                                                const double* __restrict__ phi,
                                                const double* __restrict__ wt, PointParams p,
-                                               int* __restrict__ cursor, SortedPoints sp) {
+                                               int* __restrict__ cursor, PointRec* __restrict__ rec) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
     Classified c = classify(theta[i], phi[i], wt[i], p);
     const int pos = atomicAdd(&cursor[c.seg], 1);
-    sp.ere[pos] = c.ere;
-    sp.eim[pos] = c.eim;
-    sp.h0[pos] = c.h0;
-    sp.h1[pos] = c.h1;
-    sp.h2[pos] = c.h2;
-    sp.h3[pos] = c.h3;
-    sp.seg[pos] = c.seg;
+    double2* d = reinterpret_cast<double2*>(rec + pos);
+    d[0] = make_double2(c.ere, c.eim);
+    d[1] = make_double2(c.h0, c.h1);
+    d[2] = make_double2(c.h2, c.h3);
+    reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, 0);
   }
 }
 
+// Segment containing sorted position i: the last s with seg_start[s] <= i
+// (empty segments share their start with the next one and are skipped).
+__device__ inline int seg_of(const int* seg_start, int nseg, int i) {
+  int lo = 0, hi = nseg;  // answer in [0, nseg-1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (seg_start[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Number of runs (maximal same-segment stretches) in each chunk of the sorted
-// valid points; one wavefront per chunk.
-__global__ __launch_bounds__(64) void k_count_runs(const int* __restrict__ seg,
-                                                   const int* __restrict__ seg_start, int nseg,
-                                                   int chunk, int nchunks,
-                                                   int* __restrict__ run_count) {
-  const int c = blockIdx.x;
+// valid points: the non-empty segments between the chunk's first and last
+// point, found from the segment offsets alone.
+__global__ __launch_bounds__(256) void k_count_runs(const int* __restrict__ seg_start,
+                                                    const int* __restrict__ ne_rank, int nseg,
+                                                    int chunk, int nchunks,
+                                                    int* __restrict__ run_count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nchunks) return;
   const int nvalid = seg_start[nseg];
   const int b = c * chunk, e = min(nvalid, b + chunk);
-  int cnt = 0;
-  for (int i = b + threadIdx.x; i < e; i += 64) {
-    const bool start = (i == b) || (seg[i] != seg[i - 1]);
-    cnt += __popcll(__ballot(start));
+  if (b >= e) {
+    run_count[c] = 0;
+    return;
   }
-  if (threadIdx.x == 0) run_count[c] = (b < e) ? cnt : 0;
+  const int sb = seg_of(seg_start, nseg, b), se = seg_of(seg_start, nseg, e - 1);
+  run_count[c] = ne_rank[se] - ne_rank[sb] + 1;
 }
 
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
@@ -222,19 +234,21 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
     }
     DSHT_LAUNCH_CHECK();
   }
-  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg + 1, ws.seg_start, ws.seg_cursor);
+  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg + 1, 0, ws.seg_start, ws.seg_cursor);
+  DSHT_LAUNCH_CHECK();
+  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg, 1, ws.ne_rank, nullptr);
   DSHT_LAUNCH_CHECK();
   if (Np > 0) {
-    k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.sp);
+    k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec);
     DSHT_LAUNCH_CHECK();
   }
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks > 0) {
-    k_count_runs<<<nchunks, 64, 0, s>>>(ws.sp.seg, ws.seg_start, pl.nseg, pl.chunk, nchunks,
-                                        ws.run_count);
+    k_count_runs<<<(nchunks + 255) / 256, 256, 0, s>>>(ws.seg_start, ws.ne_rank, pl.nseg, pl.chunk,
+                                                      nchunks, ws.run_count);
     DSHT_LAUNCH_CHECK();
   }
-  k_scan<<<1, 1024, 0, s>>>(ws.run_count, nchunks, ws.run_base, nullptr);
+  k_scan<<<1, 1024, 0, s>>>(ws.run_count, nchunks, 0, ws.run_base, nullptr);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

@@ -15,7 +15,7 @@
 // from d_{l,m} = (-1)^{l-m} sqrt(C(2l, l+m)) / 2^l, which always runs in the
 // direction in which the wanted solution grows.  The column carries a
 // power-of-two exponent so starting values below the FP64 range (l > 1022)
-// survive.  The starting values and A, 1/B depend only on l, so the plan
+// survive.  The starting values and A/B, 1/B depend only on l, so the plan
 // precomputes them once.
 //
 // One workgroup per l, one thread per m; FP64 throughout.  If a polar point
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
                                                         int* __restrict__ status,
                                                         const double* __restrict__ rot_top,
                                                         const int* __restrict__ rot_exp,
-                                                        const double* __restrict__ rot_A,
+                                                        const double* __restrict__ rot_AB,
                                                         const double* __restrict__ rot_iB) {
   const int ell = blockIdx.x;
   const int fl = flags[0];
@@ -59,41 +59,60 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
     return;
   }
   extern __shared__ double lds[];
-  double* sre = lds;                 // Re a'_{l m'}
-  double* sim = lds + (ell + 1);     // Im a'_{l m'}
-  double* sA = lds + 2 * (ell + 1);  // A(m')
-  double* sB = lds + 3 * (ell + 1);  // 1/B(m')
+  double* sre = lds;                  // Re a'_{l m'}
+  double* sim = lds + (ell + 1);      // Im a'_{l m'}
+  double* sAB = lds + 2 * (ell + 1);  // A(m')/B(m')
+  double* sB = lds + 3 * (ell + 1);   // 1/B(m')
   const int64_t tri = (int64_t)ell * (ell + 1) / 2;
   for (int mp = threadIdx.x; mp <= ell; mp += blockDim.x) {
     const int64_t idx = hp_index(ell, mp, Nl);
     sre[mp] = a_pol[2 * idx];
     sim[mp] = a_pol[2 * idx + 1];
-    sA[mp] = rot_A[tri + mp];
+    sAB[mp] = rot_AB[tri + mp];
     sB[mp] = rot_iB[tri + mp];
   }
   __syncthreads();
   for (int m = threadIdx.x; m <= ell; m += blockDim.x) {
+    // Step s = l - m' contributes d_{m',m} * Re a'_{m'} when l + m + m' is
+    // even and d_{m',m} * Im a'_{m'} when it is odd, i.e. by the parity of
+    // m + s: even steps use xe, odd steps xo (no divergent branch).
+    const double* xe = (m & 1) ? sim : sre;
+    const double* xo = (m & 1) ? sre : sim;
+    const double twom = 2.0 * m;
     double cur = rot_top[tri + m], nxt = 0.0;
     int E = rot_exp[tri + m];
-    const double twom = 2.0 * m;
-    double sr = 0.0, si = 0.0;
-    // parity of l + m + m' for m' = l
-    bool even = ((ell + m + ell) & 1) == 0;
-    for (int mp = ell; mp >= 1; --mp) {
-      const double v = ldexp(cur, E);  // d_{mp, m}
-      if (even) sr = fma(v, sre[mp], sr);
-      else si = fma(v, sim[mp], si);
-      const double nw = (twom * cur - sA[mp] * nxt) * sB[mp];  // d_{mp-1, m}
-      nxt = cur;
-      cur = nw;
-      if (fabs(cur) > kScaleHi) {
+    double accE = 0.0, accO = 0.0;
+    int mp = ell;
+    // d_{mp-1} = d_mp * (2m/B) - d_{mp+1} * (A/B): one dependent FMA per step
+    while (mp >= 2) {
+      const int pairs = min(4, mp >> 1);
+      for (int q = 0; q < pairs; ++q) {
+        accE = fma(ldexp(cur, E), xe[mp], accE);
+        double nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
+        nxt = cur;
+        cur = nw;
+        --mp;
+        accO = fma(ldexp(cur, E), xo[mp], accO);
+        nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
+        nxt = cur;
+        cur = nw;
+        --mp;
+      }
+      // at most 8 steps since the last check: growth per step is below 3l,
+      // so values stay below 2^400 * (3l)^8 < 2^1023
+      if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
         cur *= kScaleLo;
         nxt *= kScaleLo;
         E += 400;
       }
-      even = !even;
     }
-    const double d0 = ldexp(cur, E);
+    if (mp == 1) {
+      accE = fma(ldexp(cur, E), xe[1], accE);
+      cur = fma(cur, twom * sB[1], -nxt * sAB[1]);
+    }
+    const double d0 = ldexp(cur, E);  // d_{0,m}
+    const double sr = (m & 1) ? accO : accE;
+    const double si = (m & 1) ? accE : accO;
     const int64_t idx = hp_index(ell, m, Nl);
     alm[2 * idx] = a_equ[2 * idx] + fma(d0, sre[0], 2.0 * sr);
     alm[2 * idx + 1] = a_equ[2 * idx + 1] + fma(d0, sim[0], 2.0 * si);
@@ -104,7 +123,7 @@ int launch_rotate_combine(Plan& pl, double* alm, int* status, int mode, hipStrea
   (void)mode;
   const size_t lds = sizeof(double) * 4 * (size_t)pl.Nl;
   k_rotate_combine<<<pl.Nl, 256, lds, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, pl.ws.flags, status,
-                                           pl.rot_top, pl.rot_exp, pl.rot_A, pl.rot_iB);
+                                           pl.rot_top, pl.rot_exp, pl.rot_AB, pl.rot_iB);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

@@ -117,6 +117,20 @@ class DirectSHT:
             cl[ell] /= 2 * ell + 1.0
         return cl
 
+    # ---- stage timing (HIP events on the transform's stream) -------------
+    def set_timing(self, enable=True):
+        """Start (and reset) or stop per-stage event timing of transforms."""
+        _lib.check(_lib.load().dsht_plan_set_timing(self._plan, int(bool(enable))),
+                   "dsht_plan_set_timing")
+
+    def get_timing(self):
+        """(dict stage -> summed ms, number of transforms) since set_timing."""
+        ms = (ct.c_double * len(_lib.STAGES))()
+        n = ct.c_int64()
+        _lib.check(_lib.load().dsht_plan_get_timing(self._plan, ms, ct.byref(n)),
+                   "dsht_plan_get_timing")
+        return dict(zip(_lib.STAGES, list(ms))), n.value
+
     # ---- transforms -----------------------------------------------------
     def transform_device(self, theta, phi, wt, out=None, mode=_lib.DSHT_MODE_CALL, stream=None):
         """Enqueue one transform of device-resident float64 tensors.

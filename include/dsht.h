@@ -112,6 +112,16 @@ int dsht_plan_get_tables(dsht_plan* plan, double* Yv, double* Yd);
 int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
                    const double* wt, double* alm, int* status, int mode, void* stream);
 
+/* Stage timing.  While enabled, every dsht_transform records HIP events on
+ * its stream around its five stages: 0 point binning + counting sort,
+ * 1 scatter, 2 node/parity fold, 3 table contraction, 4 polar rotation +
+ * combine.  dsht_plan_get_timing waits for the recorded transforms and
+ * returns the summed milliseconds per stage and the number of transforms
+ * summed.  Enabling resets the sums. */
+#define DSHT_NSTAGES 5
+int dsht_plan_set_timing(dsht_plan* plan, int enable);
+int dsht_plan_get_timing(dsht_plan* plan, double stage_ms[DSHT_NSTAGES], int64_t* ntransforms);
+
 /* Message describing the calling thread's last error ("" if none). */
 const char* dsht_last_error(void);
 
