@@ -19,6 +19,8 @@ DSHT_FP64, DSHT_FP32 = 0, 1
 DSHT_MODE_CALL, DSHT_MODE_BASIC = 0, 1
 DSHT_STATUS_EQU_RANGE, DSHT_STATUS_POLAR_RANGE = 1, 2
 STAGES = ("points", "scatter", "fold", "contract", "rotate")
+DSHT_OPT_SCATTER, DSHT_OPT_CHUNK = 1, 2
+DSHT_SCATTER_MATRIX, DSHT_SCATTER_VECTOR = 0, 1
 
 # every entry point of include/dsht.h: name -> (restype, argtypes)
 _dp = ct.POINTER(ct.c_double)
@@ -34,8 +36,10 @@ SIGNATURES = {
                                       ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "dsht_plan_get_tables": (ct.c_int, [_vp, _vp, _vp]),
     "dsht_transform": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "dsht_plan_configure": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
     "dsht_plan_set_timing": (ct.c_int, [_vp, ct.c_int]),
     "dsht_plan_get_timing": (ct.c_int, [_vp, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
+    "dsht_plan_debug_rhs": (ct.c_int, [_vp, _vp, ct.c_int64]),
     "dsht_last_error": (ct.c_char_p, []),
 }
 

@@ -44,14 +44,15 @@ void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups, std::vector
 static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
 struct WsSizes {
-  int64_t points, chunks, runs;
+  int64_t points, chunks, runs, tiles;
 };
 
-static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
+static WsSizes ws_sizes(int Nx, int64_t Np, int chunk, int tile = 16384) {
   WsSizes z;
   z.points = std::max<int64_t>(Np, 1);
   z.chunks = (Np + chunk - 1) / chunk;
   z.runs = z.chunks + num_segments(Nx) + 1;
+  z.tiles = std::max<int64_t>(1, (Np + tile - 1) / tile);
   return z;
 }
 
@@ -66,6 +67,7 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
   b += align256(z.runs * sizeof(int));
   b += align256(z.runs * (int64_t)Nl * 8 * (precision == DSHT_FP32 ? 4 : 8));
   b += 4 * align256((nseg + 2) * sizeof(int));
+  b += align256(z.tiles * (nseg + 1) * sizeof(int));
   b += align256((int64_t)Nl * 2 * Npair * 16 * (precision == DSHT_FP32 ? 4 : 8));
   b += 2 * align256(Nlm * 2 * sizeof(double));
   b += 256;
@@ -75,7 +77,7 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
 static void free_workspace(Workspace& ws) {
   for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
-                  (void*)ws.ne_rank, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
+                  (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
 }
@@ -95,13 +97,14 @@ static void free_workspace(Workspace& ws) {
 // Grow the workspace to hold a transform of Np points.  Growth waits for the
 // device so buffers still in use by queued work are never freed under it.
 static int ensure_workspace(Plan& pl, int64_t Np) {
-  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
+  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk, pl.tile);
   Workspace& ws = pl.ws;
-  if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs)
+  if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
+      z.tiles <= ws.cap_tiles)
     return DSHT_OK;
   DSHT_HIP(hipDeviceSynchronize());
   WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
-               std::max(z.runs, ws.cap_runs)};
+               std::max(z.runs, ws.cap_runs), std::max(z.tiles, ws.cap_tiles)};
   free_workspace(ws);
   DSHT_ALLOC(ws.rec, want.points * sizeof(PointRec));
   DSHT_ALLOC(ws.run_count, (want.chunks + 1) * sizeof(int));
@@ -112,6 +115,7 @@ static int ensure_workspace(Plan& pl, int64_t Np) {
   DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.ne_rank, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.thist, want.tiles * (pl.nseg + 1) * sizeof(int));
   DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8));
   DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sizeof(double));
   DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sizeof(double));
@@ -119,6 +123,7 @@ static int ensure_workspace(Plan& pl, int64_t Np) {
   ws.cap_points = want.points;
   ws.cap_chunks = want.chunks;
   ws.cap_runs = want.runs;
+  ws.cap_tiles = want.tiles;
   return DSHT_OK;
 }
 
@@ -301,6 +306,54 @@ using namespace dsht;
 extern "C" {
 
 const char* dsht_last_error(void) { return g_err.c_str(); }
+
+int dsht_plan_configure(dsht_plan* p, int option, int value) {
+  if (!p) {
+    set_error("dsht_plan_configure: null plan");
+    return DSHT_EINVAL;
+  }
+  Plan& pl = p->pl;
+  if (option == DSHT_OPT_SCATTER) {
+    if (value != DSHT_SCATTER_MATRIX && value != DSHT_SCATTER_VECTOR) {
+      set_error("dsht_plan_configure: unknown scatter engine");
+      return DSHT_EINVAL;
+    }
+    pl.scatter = value;
+    return DSHT_OK;
+  }
+  if (option == DSHT_OPT_CHUNK) {
+    if (value < 64 || value > 65536) {
+      set_error("dsht_plan_configure: chunk must be in [64, 65536]");
+      return DSHT_EINVAL;
+    }
+    if (value != pl.chunk) {
+      DeviceGuard g(pl.device);
+      DSHT_HIP(hipDeviceSynchronize());  // queued transforms use the old decomposition
+      pl.chunk = value;
+      pl.ws.cap_chunks = pl.ws.cap_runs = 0;  // force a workspace resize on next use
+    }
+    return DSHT_OK;
+  }
+  set_error("dsht_plan_configure: unknown option");
+  return DSHT_EINVAL;
+}
+
+int dsht_plan_debug_rhs(dsht_plan* p, void* out, int64_t bytes) {
+  if (!p || !out) {
+    set_error("dsht_plan_debug_rhs: null argument");
+    return DSHT_EINVAL;
+  }
+  Plan& pl = p->pl;
+  const int64_t need = (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8);
+  if (bytes < need || !pl.ws.rhs) {
+    set_error("dsht_plan_debug_rhs: buffer too small or no transform yet");
+    return DSHT_EINVAL;
+  }
+  DeviceGuard g(pl.device);
+  DSHT_HIP(hipDeviceSynchronize());
+  DSHT_HIP(hipMemcpy(out, pl.ws.rhs, need, hipMemcpyDeviceToHost));
+  return DSHT_OK;
+}
 
 int dsht_plan_set_timing(dsht_plan* p, int enable) {
   if (!p) {

@@ -11,6 +11,7 @@ struct Workspace {
   int64_t cap_points = 0;  // capacity of the sorted-point arrays
   int64_t cap_chunks = 0;  // capacity of per-chunk arrays
   int64_t cap_runs = 0;    // capacity of the run partials
+  int64_t cap_tiles = 0;   // capacity of the tile histogram matrix (rows)
   int cur_chunks = 0;      // scatter chunks of the transform being enqueued
   PointRec* rec = nullptr;    // [cap_points] sorted point records
   int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
@@ -22,6 +23,7 @@ struct Workspace {
   int* seg_start = nullptr;   // [nseg + 2]
   int* seg_cursor = nullptr;  // [nseg + 2]
   int* ne_rank = nullptr;     // [nseg + 1]: non-empty segments before s
+  int* thist = nullptr;       // [cap_tiles][nseg + 1]: tile histograms, then offsets
   void* rhs = nullptr;        // [Nl][2][Npair][16] float or double
   double* a_equ = nullptr;    // [Nlm][2]
   double* a_pol = nullptr;    // [Nlm][2]
@@ -37,6 +39,8 @@ struct Plan {
   int precision = DSHT_FP64;
   // scatter decomposition
   int chunk = 2048;  // sorted points per scatter work item
+  int tile = 16384;  // input points per counting-sort tile
+  int scatter = DSHT_SCATTER_MATRIX;  // engine of FP32 plans
   // contraction layout
   int ngroups = 0;
   GroupDesc* groups = nullptr;  // device [ngroups]
@@ -74,6 +78,7 @@ int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipS
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, hipStream_t s);
 int launch_scatter(Plan& pl, int64_t Np, hipStream_t s);
+int launch_scatter_mfma(Plan& pl, int64_t Np, hipStream_t s);
 int launch_fold(Plan& pl, hipStream_t s);
 int launch_contract(Plan& pl, hipStream_t s);
 int launch_rotate_combine(Plan& pl, double* alm, int* status, int mode, hipStream_t s);

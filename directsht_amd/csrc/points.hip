@@ -216,18 +216,129 @@ __global__ __launch_bounds__(256) void k_count_runs(const int* __restrict__ seg_
   run_count[c] = ne_rank[se] - ne_rank[sb] + 1;
 }
 
+// ---- tile-based counting sort (no global atomics on segment counters) ----
+// Points are cut into tiles of `tile` consecutive input points.  Pass 1 writes
+// each tile's LDS histogram as a row of thist[tile][seg]; the column pass turns
+// the rows into per-(tile, segment) write offsets (tiles in order, after the
+// segment's start); pass 2 re-classifies the tile and places each record with
+// an LDS atomic on the tile's own offset row.
+__global__ __launch_bounds__(256) void k_tile_hist(int64_t Np, int tile,
+                                                   const double* __restrict__ theta,
+                                                   const double* __restrict__ phi,
+                                                   const double* __restrict__ wt, PointParams p,
+                                                   int* __restrict__ thist, int* __restrict__ flags) {
+  extern __shared__ int lh[];
+  const int nb = p.nseg + 1;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) lh[i] = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
+  int flag = 0, npol = 0;
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    Classified c = classify(theta[i], phi[i], wt[i], p);
+    flag |= c.flag;
+    npol += c.polar;
+    atomicAdd(&lh[c.seg], 1);
+  }
+  if (flag) atomicOr(&flags[0], flag);
+  if (npol) atomicAdd(&flags[1], npol);
+  __syncthreads();
+  int* row = thist + (int64_t)blockIdx.x * nb;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) row[i] = lh[i];
+}
+
+// Column passes over thist: 16 segments per block, each column split over 16
+// threads (tile ranges), so no thread walks more than ntiles/16 rows.
+constexpr int kColSplit = 16;
+
+__device__ inline int col_partial(const int* __restrict__ thist, int ntiles, int nb, int s, int q,
+                                  int& t0, int& t1) {
+  const int per = (ntiles + kColSplit - 1) / kColSplit;
+  t0 = min(ntiles, q * per);
+  t1 = min(ntiles, t0 + per);
+  int acc = 0;
+  if (s < nb)
+    for (int t = t0; t < t1; ++t) acc += thist[(int64_t)t * nb + s];
+  return acc;
+}
+
+// seg_count[s] = sum over tiles of thist[t][s]
+__global__ __launch_bounds__(256) void k_col_sum(const int* __restrict__ thist, int ntiles, int nb,
+                                                 int* __restrict__ seg_count) {
+  __shared__ int part[kColSplit][16];
+  const int sl = threadIdx.x % 16, q = threadIdx.x / 16;
+  const int s = blockIdx.x * 16 + sl;
+  int t0, t1;
+  part[q][sl] = col_partial(thist, ntiles, nb, s, q, t0, t1);
+  __syncthreads();
+  if (q == 0 && s < nb) {
+    int tot = 0;
+    for (int k = 0; k < kColSplit; ++k) tot += part[k][sl];
+    seg_count[s] = tot;
+  }
+}
+
+// thist[t][s] <- seg_start[s] + sum_{t' < t} thist[t'][s]
+__global__ __launch_bounds__(256) void k_col_offsets(int* __restrict__ thist, int ntiles, int nb,
+                                                     const int* __restrict__ seg_start) {
+  __shared__ int part[kColSplit][16];
+  const int sl = threadIdx.x % 16, q = threadIdx.x / 16;
+  const int s = blockIdx.x * 16 + sl;
+  int t0, t1;
+  part[q][sl] = col_partial(thist, ntiles, nb, s, q, t0, t1);
+  __syncthreads();
+  if (s >= nb) return;
+  int run = seg_start[s];
+  for (int k = 0; k < q; ++k) run += part[k][sl];
+  for (int t = t0; t < t1; ++t) {
+    const int64_t k = (int64_t)t * nb + s;
+    const int c = thist[k];
+    thist[k] = run;
+    run += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tile_place(int64_t Np, int tile,
+                                                    const double* __restrict__ theta,
+                                                    const double* __restrict__ phi,
+                                                    const double* __restrict__ wt, PointParams p,
+                                                    const int* __restrict__ toff,
+                                                    PointRec* __restrict__ rec) {
+  extern __shared__ int lo[];
+  const int nb = p.nseg + 1;
+  const int* row = toff + (int64_t)blockIdx.x * nb;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) lo[i] = row[i];
+  __syncthreads();
+  const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
+  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+    Classified c = classify(theta[i], phi[i], wt[i], p);
+    const int pos = atomicAdd(&lo[c.seg], 1);
+    double2* d = reinterpret_cast<double2*>(rec + pos);
+    d[0] = make_double2(c.ere, c.eim);
+    d[1] = make_double2(c.h0, c.h1);
+    d[2] = make_double2(c.h2, c.h3);
+    reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, 0);
+  }
+}
+
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, hipStream_t s) {
   Workspace& ws = pl.ws;
   PointParams p{pl.Nx, pl.nseg, mode, pl.xmax, pl.dx, pl.acos_xmax, pl.acos_mxmax};
   DSHT_HIP(hipMemsetAsync(ws.seg_count, 0, sizeof(int) * (pl.nseg + 2), s));
   DSHT_HIP(hipMemsetAsync(ws.flags, 0, sizeof(int) * 4, s));
+  const int nb = pl.nseg + 1;
+  const size_t lds = sizeof(int) * nb;
+  const int ntiles = (int)((Np + pl.tile - 1) / pl.tile);
+  const bool tiled = lds <= 64 * 1024 && ntiles <= ws.cap_tiles;
   const int threads = 256;
   const int64_t want = (Np + threads - 1) / threads;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
-  const size_t lds = sizeof(int) * (pl.nseg + 1);
   if (Np > 0) {
-    if (lds <= 64 * 1024) {
+    if (tiled) {
+      k_tile_hist<<<ntiles, threads, lds, s>>>(Np, pl.tile, theta, phi, wt, p, ws.thist, ws.flags);
+      DSHT_LAUNCH_CHECK();
+      k_col_sum<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_count);
+    } else if (lds <= 64 * 1024) {
       k_hist<true><<<blocks, threads, lds, s>>>(Np, theta, phi, wt, p, ws.seg_count, ws.flags);
     } else {
       k_hist<false><<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_count, ws.flags);
@@ -239,7 +350,13 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
   k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg, 1, ws.ne_rank, nullptr);
   DSHT_LAUNCH_CHECK();
   if (Np > 0) {
-    k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec);
+    if (tiled) {
+      k_col_offsets<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_start);
+      DSHT_LAUNCH_CHECK();
+      k_tile_place<<<ntiles, threads, lds, s>>>(Np, pl.tile, theta, phi, wt, p, ws.thist, ws.rec);
+    } else {
+      k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec);
+    }
     DSHT_LAUNCH_CHECK();
   }
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);

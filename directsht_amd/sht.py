@@ -117,6 +117,18 @@ class DirectSHT:
             cl[ell] /= 2 * ell + 1.0
         return cl
 
+    # ---- tuning -----------------------------------------------------------
+    def configure(self, scatter=None, chunk=None):
+        """Select the scatter engine ('matrix' | 'vector', FP32 plans) and the
+        sorted points per scatter work item.  Results do not depend on these
+        beyond rounding."""
+        lib = _lib.load()
+        if scatter is not None:
+            v = {"matrix": _lib.DSHT_SCATTER_MATRIX, "vector": _lib.DSHT_SCATTER_VECTOR}[scatter]
+            _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_SCATTER, v), "configure")
+        if chunk is not None:
+            _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_CHUNK, int(chunk)), "configure")
+
     # ---- stage timing (HIP events on the transform's stream) -------------
     def set_timing(self, enable=True):
         """Start (and reset) or stop per-stage event timing of transforms."""
@@ -130,6 +142,16 @@ class DirectSHT:
         _lib.check(_lib.load().dsht_plan_get_timing(self._plan, ms, ct.byref(n)),
                    "dsht_plan_get_timing")
         return dict(zip(_lib.STAGES, list(ms))), n.value
+
+    def debug_node_coefficients(self):
+        """Node coefficients of the last transform, shape (Nl, 2, ceil(Nx/2), 16)
+        (see dsht_plan_debug_rhs in include/dsht.h); for stage-level tests."""
+        npair = (self.Nx + 1) // 2
+        dt = np.float32 if self._precision == _lib.DSHT_FP32 else np.float64
+        out = np.empty((self.Nell, 2, npair, 16), dtype=dt)
+        _lib.check(_lib.load().dsht_plan_debug_rhs(self._plan, out.ctypes.data, out.nbytes),
+                   "dsht_plan_debug_rhs")
+        return out
 
     # ---- transforms -----------------------------------------------------
     def transform_device(self, theta, phi, wt, out=None, mode=_lib.DSHT_MODE_CALL, stream=None):

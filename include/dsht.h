@@ -112,6 +112,19 @@ int dsht_plan_get_tables(dsht_plan* plan, double* Yv, double* Yd);
 int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
                    const double* wt, double* alm, int* status, int mode, void* stream);
 
+/* Tuning options (dsht_plan_configure).  Results do not depend on them beyond
+ * rounding.
+ *   DSHT_OPT_SCATTER  scatter engine of DSHT_FP32 plans:
+ *                     DSHT_SCATTER_MATRIX (default) fp16x3 split-precision
+ *                     MFMA GEMM, or DSHT_SCATTER_VECTOR FP32 VALU recurrence.
+ *                     DSHT_FP64 plans always use the FP64 vector scatter.
+ *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536).    */
+#define DSHT_OPT_SCATTER 1
+#define DSHT_OPT_CHUNK 2
+#define DSHT_SCATTER_MATRIX 0
+#define DSHT_SCATTER_VECTOR 1
+int dsht_plan_configure(dsht_plan* plan, int option, int value);
+
 /* Stage timing.  While enabled, every dsht_transform records HIP events on
  * its stream around its five stages: 0 point binning + counting sort,
  * 1 scatter, 2 node/parity fold, 3 table contraction, 4 polar rotation +
@@ -121,6 +134,14 @@ int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const doubl
 #define DSHT_NSTAGES 5
 int dsht_plan_set_timing(dsht_plan* plan, int enable);
 int dsht_plan_get_timing(dsht_plan* plan, double stage_ms[DSHT_NSTAGES], int64_t* ntransforms);
+
+/* Diagnostic: copy the node coefficients of the plan's last transform (the
+ * right-hand side of the contraction, after the scatter and fold) to `out`:
+ * Nl * 2 * ceil(Nx/2) * 16 values, float for DSHT_FP32 plans, double for
+ * DSHT_FP64, layout [m][parity][node pair][branch*8 + {c re, c re', d re,
+ * d re', c im, c im', d im, d im'}] (primes: the odd node of the pair).
+ * Waits for the device.  Used by the stage-level tests. */
+int dsht_plan_debug_rhs(dsht_plan* plan, void* out, int64_t bytes);
 
 /* Message describing the calling thread's last error ("" if none). */
 const char* dsht_last_error(void);
