@@ -13,7 +13,8 @@ import threading
 import torch  # noqa: F401  (must precede the CDLL: shared HIP runtime)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdsht.so")
+# DSHT_LIBRARY selects an alternative build of the same C ABI (for A/B timing)
+LIB_PATH = os.environ.get("DSHT_LIBRARY") or os.path.join(HERE, "libdsht.so")
 
 DSHT_FP64, DSHT_FP32 = 0, 1
 DSHT_MODE_CALL, DSHT_MODE_BASIC = 0, 1

@@ -17,14 +17,16 @@
 // the size of the running sum; the accumulator is therefore restarted from
 // zero every 4 windows (64 points) and added into an outer FP32 sum.
 //
-// One workgroup (4 wavefronts) per (chunk of sorted points, m tile of 1024
-// orders).  Points are consumed in windows of 16 (one MFMA K step) that never
-// cross a segment boundary; a shorter window at the end of a run is padded
-// with h = 0.  Per window the workgroup generates E and B for its 16 points in
-// FP64 (powers of e1 = e^{-i phi}) into LDS, B already split into the
-// per-lane MFMA operand layout; wave w then builds the A operand of its rows
-// (m1 in [8w, 8w+8), all k) and issues 12 MFMAs.  A run's result goes to its
-// own slot part[run][half][m][4] (FP32), as for the vector scatter.
+// One workgroup (8 wavefronts) per (chunk of sorted points, m tile of 1024
+// orders).  The chunk is staged 128 points at a time: the workgroup generates
+// E and B for the block by short FP64 chains (powers of e1 = e^{-i phi} and of
+// e1^32) into LDS, B split into fp16 hi/lo in [m2][point] order so every MFMA
+// operand is one aligned 16-byte LDS read.  The block is then consumed in
+// 16-aligned windows (one MFMA K step); a window that contains a segment
+// boundary is issued once per run, with the points of other runs masked to
+// zero in A.  Wave w owns row tile w (k = w/2, 16 values of m1) and issues 6
+// MFMAs per window.  A run's result goes to its own slot part[run][half][m][4]
+// (FP32), as for the vector scatter.
 //
 // Operand layouts of v_mfma_f32_32x32x16_f16 (checked on gfx950):
 //   A: lane l holds A[l%32][8*(l/32) + j],  j = 0..7
@@ -64,7 +66,14 @@ __device__ inline cplx cpow(cplx e, int n) {
 
 }  // namespace
 
-constexpr int kWin = 16;
+constexpr int kWin = 16;    // points per MFMA K step
+constexpr int kBlock = 128; // points staged in LDS per generation phase
+constexpr int kGroup = 4;   // windows per MFMA accumulator restart
+// LDS row strides padded to 4 (mod 32) dwords: the 32 lanes of a half-wave
+// read 32 different rows (m2 or m1) at the same column, and with this stride
+// each group of 8 lanes covers all 32 banks with its 16-byte reads.
+constexpr int kRowB = kBlock + 8;   // halves per B row (68 dwords)
+constexpr int kRowE = kBlock + 4;   // floats per E row (132 dwords)
 
 // x = hi + lo with hi = fp16(x) and lo = fp16(x - hi).  x and hi are pinned
 // in registers: otherwise the compiler may produce the stored hi from one
@@ -74,12 +83,16 @@ constexpr int kWin = 16;
 __device__ inline void split_f16(float x, _Float16& hi, _Float16& lo) {
   asm volatile("" : "+v"(x));
   _Float16 h = (_Float16)x;
-  asm volatile("" : "+v"(h));  // one conversion: the stored hi is the subtracted hi
+  asm volatile("" : "+v"(h));
   hi = h;
   lo = (_Float16)(x - (float)h);
 }
 
-__global__ __launch_bounds__(256) void k_scatter_mfma(const PointRec* __restrict__ rec,
+__device__ inline uint32_t pack2(_Float16 a, _Float16 b) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+__global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restrict__ rec,
                                                       const int* __restrict__ seg_start, int nseg,
                                                       int chunk, int Nl,
                                                       const int* __restrict__ run_base,
@@ -91,128 +104,145 @@ __global__ __launch_bounds__(256) void k_scatter_mfma(const PointRec* __restrict
   const int b = c * chunk, e = min(nvalid, b + chunk);
   if (b >= e) return;
 
-  __shared__ _Float16 sB[4][64][8];   // B operand per lane: re hi, re lo, im hi, im lo
-  __shared__ float sE[2][32][kWin];   // E = e^{-i(1024 mt + 32 m1) phi_p}: [re|im][m1][p]
-  __shared__ float sH[4][kWin];       // h_k(p), 0 for padding points
+  // B = e^{-i m2 phi} as fp16 hi/lo, [re|im][hi|lo][m2][point]; E FP32
+  // [re|im][m1][point]; h_k FP32 [k][point]; segment of each point
+  __shared__ __attribute__((aligned(16))) _Float16 sB[2][2][32][kRowB];
+  __shared__ __attribute__((aligned(16))) float sE[2][32][kRowE];
+  __shared__ __attribute__((aligned(16))) float sH[4][kBlock];
+  __shared__ int sSeg[kBlock];
 
-  // [row tile T: k in {2T, 2T+1}][column tile: re | im]; acc: MFMA
-  // accumulator of the current group of windows, out: the run's outer sum
-  constexpr int kGroup = 4;
-  floatx16 acc[2][2], out[2][2];
-  const floatx16 zero = {};
-#pragma unroll
-  for (int T = 0; T < 2; ++T)
-#pragma unroll
-    for (int q = 0; q < 2; ++q) acc[T][q] = out[T][q] = zero;
-  int wgroup = 0;  // windows accumulated in acc
-
-  // this lane's A rows: i = l % 32 -> g, c1, m1; points 8*(l/32) + j
+  // wave w owns row tile w: k = w/2, m1 = 16 (w&1) + 8g + e; rows i = 16g + 8c + e
   const int ia = l & 31, bq = l >> 5;
-  const int ga = ia >> 4, ca = (ia >> 3) & 1, m1a = 8 * w + (ia & 7);
+  const int kA = w >> 1;
+  const int gA = ia >> 4, cA = (ia >> 3) & 1;
+  const int m1A = 16 * (w & 1) + 8 * gA + (ia & 7);
+
+  floatx16 acc[2], out[2];  // [column tile re | im]
+  const floatx16 zero = {};
+  acc[0] = acc[1] = out[0] = out[1] = zero;
+  int wgroup = 0;
 
   int run = run_base[c];
   int cur = rec[b].seg;
-  int pos = b;
-  while (pos < e) {
-    const int run_end = min(e, seg_start[cur + 1]);
-    while (pos < run_end) {
-      const int nwin = min(kWin, run_end - pos);
-      {
-        // generation: thread (p = tid/16, q = tid%16) makes B for m2 = 2q, 2q+1
-        // and E for m1 = 2q, 2q+1 of window point p
-        const int p = tid >> 4, q = tid & 15;
-        const bool valid = p < nwin;
+  int run_end = min(e, seg_start[cur + 1]);
+
+  for (int blk = b; blk < e; blk += kBlock) {
+    const int bn = min(kBlock, e - blk);
+    {
+      // generation: thread (pair pp = tid & 63 -> points 2pp, 2pp+1; group
+      // g = tid >> 6 -> m2 in [4g, 4g+4) and m1 in [4g, 4g+4)), FP64 chains
+      const int pp = tid & 63, g = tid >> 6;
+      cplx bz[2], ez[2], e1v[2], e32v[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int p = 2 * pp + u;
         cplx e1{1.0, 0.0};
-        double hk = 0.0;
-        if (valid) {
-          const PointRec& pr = rec[pos + p];
+        if (p < bn) {
+          const PointRec& pr = rec[blk + p];
           e1 = cplx{pr.ere, pr.eim};
-          if (q < 4) hk = q == 0 ? pr.h0 : q == 1 ? pr.h1 : q == 2 ? pr.h2 : pr.h3;
+          if (g < 4) sH[g][p] = (float)(g == 0 ? pr.h0 : g == 1 ? pr.h1 : g == 2 ? pr.h2 : pr.h3);
+          if (g == 4) sSeg[p] = pr.seg;
+        } else {
+          if (g < 4) sH[g][p] = 0.f;
+          if (g == 4) sSeg[p] = -1;
         }
-        cplx z0 = cpow(e1, 2 * q);
-        cplx z1 = cmul(z0, e1);
-        cplx e32 = csq(csq(csq(csq(csq(e1)))));
-        cplx E0 = cpow(e32, 32 * mt + 2 * q);
-        cplx E1 = cmul(E0, e32);
-        if (!valid) z0 = z1 = E0 = E1 = cplx{0.0, 0.0};
-        const int lane0 = 32 * (p >> 3), j = p & 7;
+        const cplx e4 = csq(csq(e1));
+        const cplx e32 = csq(csq(csq(e4)));
+        const cplx e128 = csq(csq(e32));
+        bz[u] = cpow(e4, g);                                           // e1^(4g)
+        ez[u] = cmul(cpow(e128, g), cpow(csq(csq(csq(e128))), mt));    // e1^(128g + 1024mt)
+        e1v[u] = e1;
+        e32v[u] = e32;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = 4 * g + q;
+        _Float16 rh0, rl0, ih0, il0, rh1, rl1, ih1, il1;
+        split_f16((float)bz[0].x, rh0, rl0);
+        split_f16((float)bz[0].y, ih0, il0);
+        split_f16((float)bz[1].x, rh1, rl1);
+        split_f16((float)bz[1].y, ih1, il1);
+        *reinterpret_cast<uint32_t*>(&sB[0][0][m][2 * pp]) = pack2(rh0, rh1);
+        *reinterpret_cast<uint32_t*>(&sB[0][1][m][2 * pp]) = pack2(rl0, rl1);
+        *reinterpret_cast<uint32_t*>(&sB[1][0][m][2 * pp]) = pack2(ih0, ih1);
+        *reinterpret_cast<uint32_t*>(&sB[1][1][m][2 * pp]) = pack2(il0, il1);
+        *reinterpret_cast<float2*>(&sE[0][m][2 * pp]) = make_float2((float)ez[0].x, (float)ez[1].x);
+        *reinterpret_cast<float2*>(&sE[1][m][2 * pp]) = make_float2((float)ez[0].y, (float)ez[1].y);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-          const cplx z = u ? z1 : z0;
-          const int ln = lane0 + 2 * q + u;
-          split_f16((float)z.x, sB[0][ln][j], sB[1][ln][j]);
-          split_f16((float)z.y, sB[2][ln][j], sB[3][ln][j]);
+          bz[u] = cmul(bz[u], e1v[u]);
+          ez[u] = cmul(ez[u], e32v[u]);
         }
-        sE[0][2 * q][p] = (float)E0.x;
-        sE[1][2 * q][p] = (float)E0.y;
-        sE[0][2 * q + 1][p] = (float)E1.x;
-        sE[1][2 * q + 1][p] = (float)E1.y;
-        if (q < 4) sH[q][p] = (float)hk;
       }
-      __syncthreads();
-      {
-        half8 ahi[2], alo[2];
+    }
+    __syncthreads();
+    for (int wo = 0; wo < bn; wo += kWin) {
+      const int whi = blk + min(wo + kWin, bn);
+      int p = blk + wo;
+      // operands of this window that do not depend on the run split
+      const int po = wo + 8 * bq;  // first point of this lane's K slice
+      const float4 E0 = *reinterpret_cast<const float4*>(&sE[cA][m1A][po]);
+      const float4 E1 = *reinterpret_cast<const float4*>(&sE[cA][m1A][po + 4]);
+      const float4 H0 = *reinterpret_cast<const float4*>(&sH[kA][po]);
+      const float4 H1 = *reinterpret_cast<const float4*>(&sH[kA][po + 4]);
+      const half8 brh = *reinterpret_cast<const half8*>(&sB[0][0][l & 31][po]);
+      const half8 brl = *reinterpret_cast<const half8*>(&sB[0][1][l & 31][po]);
+      const half8 bih = *reinterpret_cast<const half8*>(&sB[1][0][l & 31][po]);
+      const half8 bil = *reinterpret_cast<const half8*>(&sB[1][1][l & 31][po]);
+      const float ev[8] = {E0.x, E0.y, E0.z, E0.w, E1.x, E1.y, E1.z, E1.w};
+      const float hv[8] = {H0.x, H0.y, H0.z, H0.w, H1.x, H1.y, H1.z, H1.w};
+      while (p < whi) {
+        const int vhi = min(whi, run_end);  // points [p, vhi) belong to run `cur`
+        half8 ahi, alo;
 #pragma unroll
         for (int jj = 0; jj < 8; ++jj) {
-          const int p = 8 * bq + jj;
-          const float ev = sE[ca][m1a][p];
-#pragma unroll
-          for (int T = 0; T < 2; ++T) {
-            _Float16 hi, lo;
-            split_f16(sH[2 * T + ga][p] * ev, hi, lo);
-            ahi[T][jj] = hi;
-            alo[T][jj] = lo;
-          }
+          const int pg = blk + po + jj;
+          const float a = (pg >= p && pg < vhi) ? hv[jj] * ev[jj] : 0.f;
+          _Float16 hi, lo;
+          split_f16(a, hi, lo);
+          ahi[jj] = hi;
+          alo[jj] = lo;
         }
-        const half8 brh = *reinterpret_cast<const half8*>(sB[0][l]);
-        const half8 brl = *reinterpret_cast<const half8*>(sB[1][l]);
-        const half8 bih = *reinterpret_cast<const half8*>(sB[2][l]);
-        const half8 bil = *reinterpret_cast<const half8*>(sB[3][l]);
         const bool fresh = wgroup == 0;  // uniform
-#pragma unroll
-        for (int T = 0; T < 2; ++T) {
-          acc[T][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[T], brh, fresh ? zero : acc[T][0], 0, 0, 0);
-          acc[T][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[T], brl, acc[T][0], 0, 0, 0);
-          acc[T][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo[T], brh, acc[T][0], 0, 0, 0);
-          acc[T][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[T], bih, fresh ? zero : acc[T][1], 0, 0, 0);
-          acc[T][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi[T], bil, acc[T][1], 0, 0, 0);
-          acc[T][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo[T], bih, acc[T][1], 0, 0, 0);
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, brh, fresh ? zero : acc[0], 0, 0, 0);
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, brl, acc[0], 0, 0, 0);
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, brh, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bih, fresh ? zero : acc[1], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bil, acc[1], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bih, acc[1], 0, 0, 0);
+        p = vhi;
+        const bool run_done = p == run_end;
+        if (++wgroup == kGroup || run_done) {
+          out[0] += acc[0];
+          out[1] += acc[1];
+          wgroup = 0;
         }
-      }
-      __syncthreads();
-      pos += nwin;
-      if (++wgroup == kGroup || pos >= run_end) {
+        if (run_done) {
+          // flush: lane holds (k = kA, m1 = 16(w&1) + 8g + 4(l/32) + d, m2 = l%32)
 #pragma unroll
-        for (int T = 0; T < 2; ++T)
+          for (int g = 0; g < 2; ++g)
 #pragma unroll
-          for (int q = 0; q < 2; ++q) out[T][q] += acc[T][q];
-        wgroup = 0;
-      }
-    }
-    // flush the run: lane holds (k = 2T + g, m1 = 8w + 4(l/32) + d, m2 = l%32)
-#pragma unroll
-    for (int T = 0; T < 2; ++T)
-#pragma unroll
-      for (int g = 0; g < 2; ++g)
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          const int m = 1024 * mt + 32 * (8 * w + 4 * bq + d) + (l & 31);
-          if (m < Nl) {
-            const float vre = out[T][0][8 * g + d] - out[T][1][8 * g + 4 + d];
-            const float vim = out[T][1][8 * g + d] + out[T][0][8 * g + 4 + d];
-            *reinterpret_cast<float2*>(part + (((int64_t)run * 2 + T) * Nl + m) * 4 + 2 * g) =
-                make_float2(vre, vim);
+            for (int d = 0; d < 4; ++d) {
+              const int m = 1024 * mt + 32 * (16 * (w & 1) + 8 * g + 4 * bq + d) + (l & 31);
+              if (m < Nl) {
+                const float vre = out[0][8 * g + d] - out[1][8 * g + 4 + d];
+                const float vim = out[1][8 * g + d] + out[0][8 * g + 4 + d];
+                *reinterpret_cast<float2*>(part + (((int64_t)run * 2 + (kA >> 1)) * Nl + m) * 4 +
+                                           2 * (kA & 1)) = make_float2(vre, vim);
+              }
+            }
+          if (tid == 0 && mt == 0) run_seg[run] = cur;
+          out[0] = out[1] = zero;
+          if (p < e) {
+            ++run;
+            const int off = p - blk;  // point p may lie past this block
+            cur = off < bn ? sSeg[off] : rec[p].seg;
+            run_end = min(e, seg_start[cur + 1]);
           }
         }
-    if (tid == 0 && mt == 0) run_seg[run] = cur;
-#pragma unroll
-    for (int T = 0; T < 2; ++T)
-#pragma unroll
-      for (int q = 0; q < 2; ++q) out[T][q] = zero;
-    if (pos < e) {
-      ++run;
-      cur = rec[pos].seg;
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -220,7 +250,7 @@ int launch_scatter_mfma(Plan& pl, int64_t Np, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
   dim3 grid(nchunks, (pl.Nl + 1023) / 1024);
-  k_scatter_mfma<<<grid, 256, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
+  k_scatter_mfma<<<grid, 512, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
                                       pl.ws.run_base, (float*)pl.ws.part, pl.ws.run_seg);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
