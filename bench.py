@@ -29,7 +29,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "points·modes/s and alm/s at Nl=1000,1e7 pts; batched vs plain; p50/p99 latency"
 PEAK_FP32_TFLOPS = 157.3   # MI355X published dense FP32
+PEAK_FP16_TFLOPS = 2500.0  # MI355X published dense FP16 matrix
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+# HBM bytes per launch of each kernel at the default workload, measured by
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of
+# known size; the summary holding them is read when the workload matches.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_plain_c2.md")
+
+
+def read_traffic():
+    """{kernel: (fetch_bytes, write_bytes)} from the PMC summary's
+    'traffic <kernel> <fetch> <write>' lines, or {} when absent."""
+    out = {}
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic":
+                    out[parts[1]] = (float(parts[2]), float(parts[3]))
+    except OSError:
+        pass
+    return out
 
 
 def log(*a):
@@ -79,6 +99,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after timing, run a 1 GiB device copy 3 times (the known-size "
+                         "kernel that scales the PMC FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -118,11 +141,23 @@ def main():
     stage_avg = {k: v / max(ntimed, 1) for k, v in stage_ms.items()}
     log(f"{ms:.3f} ms/step; stages {stage_avg}")
 
-    # dominant kernel: the scatter (FP32 VALU), 22 flop per (point, order)
-    scat_flop = 22.0 * N * Nl
+    # Dominant kernel: the matrix-core scatter.  Algorithmic work per
+    # (point, order m): the complex products w h_k(t) e^{-i m phi} for the four
+    # Hermite weights k, as the real GEMM [(k, m1, re|im) x point] .
+    # [point x (re|im, m2)] with m = 1024 mt + 32 m1 + m2: 2*256*64/1024 = 32
+    # flop, done three times (fp16 hi*hi, hi*lo, lo*hi) = 96 MFMA flop.
+    scat_flop = 96.0 * N * Nl
     scat_tf = scat_flop / (stage_avg["scatter"] * 1e-3) / 1e12
+    valu_tf = 22.0 * N * Nl / (stage_avg["scatter"] * 1e-3) / 1e12
     tab_bytes = 2.0 * Nlm * Nx * 4
     con_gbs = tab_bytes / (stage_avg["contract"] * 1e-3) / 1e9
+    default_workload = (N == 10_000_000)
+    traffic = read_traffic() if default_workload else {}
+
+    def tr(name):
+        t = traffic.get(name)
+        return None if t is None else t[0] + t[1]
+
     result = {
         "metric": METRIC,
         "value": N * Nlm / (ms * 1e-3),
@@ -142,14 +177,27 @@ def main():
                    "Nl": Nl, "Nx": Nx, "npoints": N, "parallelism": "single GPU"},
         "alm_per_s": Nlm / (ms * 1e-3),
         "stage_ms": stage_avg,
-        "roofline": {"kernel": "k_scatter32", "bound": "mfma", "engine": "VALU FP32",
-                     "achieved": scat_tf, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": scat_tf / PEAK_FP32_TFLOPS, "traffic": None,
-                     "algorithmic": "22 flop per (point, m): 8 complex FMA-accumulates + 1 complex rotation"},
+        "roofline": {"kernel": "k_scatter_mfma", "bound": "mfma", "engine": "MFMA fp16 (x3 split)",
+                     "achieved": scat_tf, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": scat_tf / PEAK_FP16_TFLOPS, "traffic": tr("k_scatter_mfma"),
+                     "algorithmic": "96 MFMA flop per (point, m): 32-flop real GEMM form of "
+                                    "4 complex products, x3 for the fp16 hi/lo split",
+                     "valu_equivalent_tflops": valu_tf,
+                     "valu_equivalent_frac": valu_tf / PEAK_FP32_TFLOPS},
         "roofline_contract": {"kernel": "k_contract32", "bound": "hbm", "achieved": con_gbs,
                               "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": con_gbs / PEAK_HBM_GBS,
-                              "traffic": None, "algorithmic": "FP32 tables 2*Nlm*Nx*4 B"},
+                              "traffic": tr("k_contract32"),
+                              "algorithmic": "FP32 tables 2*Nlm*Nx*4 B"},
     }
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 1 GiB device copies done")
+        del src, dst
 
     if not args.no_accuracy:
         log("accuracy: FP64 plan on the same points")

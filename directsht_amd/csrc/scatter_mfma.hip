@@ -18,15 +18,16 @@
 // zero every 4 windows (64 points) and added into an outer FP32 sum.
 //
 // One workgroup (8 wavefronts) per (chunk of sorted points, m tile of 1024
-// orders).  The chunk is staged 128 points at a time: the workgroup generates
-// E and B for the block by short FP64 chains (powers of e1 = e^{-i phi} and of
-// e1^32) into LDS, B split into fp16 hi/lo in [m2][point] order so every MFMA
-// operand is one aligned 16-byte LDS read.  The block is then consumed in
-// 16-aligned windows (one MFMA K step); a window that contains a segment
-// boundary is issued once per run, with the points of other runs masked to
-// zero in A.  Wave w owns row tile w (k = w/2, 16 values of m1) and issues 6
-// MFMAs per window.  A run's result goes to its own slot part[run][half][m][4]
-// (FP32), as for the vector scatter.
+// orders).  The chunk is staged 128 points at a time: thread (point p, group
+// g = 0..3) generates B for m2 in [8g, 8g+8) and E for m1 in [8g, 8g+8) by
+// FP64 chains (steps e1 = e^{-i phi} and e1^32) into LDS, B split into fp16
+// hi/lo in [m2][point] order so every MFMA operand is one aligned 16-byte LDS
+// read.  The block is then consumed in 16-aligned windows (one MFMA K step).
+// A window inside one run (almost all of them) builds A = h_k E unmasked; a
+// window that contains a segment boundary is issued once per run, with the
+// points of other runs masked to zero in A.  Wave w owns row tile w (k = w/2,
+// 16 values of m1) and issues 6 MFMAs per window.  A run's result goes to its
+// own slot part[run][half][m][4] (FP32), as for the vector scatter.
 //
 // Operand layouts of v_mfma_f32_32x32x16_f16 (checked on gfx950):
 //   A: lane l holds A[l%32][8*(l/32) + j],  j = 0..7
@@ -68,28 +69,85 @@ __device__ inline cplx cpow(cplx e, int n) {
 
 constexpr int kWin = 16;    // points per MFMA K step
 constexpr int kBlock = 128; // points staged in LDS per generation phase
-constexpr int kGroup = 4;   // windows per MFMA accumulator restart
+// Windows per MFMA accumulator restart into an outer FP32 sum; 0 = the MFMA
+// accumulator carries the whole run.
+constexpr int kGroup = 0;
 // LDS row strides padded to 4 (mod 32) dwords: the 32 lanes of a half-wave
 // read 32 different rows (m2 or m1) at the same column, and with this stride
 // each group of 8 lanes covers all 32 banks with its 16-byte reads.
 constexpr int kRowB = kBlock + 8;   // halves per B row (68 dwords)
 constexpr int kRowE = kBlock + 4;   // floats per E row (132 dwords)
 
-// x = hi + lo with hi = fp16(x) and lo = fp16(x - hi).  x and hi are pinned
-// in registers: otherwise the compiler may produce the stored hi from one
-// rounding path (e.g. a packed conversion of the FP32 product) and lo from
-// another (a mixed-precision FMA against a differently rounded hi), and lo
-// then no longer corrects the hi that is actually used.
-__device__ inline void split_f16(float x, _Float16& hi, _Float16& lo) {
-  asm volatile("" : "+v"(x));
-  _Float16 h = (_Float16)x;
-  asm volatile("" : "+v"(h));
-  hi = h;
-  lo = (_Float16)(x - (float)h);
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// x = hi + lo for 8 values: hi = fp16(x) (round to nearest even, packed in
+// pairs), lo = fp16(x - hi) computed by a mixed-precision FMA that reads the
+// very hi register the MFMA will read (x - hi is exact in FP32, so lo is
+// correctly rounded).  One asm block keeps the compiler from deriving hi and
+// lo from differently rounded copies of x; the trailing s_nop covers the
+// VALU-write -> MFMA-read dependency, which the compiler's hazard tracking
+// does not see through inline assembly.
+__device__ inline void split8_f16(const float (&x)[8], half8& ahi, half8& alo) {
+  uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
+  asm("v_cvt_pk_f16_f32 %0, %8, %9\n\t"
+      "v_cvt_pk_f16_f32 %1, %10, %11\n\t"
+      "v_cvt_pk_f16_f32 %2, %12, %13\n\t"
+      "v_cvt_pk_f16_f32 %3, %14, %15\n\t"
+      "v_fma_mixlo_f16 %4, -%0, 1.0, %8 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %4, -%0, 1.0, %9 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixlo_f16 %5, -%1, 1.0, %10 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %5, -%1, 1.0, %11 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixlo_f16 %6, -%2, 1.0, %12 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %6, -%2, 1.0, %13 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixlo_f16 %7, -%3, 1.0, %14 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %7, -%3, 1.0, %15 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
+      "s_nop 4"
+      : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
+      : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]));
+  ahi = __builtin_bit_cast(half8, (u32x4){h0, h1, h2, h3});
+  alo = __builtin_bit_cast(half8, (u32x4){l0, l1, l2, l3});
 }
 
-__device__ inline uint32_t pack2(_Float16 a, _Float16 b) {
-  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+// hi/lo of a complex value (re, im) for the B operand: packed (re | im << 16);
+// the result goes to LDS, so no MFMA hazard applies.
+__device__ inline void split2_f16(float re, float im, uint32_t& hi, uint32_t& lo) {
+  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
+      "v_fma_mixlo_f16 %1, -%0, 1.0, %2 op_sel_hi:[1,0,0]\n\t"
+      "v_fma_mixhi_f16 %1, -%0, 1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=&v"(hi), "=&v"(lo)
+      : "v"(re), "v"(im));
+}
+
+__device__ inline _Float16 half_lo(uint32_t u) { return __builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)); }
+__device__ inline _Float16 half_hi(uint32_t u) { return __builtin_bit_cast(_Float16, (uint16_t)(u >> 16)); }
+
+// A operand of one window pass: this lane's 8 points (K slice) of row
+// (k, m1, re|im): h_k(p) E(m1, p), with points outside [p0, p1) zeroed when
+// the window holds more than one run.
+template <bool kMasked>
+__device__ inline void build_a(const float (&hv)[8], const float (&ev)[8], int pg0, int p0, int p1,
+                               half8& ahi, half8& alo) {
+  float x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const float v = hv[j] * ev[j];
+    x[j] = (!kMasked || (pg0 + j >= p0 && pg0 + j < p1)) ? v : 0.f;
+  }
+  split8_f16(x, ahi, alo);
+}
+
+// 6 MFMAs of one window pass: acc[c] (+)= A_hi B_hi + A_hi B_lo + A_lo B_hi
+// for column tiles c = re, im.  kFresh restarts the accumulators from zero.
+template <bool kFresh>
+__device__ inline void mfma6(const half8& ahi, const half8& alo, const half8& brh, const half8& brl,
+                             const half8& bih, const half8& bil, floatx16 (&acc)[2]) {
+  const floatx16 zero = {};
+  acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, brh, kFresh ? zero : acc[0], 0, 0, 0);
+  acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, brl, acc[0], 0, 0, 0);
+  acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, brh, acc[0], 0, 0, 0);
+  acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bih, kFresh ? zero : acc[1], 0, 0, 0);
+  acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bil, acc[1], 0, 0, 0);
+  acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bih, acc[1], 0, 0, 0);
 }
 
 __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restrict__ rec,
@@ -122,57 +180,48 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
   acc[0] = acc[1] = out[0] = out[1] = zero;
   int wgroup = 0;
 
-  int run = run_base[c];
-  int cur = rec[b].seg;
+  // run tracking is wave-uniform (scalar registers, scalar branches)
+  int run = __builtin_amdgcn_readfirstlane(run_base[c]);
+  int cur = __builtin_amdgcn_readfirstlane(rec[b].seg);
   int run_end = min(e, seg_start[cur + 1]);
 
   for (int blk = b; blk < e; blk += kBlock) {
     const int bn = min(kBlock, e - blk);
     {
-      // generation: thread (pair pp = tid & 63 -> points 2pp, 2pp+1; group
-      // g = tid >> 6 -> m2 in [4g, 4g+4) and m1 in [4g, 4g+4)), FP64 chains
-      const int pp = tid & 63, g = tid >> 6;
-      cplx bz[2], ez[2], e1v[2], e32v[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int p = 2 * pp + u;
-        cplx e1{1.0, 0.0};
-        if (p < bn) {
-          const PointRec& pr = rec[blk + p];
-          e1 = cplx{pr.ere, pr.eim};
-          if (g < 4) sH[g][p] = (float)(g == 0 ? pr.h0 : g == 1 ? pr.h1 : g == 2 ? pr.h2 : pr.h3);
-          if (g == 4) sSeg[p] = pr.seg;
-        } else {
-          if (g < 4) sH[g][p] = 0.f;
-          if (g == 4) sSeg[p] = -1;
-        }
-        const cplx e4 = csq(csq(e1));
-        const cplx e32 = csq(csq(csq(e4)));
-        const cplx e128 = csq(csq(e32));
-        bz[u] = cpow(e4, g);                                           // e1^(4g)
-        ez[u] = cmul(cpow(e128, g), cpow(csq(csq(csq(e128))), mt));    // e1^(128g + 1024mt)
-        e1v[u] = e1;
-        e32v[u] = e32;
+      // generation: thread (point p, group g = 0..3, uniform per wavefront):
+      // B for m2 = 8g + q and E for m1 = 8g + q, q = 0..7, by FP64 chains
+      const int p = tid & (kBlock - 1);
+      const int g = __builtin_amdgcn_readfirstlane(tid >> 7);  // = w / 2
+      cplx e1{1.0, 0.0};
+      float hval = 0.f;
+      int sg = -1;
+      if (p < bn) {
+        const PointRec& pr = rec[blk + p];
+        e1 = cplx{pr.ere, pr.eim};
+        hval = (float)(g == 0 ? pr.h0 : g == 1 ? pr.h1 : g == 2 ? pr.h2 : pr.h3);
+        sg = pr.seg;
       }
+      sH[g][p] = hval;
+      if (g == 0) sSeg[p] = sg;
+      const cplx e8 = csq(csq(csq(e1)));
+      const cplx e32 = csq(csq(e8));
+      const cplx e256 = csq(csq(csq(e32)));
+      cplx bz = cpow(e8, g);    // e1^(8g)
+      cplx ez = cpow(e256, g);  // e1^(256g)
+      if (mt > 0) ez = cmul(ez, cpow(csq(csq(e256)), mt));  // * e1^(1024mt)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = 4 * g + q;
-        _Float16 rh0, rl0, ih0, il0, rh1, rl1, ih1, il1;
-        split_f16((float)bz[0].x, rh0, rl0);
-        split_f16((float)bz[0].y, ih0, il0);
-        split_f16((float)bz[1].x, rh1, rl1);
-        split_f16((float)bz[1].y, ih1, il1);
-        *reinterpret_cast<uint32_t*>(&sB[0][0][m][2 * pp]) = pack2(rh0, rh1);
-        *reinterpret_cast<uint32_t*>(&sB[0][1][m][2 * pp]) = pack2(rl0, rl1);
-        *reinterpret_cast<uint32_t*>(&sB[1][0][m][2 * pp]) = pack2(ih0, ih1);
-        *reinterpret_cast<uint32_t*>(&sB[1][1][m][2 * pp]) = pack2(il0, il1);
-        *reinterpret_cast<float2*>(&sE[0][m][2 * pp]) = make_float2((float)ez[0].x, (float)ez[1].x);
-        *reinterpret_cast<float2*>(&sE[1][m][2 * pp]) = make_float2((float)ez[0].y, (float)ez[1].y);
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          bz[u] = cmul(bz[u], e1v[u]);
-          ez[u] = cmul(ez[u], e32v[u]);
-        }
+      for (int q = 0; q < 8; ++q) {
+        const int m = 8 * g + q;
+        uint32_t hi, lo;
+        split2_f16((float)bz.x, (float)bz.y, hi, lo);
+        sB[0][0][m][p] = half_lo(hi);
+        sB[1][0][m][p] = half_hi(hi);
+        sB[0][1][m][p] = half_lo(lo);
+        sB[1][1][m][p] = half_hi(lo);
+        sE[0][m][p] = (float)ez.x;
+        sE[1][m][p] = (float)ez.y;
+        bz = cmul(bz, e1);
+        ez = cmul(ez, e32);
       }
     }
     __syncthreads();
@@ -194,28 +243,25 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
       while (p < whi) {
         const int vhi = min(whi, run_end);  // points [p, vhi) belong to run `cur`
         half8 ahi, alo;
-#pragma unroll
-        for (int jj = 0; jj < 8; ++jj) {
-          const int pg = blk + po + jj;
-          const float a = (pg >= p && pg < vhi) ? hv[jj] * ev[jj] : 0.f;
-          _Float16 hi, lo;
-          split_f16(a, hi, lo);
-          ahi[jj] = hi;
-          alo[jj] = lo;
-        }
-        const bool fresh = wgroup == 0;  // uniform
-        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, brh, fresh ? zero : acc[0], 0, 0, 0);
-        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, brl, acc[0], 0, 0, 0);
-        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, brh, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bih, fresh ? zero : acc[1], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bil, acc[1], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bih, acc[1], 0, 0, 0);
+        const int pg0 = blk + po;
+        if (vhi == whi && p == blk + wo)  // uniform: the whole window is one run
+          build_a<false>(hv, ev, pg0, p, vhi, ahi, alo);
+        else
+          build_a<true>(hv, ev, pg0, p, vhi, ahi, alo);
+        if (wgroup == 0)  // uniform
+          mfma6<true>(ahi, alo, brh, brl, bih, bil, acc);
+        else
+          mfma6<false>(ahi, alo, brh, brl, bih, bil, acc);
         p = vhi;
         const bool run_done = p == run_end;
-        if (++wgroup == kGroup || run_done) {
-          out[0] += acc[0];
-          out[1] += acc[1];
-          wgroup = 0;
+        if constexpr (kGroup > 0) {
+          if (++wgroup == kGroup || run_done) {
+            out[0] += acc[0];
+            out[1] += acc[1];
+            wgroup = 0;
+          }
+        } else {
+          wgroup = run_done ? 0 : 1;
         }
         if (run_done) {
           // flush: lane holds (k = kA, m1 = 16(w&1) + 8g + 4(l/32) + d, m2 = l%32)
@@ -225,18 +271,23 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
             for (int d = 0; d < 4; ++d) {
               const int m = 1024 * mt + 32 * (16 * (w & 1) + 8 * g + 4 * bq + d) + (l & 31);
               if (m < Nl) {
-                const float vre = out[0][8 * g + d] - out[1][8 * g + 4 + d];
-                const float vim = out[1][8 * g + d] + out[0][8 * g + 4 + d];
+                const floatx16& r0 = kGroup > 0 ? out[0] : acc[0];
+                const floatx16& r1 = kGroup > 0 ? out[1] : acc[1];
+                const float vre = r0[8 * g + d] - r1[8 * g + 4 + d];
+                const float vim = r1[8 * g + d] + r0[8 * g + 4 + d];
                 *reinterpret_cast<float2*>(part + (((int64_t)run * 2 + (kA >> 1)) * Nl + m) * 4 +
                                            2 * (kA & 1)) = make_float2(vre, vim);
               }
             }
           if (tid == 0 && mt == 0) run_seg[run] = cur;
-          out[0] = out[1] = zero;
+          if constexpr (kGroup > 0) out[0] = out[1] = zero;
           if (p < e) {
             ++run;
             const int off = p - blk;  // point p may lie past this block
-            cur = off < bn ? sSeg[off] : rec[p].seg;
+            if (off < bn)
+              cur = __builtin_amdgcn_readfirstlane(sSeg[off]);
+            else
+              cur = __builtin_amdgcn_readfirstlane(rec[p].seg);
             run_end = min(e, seg_start[cur + 1]);
           }
         }
