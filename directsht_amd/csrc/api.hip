@@ -41,13 +41,41 @@ void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups, std::vector
   tab_vec4 = off;
 }
 
+// Launch order of the contraction's row groups.  Workgroups are dispatched to
+// the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each XCD has its
+// own L2.  Give every XCD a contiguous, row-balanced range of groups, so the
+// groups of one (m, P) -- which all read the same node coefficients -- run
+// on one XCD and share them in its L2; pad the shorter ranges with empty
+// groups (nrows = 0), which exit at once.
+std::vector<GroupDesc> xcd_schedule(const std::vector<GroupDesc>& groups, int nxcd) {
+  int64_t total = 0;
+  for (const GroupDesc& g : groups) total += g.nrows;
+  std::vector<std::vector<GroupDesc>> per(nxcd);
+  int64_t before = 0;
+  for (const GroupDesc& g : groups) {
+    const int64_t mid2 = 2 * before + g.nrows;  // twice the group's row midpoint
+    int x = (int)(mid2 * nxcd / std::max<int64_t>(1, 2 * total));
+    x = std::min(nxcd - 1, std::max(0, x));
+    per[x].push_back(g);
+    before += g.nrows;
+  }
+  size_t len = 0;
+  for (const auto& v : per) len = std::max(len, v.size());
+  GroupDesc empty{};
+  std::vector<GroupDesc> out(len * nxcd, empty);
+  for (int x = 0; x < nxcd; ++x)
+    for (size_t q = 0; q < per[x].size(); ++q) out[q * nxcd + x] = per[x][q];
+  return out;
+}
+
 static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
 struct WsSizes {
   int64_t points, chunks, runs, tiles;
 };
 
-static WsSizes ws_sizes(int Nx, int64_t Np, int chunk, int tile = 16384) {
+static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
+  const int tile = sort_tile(Np);
   WsSizes z;
   z.points = std::max<int64_t>(Np, 1);
   z.chunks = (Np + chunk - 1) / chunk;
@@ -66,7 +94,7 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
   b += 2 * align256((z.chunks + 1) * sizeof(int));
   b += align256(z.runs * sizeof(int));
   b += align256(z.runs * (int64_t)Nl * 8 * (precision == DSHT_FP32 ? 4 : 8));
-  b += 4 * align256((nseg + 2) * sizeof(int));
+  b += 6 * align256((nseg + 2) * sizeof(int));
   b += align256(z.tiles * (nseg + 1) * sizeof(int));
   b += align256((int64_t)Nl * 2 * Npair * 16 * (precision == DSHT_FP32 ? 4 : 8));
   b += 2 * align256(Nlm * 2 * sizeof(double));
@@ -77,6 +105,7 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
 static void free_workspace(Workspace& ws) {
   for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
+                  (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
                   (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
@@ -97,7 +126,7 @@ static void free_workspace(Workspace& ws) {
 // Grow the workspace to hold a transform of Np points.  Growth waits for the
 // device so buffers still in use by queued work are never freed under it.
 static int ensure_workspace(Plan& pl, int64_t Np) {
-  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk, pl.tile);
+  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
   Workspace& ws = pl.ws;
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
       z.tiles <= ws.cap_tiles)
@@ -114,6 +143,8 @@ static int ensure_workspace(Plan& pl, int64_t Np) {
   DSHT_ALLOC(ws.seg_count, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.seg_run_lo, (pl.nseg + 2) * sizeof(int));
+  DSHT_ALLOC(ws.seg_run_hi, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.ne_rank, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.thist, want.tiles * (pl.nseg + 1) * sizeof(int));
   DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8));
@@ -189,6 +220,7 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   std::vector<GroupDesc> groups;
   std::vector<int64_t> mp_off;
   build_layout(Nl, pl.Npair, groups, mp_off, pl.tab_vec4);
+  groups = xcd_schedule(groups, kXcds);
   pl.ngroups = (int)groups.size();
   Workspace& ws = pl.ws;  // DSHT_ALLOC counts bytes into ws.bytes; moved below
   DSHT_ALLOC(pl.groups, groups.size() * sizeof(GroupDesc));
@@ -439,6 +471,7 @@ int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table
   int64_t vec4 = 0;
   const int Npair = (Nx + 1) / 2;
   build_layout(Nl, Npair, groups, mp_off, vec4);
+  groups = xcd_schedule(groups, kXcds);
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   if (table_bytes)
     *table_bytes = vec4 * 4 * (precision == DSHT_FP32 ? 4 : 8) + Nlm * (3 * 8 + 4) +

@@ -74,7 +74,15 @@ __host__ __device__ inline int64_t hp_index(int ell, int m, int Nl) {
 // 1 = polar (rotated); hemisphere 0: x >= 0, 1: x < 0; bin j in [0, Nx-2].
 // Points that the reference drops (NaN theta) get seg = nseg (sorted last,
 // never scattered).
-__host__ __device__ inline int num_segments(int Nx) { return 4 * (Nx - 1); }
+__host__ __device__ // Input points per counting-sort tile: at least 4096, and no more than ~512
+// tiles (two 1024-thread tile blocks per CU fill the chip in one round).
+inline int sort_tile(int64_t Np) {
+  int64_t t = (Np + 511) / 512;
+  t = (t + 1023) / 1024 * 1024;
+  return (int)(t < 4096 ? 4096 : t);
+}
+
+inline int num_segments(int Nx) { return 4 * (Nx - 1); }
 
 // Per-point record after sorting: one 64-byte line per point, written once by
 // the placement pass and read by the scatter.

@@ -4,7 +4,8 @@
 // of sht/interp_funcs_py.py:46-60 / sht/interp_funcs_jax.py:31-48 and
 // csht/sht_helper.c:255-274, as one HBM-streaming GEMV.
 //
-// One wavefront per row group (up to 64 rows l of one (m, P)); lane r owns row
+// One wavefront per row group (up to 64 rows l of one (m, P)), launched in an
+// XCD-aware order (xcd_schedule in api.hip); lane r owns row
 // l = m + P + 2(i0 + r) and walks the node pairs.  Each step loads one
 // 4-vector {Y(x_2np), Y(x_2np+1), Y'(x_2np), Y'(x_2np+1)} per lane (a
 // contiguous 1 KB (FP32) / 2 KB (FP64) block per wavefront) and multiplies it
@@ -25,6 +26,7 @@ __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ ta
                                                    int Nl, double* __restrict__ a_equ,
                                                    double* __restrict__ a_pol) {
   const GroupDesc gd = groups[blockIdx.x];
+  if (gd.nrows == 0) return;  // padding of the XCD schedule
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
   const float4* t = tab + gd.off + r;
@@ -65,6 +67,7 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
                                                    int Nl, double* __restrict__ a_equ,
                                                    double* __restrict__ a_pol) {
   const GroupDesc gd = groups[blockIdx.x];
+  if (gd.nrows == 0) return;  // padding of the XCD schedule
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
   const double4* t = tab + gd.off + r;

@@ -10,83 +10,126 @@
 // and with s = c+ + c-, delta = c+ - c- the rows with even l-m contract s and
 // the rows with odd l-m contract delta -- one table read for both hemispheres.
 //
-// One block per (node n, branch); threads over m.  The runs of a segment are
-// contiguous in run order (runs are numbered by position in the sorted
-// array); they are found by binary search in run_seg and summed in order.
-// Output rhs[m][P][np][16] (P = parity class, np = node pair), as the
-// contraction reads it: for branch b, 8 values at offset 8b:
-//   re: c(2np), c(2np+1), d(2np), d(2np+1);  im: the same four.
+// The runs of a segment are contiguous in run order (runs are numbered by
+// position in the sorted array); k_seg_runs records each segment's run range
+// and k_fold sums them in order (deterministic).  k_fold: one thread per
+// (order m, node pair np) for both branches; the 64 lanes of a wavefront take
+// 64 consecutive m, so every partial it reads is one 512-byte (FP32) block,
+// and each thread writes the whole 16-value row rhs[m][P][np][16] (64 bytes,
+// FP32) that the contraction reads:
+//   branch b, offset 8b:  re: c(2np), c(2np+1), d(2np), d(2np+1);  im: same.
 #include "common.h"
 #include "plan.h"
 
 namespace dsht {
 
-__device__ inline int lower_bound_int(const int* a, int n, int v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
+// Every segment's [first run, last run + 1); segments without runs keep (0, 0).
+__global__ __launch_bounds__(256) void k_seg_clear(int nseg, int* __restrict__ lo,
+                                                   int* __restrict__ hi) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) lo[s] = hi[s] = 0;
 }
 
+__global__ __launch_bounds__(256) void k_seg_runs(const int* __restrict__ run_seg,
+                                                  const int* __restrict__ run_total,
+                                                  int* __restrict__ lo, int* __restrict__ hi) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nr = *run_total;
+  if (r >= nr) return;
+  const int s = run_seg[r];
+  if (r == 0 || run_seg[r - 1] != s) lo[s] = r;
+  if (r == nr - 1 || run_seg[r + 1] != s) hi[s] = r + 1;
+}
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
+constexpr int kFoldM = 64;   // orders per block (one wavefront)
+constexpr int kFoldNp = 4;   // node pairs per block (one wavefront each)
+
 template <typename TP, typename T>
-__global__ __launch_bounds__(256) void k_fold(const TP* __restrict__ part,
-                                              const int* __restrict__ run_seg,
-                                              const int* __restrict__ run_total, int Nl, int Nx,
-                                              int Npair, double dx, T* __restrict__ rhs) {
-  const int n = blockIdx.x, br = blockIdx.y;
-  const int np = n >> 1, odd = n & 1;
-  __shared__ int rr[4][2];
-  if (threadIdx.x < 4) {
-    const int h = threadIdx.x >> 1;       // hemisphere
-    const int j = n - (threadIdx.x & 1);  // bin n (v0,v1) or n-1 (v2,v3)
-    int lo = 0, hi = 0;
-    if (j >= 0 && j <= Nx - 2) {
-      const int s = (br * 2 + h) * (Nx - 1) + j;
-      const int nr = *run_total;
-      lo = lower_bound_int(run_seg, nr, s);
-      hi = lower_bound_int(run_seg, nr, s + 1);
-    }
-    rr[threadIdx.x][0] = lo;
-    rr[threadIdx.x][1] = hi;
-  }
-  __syncthreads();
-  for (int m = threadIdx.x; m < Nl; m += blockDim.x) {
-    double c[2][2] = {{0, 0}, {0, 0}}, d[2][2] = {{0, 0}, {0, 0}};  // [hemisphere][re/im]
+__global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict__ part,
+                                                           const int* __restrict__ seg_lo,
+                                                           const int* __restrict__ seg_hi,
+                                                           int Nl, int Nx, int Npair, double dx,
+                                                           T* __restrict__ rhs) {
+  using V = typename Vec2<TP>::type;
+  const int m = blockIdx.y * kFoldM + (threadIdx.x & (kFoldM - 1));
+  const int np = blockIdx.x * kFoldNp + threadIdx.x / kFoldM;
+  if (m >= Nl || np >= Npair) return;
+  const V* pv = reinterpret_cast<const V*>(part);
+  T out[2][16];  // [P][16]
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int h = q >> 1, half = q & 1;  // half 1: v2,v3 of bin n-1
-      for (int r = rr[q][0]; r < rr[q][1]; ++r) {
-        const TP* v = part + (((int64_t)r * 2 + half) * Nl + m) * 4;
-        c[h][0] += (double)v[0];
-        c[h][1] += (double)v[1];
-        d[h][0] += (double)v[2];
-        d[h][1] += (double)v[3];
-      }
+  for (int br = 0; br < 2; ++br) {
+    // c, d per (node of the pair, hemisphere), complex
+    double c[2][2][2] = {}, d[2][2][2] = {};
+#pragma unroll
+    for (int odd = 0; odd < 2; ++odd) {
+      const int n = 2 * np + odd;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          const int j = n - half;  // bin n (v0, v1) or bin n-1 (v2, v3)
+          if (j < 0 || j > Nx - 2) continue;
+          const int s = (br * 2 + h) * (Nx - 1) + j;
+          const int r0 = seg_lo[s], r1 = seg_hi[s];
+          for (int r = r0; r < r1; ++r) {
+            const V vc = pv[((int64_t)r * 4 + 2 * half) * Nl + m];
+            const V vd = pv[((int64_t)r * 4 + 2 * half + 1) * Nl + m];
+            c[odd][h][0] += (double)vc.x;
+            c[odd][h][1] += (double)vc.y;
+            d[odd][h][0] += (double)vd.x;
+            d[odd][h][1] += (double)vd.y;
+          }
+        }
     }
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
       const double sg = P ? -1.0 : 1.0;
-      T* o = rhs + (((int64_t)m * 2 + P) * Npair + np) * 16 + 8 * br;
-      o[odd] = (T)(c[0][0] + sg * c[1][0]);
-      o[2 + odd] = (T)(dx * (d[0][0] + sg * d[1][0]));
-      o[4 + odd] = (T)(c[0][1] + sg * c[1][1]);
-      o[6 + odd] = (T)(dx * (d[0][1] + sg * d[1][1]));
+#pragma unroll
+      for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+        for (int odd = 0; odd < 2; ++odd) {
+          out[P][8 * br + 4 * ri + odd] = (T)(c[odd][0][ri] + sg * c[odd][1][ri]);
+          out[P][8 * br + 4 * ri + 2 + odd] = (T)(dx * (d[odd][0][ri] + sg * d[odd][1][ri]));
+        }
     }
+  }
+#pragma unroll
+  for (int P = 0; P < 2; ++P) {
+    T* o = rhs + (((int64_t)m * 2 + P) * Npair + np) * 16;
+#pragma unroll
+    for (int q = 0; q < 16; q += 2) *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
   }
 }
 
 int launch_fold(Plan& pl, hipStream_t s) {
-  dim3 grid(2 * pl.Npair, 2);
-  const int* run_total = pl.ws.run_base + pl.ws.cur_chunks;  // scan total = number of runs
+  Workspace& ws = pl.ws;
+  const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
+  k_seg_clear<<<(pl.nseg + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi);
+  DSHT_LAUNCH_CHECK();
+  const int64_t max_runs = (int64_t)ws.cur_chunks + pl.nseg + 1;
+  k_seg_runs<<<(int)((max_runs + 255) / 256), 256, 0, s>>>(ws.run_seg, run_total, ws.seg_run_lo,
+                                                          ws.seg_run_hi);
+  DSHT_LAUNCH_CHECK();
+  dim3 grid((pl.Npair + kFoldNp - 1) / kFoldNp, (pl.Nl + kFoldM - 1) / kFoldM);
   if (pl.precision == DSHT_FP32) {
-    k_fold<float, float><<<grid, 256, 0, s>>>((const float*)pl.ws.part, pl.ws.run_seg, run_total,
-                                              pl.Nl, pl.Nx, pl.Npair, pl.dx, (float*)pl.ws.rhs);
+    k_fold<float, float><<<grid, kFoldM * kFoldNp, 0, s>>>(
+        (const float*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx,
+        (float*)ws.rhs);
   } else {
-    k_fold<double, double><<<grid, 256, 0, s>>>((const double*)pl.ws.part, pl.ws.run_seg, run_total,
-                                                pl.Nl, pl.Nx, pl.Npair, pl.dx, (double*)pl.ws.rhs);
+    k_fold<double, double><<<grid, kFoldM * kFoldNp, 0, s>>>(
+        (const double*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx,
+        (double*)ws.rhs);
   }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

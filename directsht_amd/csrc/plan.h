@@ -17,11 +17,13 @@ struct Workspace {
   int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
   int* run_base = nullptr;    // [cap_chunks + 1]: exclusive scan of run_count
   int* run_seg = nullptr;     // [cap_runs]: segment of each run
-  void* part = nullptr;       // [cap_runs][2][Nl][4]: per-run (v0,v1 | v2,v3), complex;
+  void* part = nullptr;       // [cap_runs][4][Nl][2]: per-run v_k(m), k = 0..3, complex;
                               // double (FP64 scatter) or float (FP32 scatter)
   int* seg_count = nullptr;   // [nseg + 2]
   int* seg_start = nullptr;   // [nseg + 2]
   int* seg_cursor = nullptr;  // [nseg + 2]
+  int* seg_run_lo = nullptr;  // [nseg + 2]: first run of each segment (0 if empty)
+  int* seg_run_hi = nullptr;  // [nseg + 2]: one past its last run (0 if empty)
   int* ne_rank = nullptr;     // [nseg + 1]: non-empty segments before s
   int* thist = nullptr;       // [cap_tiles][nseg + 1]: tile histograms, then offsets
   void* rhs = nullptr;        // [Nl][2][Npair][16] float or double
@@ -39,7 +41,6 @@ struct Plan {
   int precision = DSHT_FP64;
   // scatter decomposition
   int chunk = 2048;  // sorted points per scatter work item
-  int tile = 16384;  // input points per counting-sort tile
   int scatter = DSHT_SCATTER_MATRIX;  // engine of FP32 plans
   // contraction layout
   int ngroups = 0;
@@ -70,6 +71,8 @@ struct Plan {
 // Host-side layout computation shared by plan creation and estimates.
 void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups,
                   std::vector<int64_t>& mp_off, int64_t& tab_vec4);
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X (one L2 each)
+std::vector<GroupDesc> xcd_schedule(const std::vector<GroupDesc>& groups, int nxcd);
 int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk);
 
 // Kernel launchers (one per source file).

@@ -217,33 +217,51 @@ __global__ __launch_bounds__(256) void k_count_runs(const int* __restrict__ seg_
 }
 
 // ---- tile-based counting sort (no global atomics on segment counters) ----
-// Points are cut into tiles of `tile` consecutive input points.  Pass 1 writes
-// each tile's LDS histogram as a row of thist[tile][seg]; the column pass turns
-// the rows into per-(tile, segment) write offsets (tiles in order, after the
-// segment's start); pass 2 re-classifies the tile and places each record with
-// an LDS atomic on the tile's own offset row.
-__global__ __launch_bounds__(256) void k_tile_hist(int64_t Np, int tile,
-                                                   const double* __restrict__ theta,
-                                                   const double* __restrict__ phi,
-                                                   const double* __restrict__ wt, PointParams p,
-                                                   int* __restrict__ thist, int* __restrict__ flags) {
+// Points are cut into tiles of `tile` consecutive input points (sort_tile():
+// at most ~512 tiles, so the 1024-thread tile blocks fill the chip in one
+// round).  Pass 1 writes each tile's LDS histogram as a row of
+// thist[tile][seg]; the column pass turns the rows into per-(tile, segment)
+// write offsets (tiles in order, after the segment's start); pass 2
+// re-classifies the tile and places each record with an LDS atomic on the
+// tile's own offset row.  Each thread takes kSortBatch points per step so
+// their loads are in flight together.
+constexpr int kSortThreads = 1024;
+constexpr int kSortBatch = 4;
+
+__global__ __launch_bounds__(kSortThreads) void k_tile_hist(int64_t Np, int tile,
+                                                            const double* __restrict__ theta,
+                                                            const double* __restrict__ phi,
+                                                            PointParams p, int* __restrict__ thist,
+                                                            int* __restrict__ flags) {
   extern __shared__ int lh[];
   const int nb = p.nseg + 1;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) lh[i] = 0;
+  for (int i = threadIdx.x; i < nb; i += kSortThreads) lh[i] = 0;
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
   int flag = 0, npol = 0;
-  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-    Classified c = classify(theta[i], phi[i], wt[i], p);
-    flag |= c.flag;
-    npol += c.polar;
-    atomicAdd(&lh[c.seg], 1);
+  for (int64_t i0 = b + threadIdx.x; i0 < e; i0 += kSortThreads * kSortBatch) {
+    double th[kSortBatch], ph[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) {
+      const int64_t i = i0 + (int64_t)u * kSortThreads;
+      th[u] = i < e ? theta[i] : 0.0;
+      ph[u] = i < e ? phi[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) {
+      if (i0 + (int64_t)u * kSortThreads < e) {
+        const Classified c = classify(th[u], ph[u], 0.0, p);  // weights do not move points
+        flag |= c.flag;
+        npol += c.polar;
+        atomicAdd(&lh[c.seg], 1);
+      }
+    }
   }
   if (flag) atomicOr(&flags[0], flag);
   if (npol) atomicAdd(&flags[1], npol);
   __syncthreads();
   int* row = thist + (int64_t)blockIdx.x * nb;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) row[i] = lh[i];
+  for (int i = threadIdx.x; i < nb; i += kSortThreads) row[i] = lh[i];
 }
 
 // Column passes over thist: 16 segments per block, each column split over 16
@@ -297,26 +315,40 @@ __global__ __launch_bounds__(256) void k_col_offsets(int* __restrict__ thist, in
   }
 }
 
-__global__ __launch_bounds__(256) void k_tile_place(int64_t Np, int tile,
-                                                    const double* __restrict__ theta,
-                                                    const double* __restrict__ phi,
-                                                    const double* __restrict__ wt, PointParams p,
-                                                    const int* __restrict__ toff,
-                                                    PointRec* __restrict__ rec) {
+__global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int tile,
+                                                             const double* __restrict__ theta,
+                                                             const double* __restrict__ phi,
+                                                             const double* __restrict__ wt,
+                                                             PointParams p,
+                                                             const int* __restrict__ toff,
+                                                             PointRec* __restrict__ rec) {
   extern __shared__ int lo[];
   const int nb = p.nseg + 1;
   const int* row = toff + (int64_t)blockIdx.x * nb;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) lo[i] = row[i];
+  for (int i = threadIdx.x; i < nb; i += kSortThreads) lo[i] = row[i];
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
-  for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-    Classified c = classify(theta[i], phi[i], wt[i], p);
-    const int pos = atomicAdd(&lo[c.seg], 1);
-    double2* d = reinterpret_cast<double2*>(rec + pos);
-    d[0] = make_double2(c.ere, c.eim);
-    d[1] = make_double2(c.h0, c.h1);
-    d[2] = make_double2(c.h2, c.h3);
-    reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, 0);
+  for (int64_t i0 = b + threadIdx.x; i0 < e; i0 += kSortThreads * kSortBatch) {
+    double th[kSortBatch], ph[kSortBatch], ww[kSortBatch];
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) {
+      const int64_t i = i0 + (int64_t)u * kSortThreads;
+      th[u] = i < e ? theta[i] : 0.0;
+      ph[u] = i < e ? phi[i] : 0.0;
+      ww[u] = i < e ? wt[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSortBatch; ++u) {
+      if (i0 + (int64_t)u * kSortThreads < e) {
+        const Classified c = classify(th[u], ph[u], ww[u], p);
+        const int pos = atomicAdd(&lo[c.seg], 1);
+        double2* d = reinterpret_cast<double2*>(rec + pos);
+        d[0] = make_double2(c.ere, c.eim);
+        d[1] = make_double2(c.h0, c.h1);
+        d[2] = make_double2(c.h2, c.h3);
+        reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, 0);
+      }
+    }
   }
 }
 
@@ -328,14 +360,15 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
   DSHT_HIP(hipMemsetAsync(ws.flags, 0, sizeof(int) * 4, s));
   const int nb = pl.nseg + 1;
   const size_t lds = sizeof(int) * nb;
-  const int ntiles = (int)((Np + pl.tile - 1) / pl.tile);
+  const int tile = sort_tile(Np);
+  const int ntiles = (int)((Np + tile - 1) / tile);
   const bool tiled = lds <= 64 * 1024 && ntiles <= ws.cap_tiles;
   const int threads = 256;
   const int64_t want = (Np + threads - 1) / threads;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
   if (Np > 0) {
     if (tiled) {
-      k_tile_hist<<<ntiles, threads, lds, s>>>(Np, pl.tile, theta, phi, wt, p, ws.thist, ws.flags);
+      k_tile_hist<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, p, ws.thist, ws.flags);
       DSHT_LAUNCH_CHECK();
       k_col_sum<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_count);
     } else if (lds <= 64 * 1024) {
@@ -353,7 +386,7 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
     if (tiled) {
       k_col_offsets<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_start);
       DSHT_LAUNCH_CHECK();
-      k_tile_place<<<ntiles, threads, lds, s>>>(Np, pl.tile, theta, phi, wt, p, ws.thist, ws.rec);
+      k_tile_place<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, wt, p, ws.thist, ws.rec);
     } else {
       k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec);
     }

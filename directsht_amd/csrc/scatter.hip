@@ -25,9 +25,9 @@
 //         FP64).
 //
 // A run is a maximal stretch of one segment inside one chunk; each run's sums
-// go to their own slot part[run][half][m][4] (half 0: v0, v1; half 1: v2, v3;
-// complex), with no atomics, and the fold stage adds the runs of a segment in
-// a fixed order, so the reduction is deterministic.
+// go to their own slot part[run][k][m][2] (v_k, complex), with no atomics,
+// and the fold stage adds the runs of a segment in a fixed order, so the
+// reduction is deterministic.
 #include "common.h"
 #include "plan.h"
 
@@ -108,12 +108,10 @@ __global__ __launch_bounds__(64) void k_scatter64(const PointRec* __restrict__ r
     for (int k = 0; k < K; ++k) {
       const int m = mlane + k;
       if (m < Nl) {
-        double2* lo = reinterpret_cast<double2*>(part + (((int64_t)run * 2 + 0) * Nl + m) * 4);
-        double2* hi = reinterpret_cast<double2*>(part + (((int64_t)run * 2 + 1) * Nl + m) * 4);
-        lo[0] = make_double2(acc[k][0], acc[k][1]);
-        lo[1] = make_double2(acc[k][2], acc[k][3]);
-        hi[0] = make_double2(acc[k][4], acc[k][5]);
-        hi[1] = make_double2(acc[k][6], acc[k][7]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          *reinterpret_cast<double2*>(part + (((int64_t)run * 4 + q) * Nl + m) * 2) =
+              make_double2(acc[k][2 * q], acc[k][2 * q + 1]);
       }
     }
     if (lane == 0 && mt == 0) run_seg[run] = cur;
@@ -208,10 +206,10 @@ __global__ __launch_bounds__(64) void k_scatter32(const PointRec* __restrict__ r
     for (int k = 0; k < K; ++k) {
       const int m = mlane + k;
       if (m < Nl) {
-        float4* lo = reinterpret_cast<float4*>(part + (((int64_t)run * 2 + 0) * Nl + m) * 4);
-        float4* hi = reinterpret_cast<float4*>(part + (((int64_t)run * 2 + 1) * Nl + m) * 4);
-        *lo = make_float4(acc[k][0], acc[k][1], acc[k][2], acc[k][3]);
-        *hi = make_float4(acc[k][4], acc[k][5], acc[k][6], acc[k][7]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + q) * Nl + m) * 2) =
+              make_float2(acc[k][2 * q], acc[k][2 * q + 1]);
       }
     }
     if (lane == 0 && mt == 0) run_seg[run] = cur;

@@ -27,7 +27,8 @@
 // window that contains a segment boundary is issued once per run, with the
 // points of other runs masked to zero in A.  Wave w owns row tile w (k = w/2,
 // 16 values of m1) and issues 6 MFMAs per window.  A run's result goes to its
-// own slot part[run][half][m][4] (FP32), as for the vector scatter.
+// own slot part[run][k][m][2] (FP32), as for the vector scatter; the 32 lanes
+// of a half-wave write 32 consecutive m (256 contiguous bytes).
 //
 // Operand layouts of v_mfma_f32_32x32x16_f16 (checked on gfx950):
 //   A: lane l holds A[l%32][8*(l/32) + j],  j = 0..7
@@ -275,8 +276,8 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
                 const floatx16& r1 = kGroup > 0 ? out[1] : acc[1];
                 const float vre = r0[8 * g + d] - r1[8 * g + 4 + d];
                 const float vim = r1[8 * g + d] + r0[8 * g + 4 + d];
-                *reinterpret_cast<float2*>(part + (((int64_t)run * 2 + (kA >> 1)) * Nl + m) * 4 +
-                                           2 * (kA & 1)) = make_float2(vre, vim);
+                *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + kA) * Nl + m) * 2) =
+                    make_float2(vre, vim);
               }
             }
           if (tid == 0 && mt == 0) run_seg[run] = cur;
