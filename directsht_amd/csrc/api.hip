@@ -242,6 +242,9 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   DSHT_HIP(hipMemcpy(pl.rot_iB, iB.data(), iB.size() * sizeof(double), hipMemcpyHostToDevice));
   pl.table_bytes = ws.bytes;
   ws.bytes = 0;
+  pl.table_hold = std::make_shared<TableOwner>(
+      pl.device, std::vector<void*>{pl.groups, pl.mp_off, pl.tab64, pl.tab32, pl.rot_top,
+                                    pl.rot_exp, pl.rot_AB, pl.rot_iB});
   if (alloc_tables && build) {
     int rc = launch_tables(pl, precision == DSHT_FP64, precision == DSHT_FP32, nullptr, nullptr, 0);
     if (rc) return rc;
@@ -253,9 +256,13 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
 static void plan_free(Plan& pl) {
   free_workspace(pl.ws);
   for (hipEvent_t e : pl.events) (void)hipEventDestroy(e);
-  for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64, (void*)pl.tab32,
-                  (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB, (void*)pl.rot_iB})
-    if (p) (void)hipFree(p);
+  if (pl.table_hold) {
+    pl.table_hold.reset();  // frees the tables with the last plan sharing them
+  } else {                  // plan_init failed before handing them to an owner
+    for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64, (void*)pl.tab32,
+                    (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB, (void*)pl.rot_iB})
+      if (p) (void)hipFree(p);
+  }
   pl = Plan{};
 }
 
@@ -449,6 +456,52 @@ int dsht_plan_destroy(dsht_plan* p) {
   plan_free(p->pl);
   delete p;
   return DSHT_OK;
+}
+
+int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
+  if (!src || !out) {
+    set_error("dsht_plan_clone: null argument");
+    return DSHT_EINVAL;
+  }
+  dsht_plan* p = new dsht_plan();
+  Plan& d = p->pl;
+  const Plan& s = src->pl;
+  d.device = s.device;
+  d.Nl = s.Nl;
+  d.Nx = s.Nx;
+  d.Npair = s.Npair;
+  d.nseg = s.nseg;
+  d.Nlm = s.Nlm;
+  d.xmax = s.xmax;
+  d.dx = s.dx;
+  d.acos_xmax = s.acos_xmax;
+  d.acos_mxmax = s.acos_mxmax;
+  d.precision = s.precision;
+  d.chunk = s.chunk;
+  d.scatter = s.scatter;
+  d.ngroups = s.ngroups;
+  d.groups = s.groups;
+  d.mp_off = s.mp_off;
+  d.tab_vec4 = s.tab_vec4;
+  d.tab64 = s.tab64;
+  d.tab32 = s.tab32;
+  d.table_bytes = s.table_bytes;
+  d.rot_top = s.rot_top;
+  d.rot_exp = s.rot_exp;
+  d.rot_AB = s.rot_AB;
+  d.rot_iB = s.rot_iB;
+  d.table_hold = s.table_hold;
+  *out = p;
+  return DSHT_OK;
+}
+
+int dsht_plan_reserve(dsht_plan* p, int64_t Np) {
+  if (!p || Np < 0 || Np >= (int64_t)1 << 31) {
+    set_error("dsht_plan_reserve: null plan or Np outside [0, 2^31)");
+    return DSHT_EINVAL;
+  }
+  DeviceGuard g(p->pl.device);
+  return ensure_workspace(p->pl, Np);
 }
 
 int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {

@@ -1,6 +1,7 @@
 // Plan: resident tables + per-transform workspace of one DirectSHT(Nl, Nx, xmax).
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "common.h"
@@ -33,6 +34,23 @@ struct Workspace {
   int64_t bytes = 0;
 };
 
+// Frees a plan's table arrays (on the plan's device) when destroyed.
+struct TableOwner {
+  int device;
+  std::vector<void*> ptrs;
+  TableOwner(int dev, std::vector<void*> p) : device(dev), ptrs(std::move(p)) {}
+  TableOwner(const TableOwner&) = delete;
+  TableOwner& operator=(const TableOwner&) = delete;
+  ~TableOwner() {
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != device) (void)hipSetDevice(device);
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
+  }
+};
+
 struct Plan {
   int device = 0;
   int Nl = 0, Nx = 0, Npair = 0, nseg = 0;
@@ -56,6 +74,9 @@ struct Plan {
   int* rot_exp = nullptr;
   double* rot_AB = nullptr;
   double* rot_iB = nullptr;
+  // Owner of the device arrays above (groups .. rot_iB); shared by the clones
+  // of a plan (dsht_plan_clone), which free them with the last of them.
+  std::shared_ptr<TableOwner> table_hold;
   Workspace ws;
   // stage timing (dsht_plan_set_timing): a pool of event sets reused round
   // robin; a set is harvested before reuse and by dsht_plan_get_timing

@@ -352,12 +352,21 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
   }
 }
 
+__global__ __launch_bounds__(256) void k_clear2(int* __restrict__ a, int na, int* __restrict__ b,
+                                                int nb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0;
+  if (i < nb) b[i] = 0;
+}
+
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, hipStream_t s) {
   Workspace& ws = pl.ws;
   PointParams p{pl.Nx, pl.nseg, mode, pl.xmax, pl.dx, pl.acos_xmax, pl.acos_mxmax};
-  DSHT_HIP(hipMemsetAsync(ws.seg_count, 0, sizeof(int) * (pl.nseg + 2), s));
-  DSHT_HIP(hipMemsetAsync(ws.flags, 0, sizeof(int) * 4, s));
+  // counters are cleared by a kernel, not a memset node, so the transform can
+  // be captured in a HIP graph and replayed
+  k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, ws.flags, 4);
+  DSHT_LAUNCH_CHECK();
   const int nb = pl.nseg + 1;
   const size_t lds = sizeof(int) * nb;
   const int tile = sort_tile(Np);

@@ -72,6 +72,44 @@ class DirectSHT:
                 pass
             self._plan = None
 
+    # ---- concurrency helpers (no counterpart in the reference) ------------
+    def clone(self):
+        """A DirectSHT over the same resident tables with its own workspace:
+        transforms of the clone and of the original may run concurrently on
+        different streams.  The tables are freed with the last sharer."""
+        other = object.__new__(DirectSHT)
+        other.__dict__.update({k: v for k, v in self.__dict__.items()
+                               if k not in ("_plan", "_status", "_Yv", "_Yd")})
+        plan = ct.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.load().dsht_plan_clone(self._plan, ct.byref(plan)), "dsht_plan_clone")
+        other._plan = plan
+        other._status = torch.zeros(1, dtype=torch.int32, device=self.device)
+        other._Yv = other._Yd = None
+        return other
+
+    def reserve(self, npoints):
+        """Size the workspace for transforms of up to ``npoints`` points now,
+        so later transforms neither allocate nor synchronize."""
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.load().dsht_plan_reserve(self._plan, int(npoints)), "dsht_plan_reserve")
+
+    def footprint(self):
+        """HBM bytes held: tables (shared with clones) + own workspace."""
+        b = ct.c_int64()
+        _lib.check(_lib.load().dsht_plan_footprint(self._plan, ct.byref(b)), "dsht_plan_footprint")
+        return b.value
+
+    @staticmethod
+    def estimate(Nell, Nx, npoints, dflt_type="float32"):
+        """(table_bytes, workspace_bytes) a plan and a transform of
+        ``npoints`` points need -- what a server admits against."""
+        prec = _lib.DSHT_FP64 if dflt_type == "float64" else _lib.DSHT_FP32
+        tb, wb = ct.c_int64(), ct.c_int64()
+        _lib.check(_lib.load().dsht_plan_estimate(int(Nell), int(Nx), prec, int(npoints),
+                                                  ct.byref(tb), ct.byref(wb)), "dsht_plan_estimate")
+        return tb.value, wb.value
+
     # ---- table access (sht/sht.py:56,69-93) -----------------------------
     def _fetch_tables(self):
         Yv = np.empty((self.Nlm, self.Nx), dtype=np.float64)

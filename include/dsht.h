@@ -90,7 +90,21 @@ typedef struct dsht_plan dsht_plan;
 int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** plan);
 int dsht_plan_destroy(dsht_plan* plan);
 
-/* Bytes of HBM the plan holds (tables + current workspace). */
+/* A second plan over the same resident tables, with a workspace of its own,
+ * so transforms at the same (Nl, Nx) can run concurrently on different
+ * streams (the reference runs one transform at a time; this extends the plan
+ * API for servers and batched callers).  The tables are freed with the last
+ * plan that shares them. */
+int dsht_plan_clone(const dsht_plan* src, dsht_plan** clone);
+
+/* Grow the plan's workspace now to hold a transform of Np points, so that
+ * later dsht_transform calls with at most Np points allocate nothing and
+ * never synchronize (what HIP graph capture and latency-bound callers need).
+ * Growth waits for the device. */
+int dsht_plan_reserve(dsht_plan* plan, int64_t Np);
+
+/* Bytes of HBM the plan holds: its tables (shared with its clones, see
+ * dsht_plan_clone) + its own current workspace. */
 int dsht_plan_footprint(const dsht_plan* plan, int64_t* bytes);
 
 /* HBM a plan would need for its tables and for a transform of Np points
@@ -107,8 +121,9 @@ int dsht_plan_get_tables(dsht_plan* plan, double* Yv, double* Yd);
  * healpy order, overwritten.  status: device int written with DSHT_STATUS_*
  * bits (may be NULL).  mode: DSHT_MODE_CALL or DSHT_MODE_BASIC.  Work is
  * enqueued on `stream` (a hipStream_t; 0 = default stream) and the call
- * returns without synchronizing.  Concurrent transforms on one plan must be
- * serialized by the caller (they share the plan's workspace). */
+ * returns without synchronizing (unless the workspace has to grow; see
+ * dsht_plan_reserve).  Concurrent transforms on one plan must be serialized
+ * by the caller (they share the plan's workspace); clones do not share it. */
 int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
                    const double* wt, double* alm, int* status, int mode, void* stream);
 

@@ -1,0 +1,148 @@
+"""Shared-table clones, graph-captured batches and the transform server.
+
+The bar (north star): every result of a batched or served transform equals
+the same transform run alone, within the precision's tolerance (FP32 1e-6,
+FP64 1e-11 relative to max|a|), and matches the FP64 oracle.  Graph replays
+are checked one by one against the eager result, with the program's own
+eager work and synchronizations between replays.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"float64": 1e-11, "float32": 1e-6}
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def rel(a, b):
+    a = a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)
+    b = b.cpu().numpy() if hasattr(b, "cpu") else np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def uniform_sets(torch, seed, sizes, device="cuda"):
+    g = torch.Generator(device=device).manual_seed(seed)
+    out = []
+    for n in sizes:
+        th = torch.acos(2 * torch.rand(n, generator=g, device=device, dtype=torch.float64) - 1)
+        ph = 2 * math.pi * torch.rand(n, generator=g, device=device, dtype=torch.float64)
+        w = 0.5 + torch.rand(n, generator=g, device=device, dtype=torch.float64)
+        out.append((th, ph, w))
+    return out
+
+
+@pytest.mark.parametrize("dflt_type", ["float64", "float32"])
+def test_clone_matches_original(torch_cuda, dflt_type):
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    (th, ph, w), = uniform_sets(torch, 1, [30000])
+    base = DirectSHT(96, 192, dflt_type=dflt_type)
+    a0 = base(th, ph, w).clone()
+    c = base.clone()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a1, _ = c.transform_device(th, ph, w, stream=s)
+        a2, _ = base.transform_device(th, ph, w)  # concurrent, other workspace
+    torch.cuda.synchronize()
+    assert rel(a1, a0) < TOL[dflt_type]
+    assert rel(a2, a0) < TOL[dflt_type]
+    del base  # the clone keeps the tables alive
+    a3 = c(th, ph, w)
+    assert rel(a3, a0) < TOL[dflt_type]
+
+
+@pytest.mark.parametrize("dflt_type", ["float64", "float32"])
+def test_batch_graph_replays_match_alone(torch_cuda, dflt_type):
+    torch = torch_cuda
+    from directsht_amd.batch import Autotuner, BatchTransform
+    from directsht_amd.sht import DirectSHT
+    rng = np.random.default_rng(7)
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(1e3), np.log(2e5), 12))]
+    nls = [int(x) for x in rng.choice([32, 64, 128], len(sizes))]
+    data = uniform_sets(torch, 2, sizes)
+    sets = [d + (nl,) for d, nl in zip(data, nls)]
+    tuner = Autotuner(candidates=(("matrix", 2048), ("vector", 2048)))
+    bt = BatchTransform(sets, dflt_type=dflt_type, nstreams=3, tuner=tuner)
+    alone = [DirectSHT(nl, 2 * nl, dflt_type=dflt_type)(*d) for d, nl in zip(data, nls)]
+    eager = [a.clone() for a in bt.run_eager()]
+    torch.cuda.synchronize()
+    for i in range(len(sets)):
+        assert rel(eager[i], alone[i]) < TOL[dflt_type], i
+    for rep in range(3):
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        outs = bt.run()
+        torch.cuda.synchronize()
+        float(junk)
+        for i in range(len(sets)):
+            assert rel(outs[i], eager[i]) < TOL[dflt_type], (rep, i)
+
+
+def test_batch_matches_oracle(torch_cuda):
+    torch = torch_cuda
+    from oracle import refalg
+    from directsht_amd.batch import BatchTransform
+    data = uniform_sets(torch, 3, [4000, 9000])
+    sets = [data[0] + (24,), data[1] + (40,)]
+    bt = BatchTransform(sets, dflt_type="float64", nstreams=2)
+    outs = bt.run()
+    torch.cuda.synchronize()
+    for (th, ph, w), nl, a in zip(data, (24, 40), outs):
+        tables = refalg.make_tables(nl, 2 * nl, 0.75)
+        ref = refalg.direct_sht(tables, th.cpu().numpy(), ph.cpu().numpy(), w.cpu().numpy())
+        assert rel(a, ref) < 1e-11
+
+
+def test_server_alone_vs_mix(torch_cuda):
+    torch = torch_cuda
+    from directsht_amd.serve import TransformServer
+    from directsht_amd.sht import DirectSHT
+    rng = np.random.default_rng(11)
+    n_req = 24
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(1e3), np.log(3e5), n_req))]
+    nls = [int(x) for x in rng.choice([32, 64, 96], n_req)]
+    data = uniform_sets(torch, 5, sizes)
+    host = [tuple(a.cpu().numpy() for a in d) for d in data]
+    alone = {}
+    for nl in set(nls):
+        alone[nl] = DirectSHT(nl, 2 * nl, dflt_type="float32")
+    ref = [alone[nl](*d) for d, nl in zip(data, nls)]
+    with TransformServer(max_inflight=4, dflt_type="float32") as srv:
+        futs = []
+        for i, (d, h, nl) in enumerate(zip(data, host, nls)):
+            futs.append(srv.submit(*(h if i % 2 else d), Nl=nl))  # host and device inputs
+        res = [f.result(timeout=120) for f in futs]
+        st = srv.stats()
+    for i, (a, r) in enumerate(zip(res, ref)):
+        assert rel(a, r) < TOL["float32"], i
+    assert st["requests"] == n_req and st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_server_admission_by_hbm(torch_cuda):
+    torch = torch_cuda
+    from directsht_amd.serve import TransformServer
+    from directsht_amd.sht import DirectSHT
+    data = uniform_sets(torch, 6, [200000] * 6)
+    tb, wb = DirectSHT.estimate(64, 128, 200000, "float32")
+    need = tb + wb + 16 * 64 * 65 // 2
+    # room for the tables and about two requests at a time
+    with TransformServer(hbm_budget=tb + 2 * (wb + 16 * 64 * 65 // 2) + (wb // 2),
+                         max_inflight=4, dflt_type="float32") as srv:
+        futs = [srv.submit(*d, Nl=64) for d in data]
+        res = [f.result(timeout=120) for f in futs]
+        big = srv.submit(*uniform_sets(torch, 7, [10])[0], Nl=2000)  # tables alone exceed the budget
+        with pytest.raises(MemoryError):
+            big.result(timeout=60)
+    ref = DirectSHT(64, 128, dflt_type="float32")
+    for d, a in zip(data, res):
+        assert rel(a, ref(*d)) < TOL["float32"]
+    assert need > 0
