@@ -22,6 +22,7 @@ DSHT_STATUS_EQU_RANGE, DSHT_STATUS_POLAR_RANGE = 1, 2
 STAGES = ("points", "scatter", "fold", "contract", "rotate")
 DSHT_OPT_SCATTER, DSHT_OPT_CHUNK = 1, 2
 DSHT_SCATTER_MATRIX, DSHT_SCATTER_VECTOR = 0, 1
+MAX_SETS = 64  # sets per batched call (dsht_transform_sets / dsht_transform_many)
 
 # every entry point of include/dsht.h: name -> (restype, argtypes)
 _dp = ct.POINTER(ct.c_double)
@@ -33,12 +34,16 @@ SIGNATURES = {
     "dsht_plan_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_double, ct.c_int, ct.POINTER(_vp)]),
     "dsht_plan_destroy": (ct.c_int, [_vp]),
     "dsht_plan_clone": (ct.c_int, [_vp, ct.POINTER(_vp)]),
-    "dsht_plan_reserve": (ct.c_int, [_vp, ct.c_int64]),
+    "dsht_plan_reserve": (ct.c_int, [_vp, ct.c_int64, ct.c_int]),
     "dsht_plan_footprint": (ct.c_int, [_vp, ct.POINTER(ct.c_int64)]),
     "dsht_plan_estimate": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int64,
                                       ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "dsht_plan_get_tables": (ct.c_int, [_vp, _vp, _vp]),
     "dsht_transform": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "dsht_transform_sets": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, ct.c_int, _vp, ct.c_int64, _vp,
+                                       _vp, ct.c_int, _vp]),
+    "dsht_transform_many": (ct.c_int, [_vp, ct.c_int, ct.POINTER(ct.c_int64), ct.POINTER(_vp),
+                                       ct.POINTER(_vp), ct.POINTER(_vp), _vp, _vp, ct.c_int, _vp]),
     "dsht_plan_configure": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
     "dsht_plan_set_timing": (ct.c_int, [_vp, ct.c_int]),
     "dsht_plan_get_timing": (ct.c_int, [_vp, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),

@@ -3,9 +3,13 @@
 A batch is a list of point sets (device-resident theta, phi, w; each with its
 own Nl, Nx).  BatchTransform prepares everything once -- resident tables per
 (Nl, Nx), a tuned configuration per (Nl, size bucket), workspaces reserved to
-size, output buffers -- and captures the whole batch as one HIP graph:
-transforms are spread over several streams (longest first, onto the least
-loaded stream), so small transforms overlap with large ones, and the graph
+size, output buffers -- and captures the whole batch as one HIP graph.  Sets
+that share tables and configuration go through one dsht_transform_many call
+(up to 64 sets): each set is binned, scattered and folded into its own slot,
+and the table contraction (a matrix-core GEMM over all slots) and the polar
+rotation run once for the call, so the tables are streamed from HBM once per
+call instead of once per set.  Calls are spread over several streams
+(longest first, onto the least loaded stream) so they overlap, and the graph
 is replayed with a single launch.  Nothing in a replay allocates or
 synchronizes.
 
@@ -20,6 +24,7 @@ import os
 
 import torch
 
+from . import _lib
 from .sht import DirectSHT
 
 CANDIDATES = (("matrix", 1024), ("matrix", 2048), ("matrix", 4096), ("vector", 2048))
@@ -113,50 +118,68 @@ class BatchTransform:
                 self.bases[(Nl, Nx)] = DirectSHT(Nl, Nx, xmax, dflt_type, device=self.device)
         # tuned configuration per set
         self.config = [self.tuner.choose(self.bases[(s[3], s[4])], s[0].numel()) for s in self.sets]
-        # assignment to streams: longest first onto the least loaded stream
-        cost = [self._cost(s) for s in self.sets]
-        order = sorted(range(len(self.sets)), key=lambda i: -cost[i])
-        self.nstreams = max(1, int(nstreams))
+        # calls: sets with the same tables and configuration, up to MAX_SETS per
+        # dsht_transform_many call (one table stream for the whole call)
+        groups = {}
+        for i, s in enumerate(self.sets):
+            groups.setdefault((s[3], s[4]) + tuple(self.config[i]), []).append(i)
+        self.calls = []
+        for key, idx in groups.items():
+            idx.sort(key=lambda i: -self.sets[i][0].numel())
+            for c0 in range(0, len(idx), _lib.MAX_SETS):
+                self.calls.append((key, idx[c0:c0 + _lib.MAX_SETS]))
+        # assignment of calls to streams: longest first onto the least loaded
+        cost = [self._call_cost(c) for c in self.calls]
+        order = sorted(range(len(self.calls)), key=lambda i: -cost[i])
+        self.nstreams = max(1, min(int(nstreams), len(self.calls)))
         load = [0.0] * self.nstreams
         self.assign = [[] for _ in range(self.nstreams)]
         for i in order:
             j = min(range(self.nstreams), key=lambda q: load[q])
             self.assign[j].append(i)
             load[j] += cost[i]
-        # one workspace per (stream, table, config), reserved for its largest set
+        # one workspace per (stream, tables, configuration), reserved for its
+        # largest set and call
         self.ws = {}
-        for j, idx in enumerate(self.assign):
-            for i in idx:
-                th, _, _, Nl, Nx = self.sets[i]
-                key = (j, Nl, Nx) + tuple(self.config[i])
-                if key not in self.ws:
-                    sht = self.bases[(Nl, Nx)].clone()
-                    sht.configure(scatter=self.config[i][0], chunk=self.config[i][1])
-                    self.ws[key] = [sht, 0]
-                self.ws[key][1] = max(self.ws[key][1], th.numel())
-        for sht, n in self.ws.values():
-            sht.reserve(n)
-        self.out = [torch.empty(s[3] * (s[3] + 1) // 2, dtype=torch.complex128, device=self.device)
-                    for s in self.sets]
+        for j, calls in enumerate(self.assign):
+            for ci in calls:
+                key, idx = self.calls[ci]
+                wk = (j,) + key
+                if wk not in self.ws:
+                    sht = self.bases[key[:2]].clone()
+                    sht.configure(scatter=key[2], chunk=key[3])
+                    self.ws[wk] = [sht, 0, 1]
+                self.ws[wk][1] = max(self.ws[wk][1], max(self.sets[i][0].numel() for i in idx))
+                self.ws[wk][2] = max(self.ws[wk][2], len(idx))
+        for sht, n, k in self.ws.values():
+            sht.reserve(n, k)
+        self.call_out = [torch.empty((len(idx), key[0] * (key[0] + 1) // 2), dtype=torch.complex128,
+                                     device=self.device) for key, idx in self.calls]
+        self.call_status = [torch.zeros(len(idx), dtype=torch.int32, device=self.device)
+                            for _, idx in self.calls]
+        self.out = [None] * len(self.sets)
+        for ci, (key, idx) in enumerate(self.calls):
+            for r, i in enumerate(idx):
+                self.out[i] = self.call_out[ci][r]
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(self.nstreams)]
         self.graph = None
         if graph:
             self._capture()
 
-    @staticmethod
-    def _cost(s):
-        th, _, _, Nl, Nx = s
-        Nlm = Nl * (Nl + 1) // 2
-        return th.numel() * Nl + 0.5 * Nlm * Nx  # scatter ~ N Nl, table stream ~ Nlm Nx
+    def _call_cost(self, call):
+        key, idx = call
+        Nl, Nx = key[0], key[1]
+        return sum(self.sets[i][0].numel() * Nl for i in idx) + 0.5 * Nl * (Nl + 1) // 2 * Nx
 
     def _enqueue(self, origin):
-        for j, idx in enumerate(self.assign):
+        for j, calls in enumerate(self.assign):
             s = self.streams[j]
             s.wait_stream(origin)
-            for i in idx:
-                th, ph, w, Nl, Nx = self.sets[i]
-                sht = self.ws[(j, Nl, Nx) + tuple(self.config[i])][0]
-                sht.transform_device(th, ph, w, out=self.out[i], stream=s)
+            for ci in calls:
+                key, idx = self.calls[ci]
+                sht = self.ws[(j,) + key][0]
+                sht.transform_many_device([self.sets[i][:3] for i in idx], out=self.call_out[ci],
+                                          stream=s, status=self.call_status[ci])
         for s in self.streams:
             origin.wait_stream(s)
 

@@ -106,7 +106,8 @@ static void free_workspace(Workspace& ws) {
   for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
                   (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
-                  (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
+                  (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.rmax, (void*)ws.wsort,
+                  (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
 }
@@ -123,23 +124,28 @@ static void free_workspace(Workspace& ws) {
     ws.bytes += (int64_t)n_;                                                            \
   } while (0)
 
-// Grow the workspace to hold a transform of Np points.  Growth waits for the
-// device so buffers still in use by queued work are never freed under it.
-static int ensure_workspace(Plan& pl, int64_t Np) {
+// Grow the workspace to hold a transform of Np points (per set) with nsets
+// sets, and the sorted-weight array if `wsort` is asked for.  Growth waits
+// for the device so buffers still in use by queued work are never freed
+// under it.
+static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false) {
   const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
   Workspace& ws = pl.ws;
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
-      z.tiles <= ws.cap_tiles)
+      z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort))
     return DSHT_OK;
   DSHT_HIP(hipDeviceSynchronize());
   WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
                std::max(z.runs, ws.cap_runs), std::max(z.tiles, ws.cap_tiles)};
+  const int sets = std::max(nsets, ws.cap_sets);
+  const bool with_w = wsort || ws.has_wsort;
   free_workspace(ws);
+  const int fsz = pl.precision == DSHT_FP32 ? 4 : 8;
   DSHT_ALLOC(ws.rec, want.points * sizeof(PointRec));
   DSHT_ALLOC(ws.run_count, (want.chunks + 1) * sizeof(int));
   DSHT_ALLOC(ws.run_base, (want.chunks + 1) * sizeof(int));
   DSHT_ALLOC(ws.run_seg, want.runs * sizeof(int));
-  DSHT_ALLOC(ws.part, want.runs * (int64_t)pl.Nl * 8 * (pl.precision == DSHT_FP32 ? 4 : 8));
+  DSHT_ALLOC(ws.part, want.runs * (int64_t)pl.Nl * 8 * fsz);
   DSHT_ALLOC(ws.seg_count, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
@@ -147,14 +153,18 @@ static int ensure_workspace(Plan& pl, int64_t Np) {
   DSHT_ALLOC(ws.seg_run_hi, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.ne_rank, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.thist, want.tiles * (pl.nseg + 1) * sizeof(int));
-  DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8));
-  DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sizeof(double));
-  DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sizeof(double));
-  DSHT_ALLOC(ws.flags, 4 * sizeof(int));
+  DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * sets * fsz);
+  DSHT_ALLOC(ws.rmax, sets * sizeof(unsigned));
+  if (with_w) DSHT_ALLOC(ws.wsort, want.points * sizeof(double));
+  DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sets * sizeof(double));
+  DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sets * sizeof(double));
+  DSHT_ALLOC(ws.flags, 4 * sets * sizeof(int));
   ws.cap_points = want.points;
   ws.cap_chunks = want.chunks;
   ws.cap_runs = want.runs;
   ws.cap_tiles = want.tiles;
+  ws.cap_sets = sets;
+  ws.has_wsort = with_w;
   return DSHT_OK;
 }
 
@@ -291,25 +301,51 @@ static int harvest(Plan& pl, int set) {
   return DSHT_OK;
 }
 
-static int run_transform(Plan& pl, int64_t Np, const double* theta, const double* phi,
-                         const double* wt, double* alm, int* status, int mode, hipStream_t s) {
-  if (Np < 0 || Np >= (int64_t)1 << 31) {
-    set_error("dsht_transform: need 0 <= Np < 2^31");
+// One (batched) transform.  Either `shared`: nsets weight rows
+// wsets[set * ldw + i] on one point set (theta[0], phi[0], Np[0]); or nsets
+// independent point sets (theta[set], phi[set], wt[set], Np[set]).  Output
+// alm[set][Nlm][2], status[set] (may be NULL).
+struct BatchArgs {
+  int nsets = 1;
+  bool shared = false;
+  const int64_t* Np = nullptr;
+  const double* const* theta = nullptr;
+  const double* const* phi = nullptr;
+  const double* const* wt = nullptr;
+  const double* wsets = nullptr;
+  int64_t ldw = 0;
+};
+
+static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int mode,
+                     hipStream_t s) {
+  if (a.nsets < 1 || a.nsets > kMaxSets) {
+    set_error("dsht: need 1 <= nsets <= " + std::to_string(kMaxSets));
     return DSHT_EINVAL;
   }
-  if (Np > 0 && (!theta || !phi || !wt)) {
-    set_error("dsht_transform: null input pointer");
+  if (!alm || !a.Np || !a.theta || !a.phi) {
+    set_error("dsht_transform: null argument");
     return DSHT_EINVAL;
   }
-  if (!alm) {
-    set_error("dsht_transform: null output pointer");
-    return DSHT_EINVAL;
+  const int npts_sets = a.shared ? 1 : a.nsets;
+  int64_t maxNp = 0;
+  for (int k = 0; k < npts_sets; ++k) {
+    const int64_t n = a.Np[k];
+    if (n < 0 || n >= (int64_t)1 << 31) {
+      set_error("dsht_transform: need 0 <= Np < 2^31");
+      return DSHT_EINVAL;
+    }
+    const bool wok = a.shared ? (a.wsets != nullptr && a.ldw >= n) : (a.wt && a.wt[k]);
+    if (n > 0 && (!a.theta[k] || !a.phi[k] || !wok)) {
+      set_error("dsht_transform: null input pointer or ldw < Np");
+      return DSHT_EINVAL;
+    }
+    maxNp = std::max(maxNp, n);
   }
-  int rc = ensure_workspace(pl, Np);
+  int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared);
   if (rc) return rc;
-  pl.ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+  Workspace& ws = pl.ws;
   hipEvent_t* ev = nullptr;
-  if (pl.timing) {
+  if (pl.timing && a.nsets == 1 && !a.shared) {
     const int set = pl.next_set;
     pl.next_set = (pl.next_set + 1) % Plan::kTimingSets;
     if ((rc = harvest(pl, set))) return rc;
@@ -321,17 +357,49 @@ static int run_transform(Plan& pl, int64_t Np, const double* theta, const double
     return DSHT_OK;
   };
   if ((rc = mark(0))) return rc;
-  if ((rc = launch_points(pl, Np, theta, phi, wt, mode, s))) return rc;
-  if ((rc = mark(1))) return rc;
-  if ((rc = launch_scatter(pl, Np, s))) return rc;
-  if ((rc = mark(2))) return rc;
-  if ((rc = launch_fold(pl, s))) return rc;
-  if ((rc = mark(3))) return rc;
-  if ((rc = launch_contract(pl, s))) return rc;
+  if (a.shared) {
+    const int64_t Np = a.Np[0];
+    ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+    if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
+    for (int set = 0; set < a.nsets; ++set) {
+      if ((rc = launch_gather_weights(pl, Np, a.wsets + set * a.ldw, s))) return rc;
+      if ((rc = launch_scatter(pl, Np, ws.wsort, s))) return rc;
+      if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
+    }
+  } else {
+    for (int set = 0; set < a.nsets; ++set) {
+      const int64_t Np = a.Np[set];
+      ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+      if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
+        return rc;
+      if ((rc = mark(1))) return rc;
+      if ((rc = launch_scatter(pl, Np, nullptr, s))) return rc;
+      if ((rc = mark(2))) return rc;
+      if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
+      if ((rc = mark(3))) return rc;
+    }
+  }
+  if ((rc = launch_contract(pl, a.nsets, s))) return rc;
   if ((rc = mark(4))) return rc;
-  if ((rc = launch_rotate_combine(pl, alm, status, mode, s))) return rc;
+  if ((rc = launch_rotate_combine(pl, a.nsets, ws.flags, a.shared ? 0 : 4, alm, status, s))) return rc;
   if ((rc = mark(5))) return rc;
   return DSHT_OK;
+}
+
+static int run_transform(Plan& pl, int64_t Np, const double* theta, const double* phi,
+                         const double* wt, double* alm, int* status, int mode, hipStream_t s) {
+  BatchArgs a;
+  a.nsets = 1;
+  a.Np = &Np;
+  a.theta = &theta;
+  a.phi = &phi;
+  a.wt = &wt;
+  if (Np == 0) {  // no points: inputs may be NULL
+    static const double dummy = 0.0;
+    static const double* dp = &dummy;
+    a.theta = a.phi = a.wt = &dp;
+  }
+  return run_batch(pl, a, alm, status, mode, s);
 }
 
 }  // namespace dsht
@@ -495,13 +563,13 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   return DSHT_OK;
 }
 
-int dsht_plan_reserve(dsht_plan* p, int64_t Np) {
-  if (!p || Np < 0 || Np >= (int64_t)1 << 31) {
-    set_error("dsht_plan_reserve: null plan or Np outside [0, 2^31)");
+int dsht_plan_reserve(dsht_plan* p, int64_t Np, int nsets) {
+  if (!p || Np < 0 || Np >= (int64_t)1 << 31 || nsets < 1 || nsets > kMaxSets) {
+    set_error("dsht_plan_reserve: null plan, Np outside [0, 2^31) or nsets outside [1, 64]");
     return DSHT_EINVAL;
   }
   DeviceGuard g(p->pl.device);
-  return ensure_workspace(p->pl, Np);
+  return ensure_workspace(p->pl, Np, nsets, nsets > 1);
 }
 
 int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {
@@ -574,6 +642,54 @@ int dsht_transform(dsht_plan* p, int64_t Np, const double* theta, const double* 
   }
   DeviceGuard g(p->pl.device);
   return run_transform(p->pl, Np, theta, phi, wt, alm, status, mode, (hipStream_t)stream);
+}
+
+int dsht_transform_sets(dsht_plan* p, int64_t Np, const double* theta, const double* phi,
+                        int nsets, const double* wt, int64_t ldw, double* alm, int* status,
+                        int mode, void* stream) {
+  if (!p) {
+    set_error("dsht_transform_sets: null plan");
+    return DSHT_EINVAL;
+  }
+  if (mode != DSHT_MODE_CALL && mode != DSHT_MODE_BASIC) {
+    set_error("dsht_transform_sets: bad mode");
+    return DSHT_EINVAL;
+  }
+  BatchArgs a;
+  a.nsets = nsets;
+  a.shared = true;
+  a.Np = &Np;
+  a.theta = &theta;
+  a.phi = &phi;
+  a.wsets = wt;
+  a.ldw = ldw;
+  DeviceGuard g(p->pl.device);
+  return run_batch(p->pl, a, alm, status, mode, (hipStream_t)stream);
+}
+
+int dsht_transform_many(dsht_plan* p, int nsets, const int64_t* Np, const double* const* theta,
+                        const double* const* phi, const double* const* wt, double* alm,
+                        int* status, int mode, void* stream) {
+  if (!p) {
+    set_error("dsht_transform_many: null plan");
+    return DSHT_EINVAL;
+  }
+  if (mode != DSHT_MODE_CALL && mode != DSHT_MODE_BASIC) {
+    set_error("dsht_transform_many: bad mode");
+    return DSHT_EINVAL;
+  }
+  if (!wt) {
+    set_error("dsht_transform_many: null weight array");
+    return DSHT_EINVAL;
+  }
+  BatchArgs a;
+  a.nsets = nsets;
+  a.Np = Np;
+  a.theta = theta;
+  a.phi = phi;
+  a.wt = wt;
+  DeviceGuard g(p->pl.device);
+  return run_batch(p->pl, a, alm, status, mode, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------

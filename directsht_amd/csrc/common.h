@@ -88,9 +88,10 @@ inline int num_segments(int Nx) { return 4 * (Nx - 1); }
 // the placement pass and read by the scatter.
 struct alignas(64) PointRec {
   double ere, eim;         // e1 = e^{-i phi} = (cos phi, -sin phi)
-  double h0, h1, h2, h3;   // w * Hermite basis at t (dx applied at the fold)
+  double h0, h1, h2, h3;   // w * Hermite basis at t (dx applied at the fold);
+                           // the basis alone (w = 1) for weight-set transforms
   int seg;                 // segment id
-  int pad;
+  int idx;                 // the point's position in the caller's input arrays
 };
 static_assert(sizeof(PointRec) == 64, "PointRec must be one 64-byte line");
 

@@ -23,14 +23,17 @@ constexpr int kInner = 16;  // node pairs per FP32 partial sum
 __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ tab,
                                                    const float4* __restrict__ rhs,
                                                    const GroupDesc* __restrict__ groups, int Npair,
-                                                   int Nl, double* __restrict__ a_equ,
+                                                   int Nl, int nsets, int set,
+                                                   double* __restrict__ a_equ,
                                                    double* __restrict__ a_pol) {
   const GroupDesc gd = groups[blockIdx.x];
   if (gd.nrows == 0) return;  // padding of the XCD schedule
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
   const float4* t = tab + gd.off + r;
-  const float4* q = rhs + (int64_t)(gd.m * 2 + gd.P) * Npair * 4;
+  // the set's four coefficient vectors of node pair np start at q[np * qs]
+  const float4* q = rhs + ((int64_t)(gd.m * 2 + gd.P) * Npair * nsets + set) * 4;
+  const int64_t qs = 4 * (int64_t)nsets;
   const int64_t stride = gd.nrows;
   double aer = 0, aei = 0, apr = 0, api = 0;
   for (int np0 = 0; np0 < Npair; np0 += kInner) {
@@ -39,8 +42,8 @@ __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ ta
 #pragma unroll 8
     for (int np = np0; np < np1; ++np) {
       const float4 y = valid ? t[np * stride] : make_float4(0.f, 0.f, 0.f, 0.f);
-      const float4 qer = q[np * 4 + 0], qei = q[np * 4 + 1];
-      const float4 qpr = q[np * 4 + 2], qpi = q[np * 4 + 3];
+      const float4 qer = q[np * qs + 0], qei = q[np * qs + 1];
+      const float4 qpr = q[np * qs + 2], qpi = q[np * qs + 3];
       fer = fmaf(y.x, qer.x, fmaf(y.y, qer.y, fmaf(y.z, qer.z, fmaf(y.w, qer.w, fer))));
       fei = fmaf(y.x, qei.x, fmaf(y.y, qei.y, fmaf(y.z, qei.z, fmaf(y.w, qei.w, fei))));
       fpr = fmaf(y.x, qpr.x, fmaf(y.y, qpr.y, fmaf(y.z, qpr.z, fmaf(y.w, qpr.w, fpr))));
@@ -53,7 +56,7 @@ __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ ta
   }
   if (valid) {
     const int ell = gd.m + gd.P + 2 * (gd.i0 + r);
-    const int64_t idx = hp_index(ell, gd.m, Nl);
+    const int64_t idx = hp_index(ell, gd.m, Nl) + (int64_t)set * Nl * (Nl + 1) / 2;
     a_equ[2 * idx] = aer;
     a_equ[2 * idx + 1] = aei;
     a_pol[2 * idx] = apr;
@@ -64,21 +67,24 @@ __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ ta
 __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ tab,
                                                    const double4* __restrict__ rhs,
                                                    const GroupDesc* __restrict__ groups, int Npair,
-                                                   int Nl, double* __restrict__ a_equ,
+                                                   int Nl, int nsets, int set,
+                                                   double* __restrict__ a_equ,
                                                    double* __restrict__ a_pol) {
   const GroupDesc gd = groups[blockIdx.x];
   if (gd.nrows == 0) return;  // padding of the XCD schedule
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
   const double4* t = tab + gd.off + r;
-  const double4* q = rhs + (int64_t)(gd.m * 2 + gd.P) * Npair * 4;
+  // the set's four coefficient vectors of node pair np start at q[np * qs]
+  const double4* q = rhs + ((int64_t)(gd.m * 2 + gd.P) * Npair * nsets + set) * 4;
+  const int64_t qs = 4 * (int64_t)nsets;
   const int64_t stride = gd.nrows;
   double aer = 0, aei = 0, apr = 0, api = 0;
 #pragma unroll 4
   for (int np = 0; np < Npair; ++np) {
     const double4 y = valid ? t[np * stride] : make_double4(0, 0, 0, 0);
-    const double4 qer = q[np * 4 + 0], qei = q[np * 4 + 1];
-    const double4 qpr = q[np * 4 + 2], qpi = q[np * 4 + 3];
+    const double4 qer = q[np * qs + 0], qei = q[np * qs + 1];
+    const double4 qpr = q[np * qs + 2], qpi = q[np * qs + 3];
     aer = fma(y.x, qer.x, fma(y.y, qer.y, fma(y.z, qer.z, fma(y.w, qer.w, aer))));
     aei = fma(y.x, qei.x, fma(y.y, qei.y, fma(y.z, qei.z, fma(y.w, qei.w, aei))));
     apr = fma(y.x, qpr.x, fma(y.y, qpr.y, fma(y.z, qpr.z, fma(y.w, qpr.w, apr))));
@@ -86,7 +92,7 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
   }
   if (valid) {
     const int ell = gd.m + gd.P + 2 * (gd.i0 + r);
-    const int64_t idx = hp_index(ell, gd.m, Nl);
+    const int64_t idx = hp_index(ell, gd.m, Nl) + (int64_t)set * Nl * (Nl + 1) / 2;
     a_equ[2 * idx] = aer;
     a_equ[2 * idx + 1] = aei;
     a_pol[2 * idx] = apr;
@@ -94,17 +100,20 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
   }
 }
 
-int launch_contract(Plan& pl, hipStream_t s) {
-  if (pl.precision == DSHT_FP32) {
-    k_contract32<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const float4*>(pl.tab32),
-                                           reinterpret_cast<const float4*>(pl.ws.rhs), pl.groups,
-                                           pl.Npair, pl.Nl, pl.ws.a_equ, pl.ws.a_pol);
-  } else {
-    k_contract64<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const double4*>(pl.tab64),
-                                           reinterpret_cast<const double4*>(pl.ws.rhs), pl.groups,
-                                           pl.Npair, pl.Nl, pl.ws.a_equ, pl.ws.a_pol);
+int launch_contract(Plan& pl, int nsets, hipStream_t s) {
+  if (pl.precision == DSHT_FP32 && nsets >= kMfmaMinSets) return launch_contract_mfma(pl, nsets, s);
+  for (int set = 0; set < nsets; ++set) {
+    if (pl.precision == DSHT_FP32) {
+      k_contract32<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const float4*>(pl.tab32),
+                                             reinterpret_cast<const float4*>(pl.ws.rhs), pl.groups,
+                                             pl.Npair, pl.Nl, nsets, set, pl.ws.a_equ, pl.ws.a_pol);
+    } else {
+      k_contract64<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const double4*>(pl.tab64),
+                                             reinterpret_cast<const double4*>(pl.ws.rhs), pl.groups,
+                                             pl.Npair, pl.Nl, nsets, set, pl.ws.a_equ, pl.ws.a_pol);
+    }
+    DSHT_LAUNCH_CHECK();
   }
-  DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
 

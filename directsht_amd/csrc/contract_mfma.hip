@@ -1,0 +1,147 @@
+// Stage (c) for many right-hand sides at once (DSHT_FP32 plans): the table
+// contraction of a batched transform as a matrix-core GEMM,
+//   a_s(l, m) = sum_n Y_lm(x_n) c_s(m, n) + Y'_lm(x_n) d_s(m, n)
+// for every set s (weight set or point set), both branches and both parts --
+// the dot products of sht/interp_funcs_py.py:46-60, once per set in the
+// reference, here with every table entry read once for all sets.
+//
+// Per row group (up to 64 rows l of one (m, P), the layout the plain
+// contraction streams) the GEMM is
+//   C[l][col] = sum_K T[l][K] R[K][col],
+//   K = (node pair np, {Y(2np), Y(2np+1), Y'(2np), Y'(2np+1)}),
+//   col = (set, branch, re|im)  (4 per set),
+// run as v_mfma_f32_32x32x16_f16 with both operands split into fp16 hi + lo
+// and three products hi*hi + hi*lo + lo*hi accumulated in FP32 (~2^-22 per
+// product).  fp16 range: the Y' entries are multiplied by dx (|Y' dx| stays
+// below ~12 for l < 2048) and the d coefficients divided by it; every set's
+// coefficients are scaled by a power of two that brings max(|c|, |d / dx|) of
+// the set (recorded by the fold) to [2^13, 2^14), and the result is scaled
+// back exactly.
+//
+// One workgroup (8 wavefronts) per row group.  Per chunk of 16 node pairs
+// (4 MFMA K steps) the workgroup stages the table block (64 rows x 64 K),
+// split into hi/lo, in LDS in the MFMA A-operand layout; wavefront w owns
+// column tile w (32 columns = 8 sets) and both row tiles, loads its B
+// operands straight from the node coefficients rhs[m][P][np][set][16] (32
+// consecutive columns = 512 contiguous bytes per node pair) and issues 6
+// MFMAs per K step.  Accumulators restart every chunk and are added into an
+// outer FP32 sum.
+#include "common.h"
+#include "plan.h"
+#include "split.h"
+
+namespace dsht {
+
+constexpr int kCK = 16;         // node pairs per chunk (4 MFMA K steps of 16)
+constexpr int kCThreads = 512;  // 8 wavefronts
+
+__global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
+    const float4* __restrict__ tab, const float* __restrict__ rhs,
+    const GroupDesc* __restrict__ groups, int Npair, int Nl, int nsets, float dx,
+    const unsigned* __restrict__ rmax, double* __restrict__ a_equ, double* __restrict__ a_pol) {
+  const GroupDesc gd = groups[blockIdx.x];
+  if (gd.nrows == 0) return;  // padding of the XCD schedule
+  // [K step][row tile][hi|lo][lane]: 8 halves per lane, as one 16-byte read
+  __shared__ __attribute__((aligned(16))) u32x4 sA[4][2][2][64];
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5;
+  const int ncol = 4 * nsets, nct = (ncol + 31) >> 5;
+  const bool active = w < nct;  // wave-uniform
+  const int col = 32 * w + (l & 31);
+  const bool colv = active && col < ncol;
+  const int set = colv ? col >> 2 : 0;
+  const int br = (col >> 1) & 1, ri = col & 1;
+  float bsc = 1.f;
+  if (colv) {
+    const float mx = __uint_as_float(rmax[set]);
+    if (mx > 0.f) bsc = ldexpf(1.f, 13 - ilogbf(mx));
+  }
+  const float bscd = bsc / dx;
+  const int64_t qstride = 16 * (int64_t)nsets;  // floats per node pair
+  const float* qb = rhs + ((int64_t)(gd.m * 2 + gd.P) * Npair * nsets + set) * 16 + br * 8 + ri * 4;
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  const floatx16 zero = {};
+  floatx16 out[2] = {zero, zero}, acc[2];
+
+  for (int c0 = 0; c0 < Npair; c0 += kCK) {
+    // this lane's B operands for the chunk: K step t, node pairs c0+4t+2h+{0,1}
+    float4 bq[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int np = c0 + 4 * t + 2 * h + u;
+        bq[t][u] = (colv && np < Npair) ? *reinterpret_cast<const float4*>(qb + np * qstride) : zero4;
+      }
+    // stage the table block: entry e = (node pair c0 + e/64, row e%64)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int e = tid + kCThreads * v;
+      const int u = e >> 6, r = e & 63;
+      const int np = c0 + u;
+      float4 y = (np < Npair && r < gd.nrows) ? tab[gd.off + (int64_t)np * gd.nrows + r] : zero4;
+      y.z *= dx;
+      y.w *= dx;
+      uint32_t h01, l01, h23, l23;
+      split2_f16(y.x, y.y, h01, l01);
+      split2_f16(y.z, y.w, h23, l23);
+      // K index within the step: 4 (u % 4) + q -> lane half (u % 4) / 2, slot 4 ((u % 4) % 2) + q
+      const int t = u >> 2, hh = (u >> 1) & 1, half = u & 1;
+      const int rt = r >> 5, lane = hh * 32 + (r & 31);
+      reinterpret_cast<u32x2*>(&sA[t][rt][0][lane])[half] = (u32x2){h01, h23};
+      reinterpret_cast<u32x2*>(&sA[t][rt][1][lane])[half] = (u32x2){l01, l23};
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float x[8] = {bq[t][0].x * bsc,  bq[t][0].y * bsc,  bq[t][0].z * bscd,
+                            bq[t][0].w * bscd, bq[t][1].x * bsc,  bq[t][1].y * bsc,
+                            bq[t][1].z * bscd, bq[t][1].w * bscd};
+        half8 bhi, blo;
+        split8_f16(x, bhi, blo);
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+          const half8 ahi = __builtin_bit_cast(half8, sA[t][rt][0][l]);
+          const half8 alo = __builtin_bit_cast(half8, sA[t][rt][1][l]);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bhi, t == 0 ? zero : acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, blo, acc[rt], 0, 0, 0);
+          acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bhi, acc[rt], 0, 0, 0);
+        }
+      }
+      out[0] += acc[0];
+      out[1] += acc[1];
+    }
+    __syncthreads();
+  }
+
+  if (colv) {
+    // C layout: lane l, register r holds C[8 (r/4) + 4 (l/32) + r%4][l%32]
+    const double inv = 1.0 / (double)bsc;  // exact: a power of two
+    double* dst = br ? a_pol : a_equ;
+    const int64_t base = (int64_t)set * ((int64_t)Nl * (Nl + 1) / 2);
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 32 * rt + 8 * (r >> 2) + 4 * h + (r & 3);
+        if (row < gd.nrows) {
+          const int ell = gd.m + gd.P + 2 * (gd.i0 + row);
+          dst[(base + hp_index(ell, gd.m, Nl)) * 2 + ri] = (double)out[rt][r] * inv;
+        }
+      }
+  }
+}
+
+int launch_contract_mfma(Plan& pl, int nsets, hipStream_t s) {
+  if (4 * nsets > 32 * (kCThreads / 64)) {
+    set_error("dsht: at most 64 sets per matrix-core contraction");
+    return DSHT_EINVAL;
+  }
+  k_contract_mfma<<<pl.ngroups, kCThreads, 0, s>>>(
+      reinterpret_cast<const float4*>(pl.tab32), reinterpret_cast<const float*>(pl.ws.rhs),
+      pl.groups, pl.Npair, pl.Nl, nsets, (float)pl.dx, pl.ws.rmax, pl.ws.a_equ, pl.ws.a_pol);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+}  // namespace dsht

@@ -15,9 +15,12 @@
 // and k_fold sums them in order (deterministic).  k_fold: one thread per
 // (order m, node pair np) for both branches; the 64 lanes of a wavefront take
 // 64 consecutive m, so every partial it reads is one 512-byte (FP32) block,
-// and each thread writes the whole 16-value row rhs[m][P][np][16] (64 bytes,
-// FP32) that the contraction reads:
+// and each thread writes the whole 16-value row rhs[m][P][np][set][16] (64
+// bytes, FP32) that the contraction reads (set = weight set or point set of a
+// batched transform, one slot for a single transform):
 //   branch b, offset 8b:  re: c(2np), c(2np+1), d(2np), d(2np+1);  im: same.
+// It also records max(|c|, |d/dx|) of the set (as float bits, atomicMax) for
+// the scaling of the matrix-core contraction.
 #include "common.h"
 #include "plan.h"
 
@@ -25,9 +28,11 @@ namespace dsht {
 
 // Every segment's [first run, last run + 1); segments without runs keep (0, 0).
 __global__ __launch_bounds__(256) void k_seg_clear(int nseg, int* __restrict__ lo,
-                                                   int* __restrict__ hi) {
+                                                   int* __restrict__ hi,
+                                                   unsigned* __restrict__ rmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nseg) lo[s] = hi[s] = 0;
+  if (s == 0) *rmax = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_seg_runs(const int* __restrict__ run_seg,
@@ -60,15 +65,17 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
                                                            const int* __restrict__ seg_lo,
                                                            const int* __restrict__ seg_hi,
                                                            int Nl, int Nx, int Npair, double dx,
-                                                           T* __restrict__ rhs) {
+                                                           int nsets, int set, T* __restrict__ rhs,
+                                                           unsigned* __restrict__ rmax) {
   using V = typename Vec2<TP>::type;
   const int m = blockIdx.y * kFoldM + (threadIdx.x & (kFoldM - 1));
   const int np = blockIdx.x * kFoldNp + threadIdx.x / kFoldM;
-  if (m >= Nl || np >= Npair) return;
+  const bool live = m < Nl && np < Npair;
+  float vmax = 0.f;  // max |c|, |d / dx| written by this thread
   const V* pv = reinterpret_cast<const V*>(part);
   T out[2][16];  // [P][16]
 #pragma unroll
-  for (int br = 0; br < 2; ++br) {
+  for (int br = 0; br < 2 && live; ++br) {
     // c, d per (node of the pair, hemisphere), complex
     double c[2][2][2] = {}, d[2][2][2] = {};
 #pragma unroll
@@ -99,23 +106,33 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
       for (int ri = 0; ri < 2; ++ri)
 #pragma unroll
         for (int odd = 0; odd < 2; ++odd) {
-          out[P][8 * br + 4 * ri + odd] = (T)(c[odd][0][ri] + sg * c[odd][1][ri]);
-          out[P][8 * br + 4 * ri + 2 + odd] = (T)(dx * (d[odd][0][ri] + sg * d[odd][1][ri]));
+          const double cv = c[odd][0][ri] + sg * c[odd][1][ri];
+          const double dv = d[odd][0][ri] + sg * d[odd][1][ri];
+          out[P][8 * br + 4 * ri + odd] = (T)cv;
+          out[P][8 * br + 4 * ri + 2 + odd] = (T)(dx * dv);
+          vmax = fmaxf(vmax, (float)fmax(fabs(cv), fabs(dv)));
         }
     }
   }
+  if (live) {
 #pragma unroll
-  for (int P = 0; P < 2; ++P) {
-    T* o = rhs + (((int64_t)m * 2 + P) * Npair + np) * 16;
+    for (int P = 0; P < 2; ++P) {
+      T* o = rhs + ((((int64_t)m * 2 + P) * Npair + np) * nsets + set) * 16;
 #pragma unroll
-    for (int q = 0; q < 16; q += 2) *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
+      for (int q = 0; q < 16; q += 2)
+        *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
+    }
   }
+  // per-set magnitude bound for the matrix-core contraction's scaling
+  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+  if ((threadIdx.x & 63) == 0 && vmax > 0.f) atomicMax(rmax, __float_as_uint(vmax));
 }
 
-int launch_fold(Plan& pl, hipStream_t s) {
+int launch_fold(Plan& pl, int nsets, int set, hipStream_t s) {
   Workspace& ws = pl.ws;
   const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
-  k_seg_clear<<<(pl.nseg + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi);
+  unsigned* rmax = ws.rmax + set;
+  k_seg_clear<<<(pl.nseg + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi, rmax);
   DSHT_LAUNCH_CHECK();
   const int64_t max_runs = (int64_t)ws.cur_chunks + pl.nseg + 1;
   k_seg_runs<<<(int)((max_runs + 255) / 256), 256, 0, s>>>(ws.run_seg, run_total, ws.seg_run_lo,
@@ -124,12 +141,12 @@ int launch_fold(Plan& pl, hipStream_t s) {
   dim3 grid((pl.Npair + kFoldNp - 1) / kFoldNp, (pl.Nl + kFoldM - 1) / kFoldM);
   if (pl.precision == DSHT_FP32) {
     k_fold<float, float><<<grid, kFoldM * kFoldNp, 0, s>>>(
-        (const float*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx,
-        (float*)ws.rhs);
+        (const float*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
+        set, (float*)ws.rhs, rmax);
   } else {
     k_fold<double, double><<<grid, kFoldM * kFoldNp, 0, s>>>(
-        (const double*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx,
-        (double*)ws.rhs);
+        (const double*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
+        set, (double*)ws.rhs, rmax);
   }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

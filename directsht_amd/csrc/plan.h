@@ -13,6 +13,8 @@ struct Workspace {
   int64_t cap_chunks = 0;  // capacity of per-chunk arrays
   int64_t cap_runs = 0;    // capacity of the run partials
   int64_t cap_tiles = 0;   // capacity of the tile histogram matrix (rows)
+  int cap_sets = 0;        // sets of a batched transform the per-set arrays hold
+  bool has_wsort = false;  // wsort allocated (weight-set transforms)
   int cur_chunks = 0;      // scatter chunks of the transform being enqueued
   PointRec* rec = nullptr;    // [cap_points] sorted point records
   int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
@@ -27,10 +29,12 @@ struct Workspace {
   int* seg_run_hi = nullptr;  // [nseg + 2]: one past its last run (0 if empty)
   int* ne_rank = nullptr;     // [nseg + 1]: non-empty segments before s
   int* thist = nullptr;       // [cap_tiles][nseg + 1]: tile histograms, then offsets
-  void* rhs = nullptr;        // [Nl][2][Npair][16] float or double
-  double* a_equ = nullptr;    // [Nlm][2]
-  double* a_pol = nullptr;    // [Nlm][2]
-  int* flags = nullptr;       // [4]: status bits, n_polar, n_runs, unused
+  void* rhs = nullptr;        // [Nl][2][Npair][cap_sets][16] float or double
+  unsigned* rmax = nullptr;   // [cap_sets]: max(|c|, |d/dx|) per set (float bits)
+  double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
+  double* a_equ = nullptr;    // [cap_sets][Nlm][2]
+  double* a_pol = nullptr;    // [cap_sets][Nlm][2]
+  int* flags = nullptr;       // [cap_sets][4]: status bits, n_polar, unused, unused
   int64_t bytes = 0;
 };
 
@@ -99,12 +103,22 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk);
 // Kernel launchers (one per source file).
 int launch_tables(const Plan& pl, bool w64, bool w32, double* hpYv, double* hpYd, hipStream_t s);
 int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipStream_t s);
+// binning + counting sort; wt NULL = unit weights; flags: the set's 4 status ints
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
-                  int mode, hipStream_t s);
-int launch_scatter(Plan& pl, int64_t Np, hipStream_t s);
-int launch_scatter_mfma(Plan& pl, int64_t Np, hipStream_t s);
-int launch_fold(Plan& pl, hipStream_t s);
-int launch_contract(Plan& pl, hipStream_t s);
-int launch_rotate_combine(Plan& pl, double* alm, int* status, int mode, hipStream_t s);
+                  int mode, int* flags, hipStream_t s);
+// ws.wsort[pos] = w[rec[pos].idx] for the sorted valid points
+int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s);
+// wsort NULL: the records' own weights
+int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
+int launch_scatter_mfma(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
+// node/parity fold of the last scatter into slot `set` of nsets
+int launch_fold(Plan& pl, int nsets, int set, hipStream_t s);
+// contraction of all nsets slots into a_equ / a_pol [set][Nlm][2]
+constexpr int kMfmaMinSets = 4;  // FP32 plans: matrix-core contraction from this many sets
+constexpr int kMaxSets = 64;     // sets per batched transform
+int launch_contract(Plan& pl, int nsets, hipStream_t s);
+int launch_contract_mfma(Plan& pl, int nsets, hipStream_t s);
+int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
+                          int* status, hipStream_t s);
 
 }  // namespace dsht

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_hist(int64_t Np, const double* __restri
   int flag = 0, npol = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
-    Classified c = classify(theta[i], phi[i], wt[i], p);
+    Classified c = classify(theta[i], phi[i], 0.0, p);  // weights do not move points
     flag |= c.flag;
     npol += c.polar;
     if (kLds) atomicAdd(&lds_hist[c.seg], 1);
@@ -175,13 +175,13 @@ __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restr
                                                int* __restrict__ cursor, PointRec* __restrict__ rec) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
-    Classified c = classify(theta[i], phi[i], wt[i], p);
+    Classified c = classify(theta[i], phi[i], wt ? wt[i] : 1.0, p);
     const int pos = atomicAdd(&cursor[c.seg], 1);
     double2* d = reinterpret_cast<double2*>(rec + pos);
     d[0] = make_double2(c.ere, c.eim);
     d[1] = make_double2(c.h0, c.h1);
     d[2] = make_double2(c.h2, c.h3);
-    reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, 0);
+    reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)i);
   }
 }
 
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
       const int64_t i = i0 + (int64_t)u * kSortThreads;
       th[u] = i < e ? theta[i] : 0.0;
       ph[u] = i < e ? phi[i] : 0.0;
-      ww[u] = i < e ? wt[i] : 0.0;
+      ww[u] = i < e ? (wt ? wt[i] : 1.0) : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kSortBatch; ++u) {
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
         d[0] = make_double2(c.ere, c.eim);
         d[1] = make_double2(c.h0, c.h1);
         d[2] = make_double2(c.h2, c.h3);
-        reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, 0);
+        reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)(i0 + (int64_t)u * kSortThreads));
       }
     }
   }
@@ -359,13 +359,30 @@ __global__ __launch_bounds__(256) void k_clear2(int* __restrict__ a, int na, int
   if (i < nb) b[i] = 0;
 }
 
+// One weight set into the sorted order of the points: wsort[pos] = w[idx].
+__global__ __launch_bounds__(256) void k_gather_weights(int64_t Np, const int* __restrict__ nvalid,
+                                                        const PointRec* __restrict__ rec,
+                                                        const double* __restrict__ w,
+                                                        double* __restrict__ wsort) {
+  const int64_t pos = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (pos < Np && pos < *nvalid) wsort[pos] = w[rec[pos].idx];
+}
+
+int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s) {
+  if (Np == 0) return DSHT_OK;
+  k_gather_weights<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, pl.ws.seg_start + pl.nseg, pl.ws.rec,
+                                                            w, pl.ws.wsort);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
-                  int mode, hipStream_t s) {
+                  int mode, int* flags, hipStream_t s) {
   Workspace& ws = pl.ws;
   PointParams p{pl.Nx, pl.nseg, mode, pl.xmax, pl.dx, pl.acos_xmax, pl.acos_mxmax};
   // counters are cleared by a kernel, not a memset node, so the transform can
   // be captured in a HIP graph and replayed
-  k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, ws.flags, 4);
+  k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, flags, 4);
   DSHT_LAUNCH_CHECK();
   const int nb = pl.nseg + 1;
   const size_t lds = sizeof(int) * nb;
@@ -377,13 +394,13 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
   if (Np > 0) {
     if (tiled) {
-      k_tile_hist<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, p, ws.thist, ws.flags);
+      k_tile_hist<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, p, ws.thist, flags);
       DSHT_LAUNCH_CHECK();
       k_col_sum<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_count);
     } else if (lds <= 64 * 1024) {
-      k_hist<true><<<blocks, threads, lds, s>>>(Np, theta, phi, wt, p, ws.seg_count, ws.flags);
+      k_hist<true><<<blocks, threads, lds, s>>>(Np, theta, phi, wt, p, ws.seg_count, flags);
     } else {
-      k_hist<false><<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_count, ws.flags);
+      k_hist<false><<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_count, flags);
     }
     DSHT_LAUNCH_CHECK();
   }

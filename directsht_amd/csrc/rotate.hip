@@ -29,103 +29,155 @@ namespace dsht {
 constexpr double kScaleHi = 0x1p400;
 constexpr double kScaleLo = 0x1p-400;
 
+// G sets per workgroup share the d^l(pi/2) columns generated by the
+// recursion; each set has its own a_equ / a_pol ([set][Nlm][2]) and its own
+// status flags (flags[set * flag_stride], flag_stride 0 when all sets come
+// from the same points).
+template <int G>
 __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict__ a_equ,
                                                         const double* __restrict__ a_pol,
-                                                        double* __restrict__ alm, int Nl,
+                                                        double* __restrict__ alm, int Nl, int nsets,
                                                         const int* __restrict__ flags,
-                                                        int* __restrict__ status,
+                                                        int flag_stride, int* __restrict__ status,
                                                         const double* __restrict__ rot_top,
                                                         const int* __restrict__ rot_exp,
                                                         const double* __restrict__ rot_AB,
                                                         const double* __restrict__ rot_iB) {
   const int ell = blockIdx.x;
-  const int fl = flags[0];
-  const int npol = flags[1];
-  if (status && ell == 0 && threadIdx.x == 0) *status = fl;
-  if (fl & DSHT_STATUS_POLAR_RANGE) {
-    for (int m = threadIdx.x; m <= ell; m += blockDim.x) {
-      const int64_t idx = hp_index(ell, m, Nl);
-      alm[2 * idx] = 0.0;
-      alm[2 * idx + 1] = 0.0;
+  const int s0 = blockIdx.y * G;
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  // per set: 0 = not in this block, 1 = zeros (polar range failure),
+  // 2 = equatorial part only (no polar point), 3 = rotate and combine
+  int mode[G];
+  bool any_rot = false;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int set = s0 + g;
+    mode[g] = 0;
+    if (set < nsets) {
+      const int fl = flags[set * flag_stride];
+      const int npol = flags[set * flag_stride + 1];
+      if (status && ell == 0 && threadIdx.x == 0) status[set] = fl;
+      mode[g] = (fl & DSHT_STATUS_POLAR_RANGE) ? 1 : (npol == 0 ? 2 : 3);
+      any_rot |= mode[g] == 3;
     }
-    return;
-  }
-  if (npol == 0) {
-    for (int m = threadIdx.x; m <= ell; m += blockDim.x) {
-      const int64_t idx = hp_index(ell, m, Nl);
-      alm[2 * idx] = a_equ[2 * idx];
-      alm[2 * idx + 1] = a_equ[2 * idx + 1];
-    }
-    return;
   }
   extern __shared__ double lds[];
-  double* sre = lds;                  // Re a'_{l m'}
-  double* sim = lds + (ell + 1);      // Im a'_{l m'}
-  double* sAB = lds + 2 * (ell + 1);  // A(m')/B(m')
-  double* sB = lds + 3 * (ell + 1);   // 1/B(m')
+  double* sAB = lds;               // A(m')/B(m')
+  double* sB = lds + (ell + 1);    // 1/B(m')
+  double* sx = lds + 2 * (ell + 1);  // [g][re|im][m']: a'_{l m'} of the sets
   const int64_t tri = (int64_t)ell * (ell + 1) / 2;
-  for (int mp = threadIdx.x; mp <= ell; mp += blockDim.x) {
-    const int64_t idx = hp_index(ell, mp, Nl);
-    sre[mp] = a_pol[2 * idx];
-    sim[mp] = a_pol[2 * idx + 1];
-    sAB[mp] = rot_AB[tri + mp];
-    sB[mp] = rot_iB[tri + mp];
+  if (any_rot) {
+    for (int mp = threadIdx.x; mp <= ell; mp += blockDim.x) {
+      sAB[mp] = rot_AB[tri + mp];
+      sB[mp] = rot_iB[tri + mp];
+      const int64_t idx = hp_index(ell, mp, Nl);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const bool r = mode[g] == 3;
+        const double* ap = a_pol + ((int64_t)(s0 + g) * Nlm + idx) * 2;
+        sx[(2 * g) * (ell + 1) + mp] = r ? ap[0] : 0.0;
+        sx[(2 * g + 1) * (ell + 1) + mp] = r ? ap[1] : 0.0;
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int m = threadIdx.x; m <= ell; m += blockDim.x) {
-    // Step s = l - m' contributes d_{m',m} * Re a'_{m'} when l + m + m' is
-    // even and d_{m',m} * Im a'_{m'} when it is odd, i.e. by the parity of
-    // m + s: even steps use xe, odd steps xo (no divergent branch).
-    const double* xe = (m & 1) ? sim : sre;
-    const double* xo = (m & 1) ? sre : sim;
-    const double twom = 2.0 * m;
-    double cur = rot_top[tri + m], nxt = 0.0;
-    int E = rot_exp[tri + m];
-    double accE = 0.0, accO = 0.0;
-    int mp = ell;
-    // d_{mp-1} = d_mp * (2m/B) - d_{mp+1} * (A/B): one dependent FMA per step
-    while (mp >= 2) {
-      const int pairs = min(4, mp >> 1);
-      for (int q = 0; q < pairs; ++q) {
-        accE = fma(ldexp(cur, E), xe[mp], accE);
-        double nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
-        nxt = cur;
-        cur = nw;
-        --mp;
-        accO = fma(ldexp(cur, E), xo[mp], accO);
-        nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
-        nxt = cur;
-        cur = nw;
-        --mp;
-      }
-      // at most 8 steps since the last check: growth per step is below 3l,
-      // so values stay below 2^400 * (3l)^8 < 2^1023
-      if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
-        cur *= kScaleLo;
-        nxt *= kScaleLo;
-        E += 400;
-      }
-    }
-    if (mp == 1) {
-      accE = fma(ldexp(cur, E), xe[1], accE);
-      cur = fma(cur, twom * sB[1], -nxt * sAB[1]);
-    }
-    const double d0 = ldexp(cur, E);  // d_{0,m}
-    const double sr = (m & 1) ? accO : accE;
-    const double si = (m & 1) ? accE : accO;
     const int64_t idx = hp_index(ell, m, Nl);
-    alm[2 * idx] = a_equ[2 * idx] + fma(d0, sre[0], 2.0 * sr);
-    alm[2 * idx + 1] = a_equ[2 * idx + 1] + fma(d0, sim[0], 2.0 * si);
+    double accE[G], accO[G];
+    double d0 = 0.0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) accE[g] = accO[g] = 0.0;
+    if (any_rot) {
+      // Step s = l - m' contributes d_{m',m} * Re a'_{m'} when l + m + m' is
+      // even and d_{m',m} * Im a'_{m'} when it is odd, i.e. by the parity of
+      // m + s: even steps use xe, odd steps xo (no divergent branch).
+      const int pe = (m & 1), po = 1 - (m & 1);  // re|im row of even / odd steps
+      const double twom = 2.0 * m;
+      double cur = rot_top[tri + m], nxt = 0.0;
+      int E = rot_exp[tri + m];
+      int mp = ell;
+      // d_{mp-1} = d_mp * (2m/B) - d_{mp+1} * (A/B): one dependent FMA per step
+      while (mp >= 2) {
+        const int pairs = min(4, mp >> 1);
+        for (int q = 0; q < pairs; ++q) {
+          double dv = ldexp(cur, E);
+#pragma unroll
+          for (int g = 0; g < G; ++g) accE[g] = fma(dv, sx[(2 * g + pe) * (ell + 1) + mp], accE[g]);
+          double nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
+          nxt = cur;
+          cur = nw;
+          --mp;
+          dv = ldexp(cur, E);
+#pragma unroll
+          for (int g = 0; g < G; ++g) accO[g] = fma(dv, sx[(2 * g + po) * (ell + 1) + mp], accO[g]);
+          nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
+          nxt = cur;
+          cur = nw;
+          --mp;
+        }
+        // at most 8 steps since the last check: growth per step is below 3l,
+        // so values stay below 2^400 * (3l)^8 < 2^1023
+        if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
+          cur *= kScaleLo;
+          nxt *= kScaleLo;
+          E += 400;
+        }
+      }
+      if (mp == 1) {
+        const double dv = ldexp(cur, E);
+#pragma unroll
+        for (int g = 0; g < G; ++g) accE[g] = fma(dv, sx[(2 * g + pe) * (ell + 1) + 1], accE[g]);
+        cur = fma(cur, twom * sB[1], -nxt * sAB[1]);
+      }
+      d0 = ldexp(cur, E);  // d_{0,m}
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (mode[g] == 0) continue;
+      const int64_t o = ((int64_t)(s0 + g) * Nlm + idx) * 2;
+      double re = 0.0, im = 0.0;
+      if (mode[g] >= 2) {
+        re = a_equ[o];
+        im = a_equ[o + 1];
+      }
+      if (mode[g] == 3) {
+        const double sr = (m & 1) ? accO[g] : accE[g];
+        const double si = (m & 1) ? accE[g] : accO[g];
+        re += fma(d0, sx[(2 * g) * (ell + 1)], 2.0 * sr);
+        im += fma(d0, sx[(2 * g + 1) * (ell + 1)], 2.0 * si);
+      }
+      alm[o] = re;
+      alm[o + 1] = im;
+    }
   }
 }
 
-int launch_rotate_combine(Plan& pl, double* alm, int* status, int mode, hipStream_t s) {
-  (void)mode;
-  const size_t lds = sizeof(double) * 4 * (size_t)pl.Nl;
-  k_rotate_combine<<<pl.Nl, 256, lds, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, pl.ws.flags, status,
-                                           pl.rot_top, pl.rot_exp, pl.rot_AB, pl.rot_iB);
+template <int G>
+static int launch_rotate_g(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
+                           int* status, hipStream_t s) {
+  const size_t lds = sizeof(double) * (2 + 2 * G) * (size_t)pl.Nl;
+  if (lds > 64 * 1024)
+    DSHT_HIP(hipFuncSetAttribute((const void*)k_rotate_combine<G>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  dim3 grid(pl.Nl, (nsets + G - 1) / G);
+  k_rotate_combine<G><<<grid, 256, lds, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, nsets, flags,
+                                             flag_stride, status, pl.rot_top, pl.rot_exp, pl.rot_AB,
+                                             pl.rot_iB);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
+}
+
+// alm[set] = a_equ[set] + D(R^-1) a_pol[set] for nsets sets; status[set]
+// (may be NULL) receives the set's DSHT_STATUS_* bits.
+int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
+                          int* status, hipStream_t s) {
+  const size_t per = sizeof(double) * (size_t)pl.Nl;  // LDS bytes per row of Nl doubles
+  const size_t cap = 160 * 1024;                       // LDS of one gfx950 workgroup
+  if (nsets >= 8 && 18 * per <= cap) return launch_rotate_g<8>(pl, nsets, flags, flag_stride, alm, status, s);
+  if (nsets >= 4 && 10 * per <= cap) return launch_rotate_g<4>(pl, nsets, flags, flag_stride, alm, status, s);
+  if (nsets >= 2 && 6 * per <= cap) return launch_rotate_g<2>(pl, nsets, flags, flag_stride, alm, status, s);
+  return launch_rotate_g<1>(pl, nsets, flags, flag_stride, alm, status, s);
 }
 
 }  // namespace dsht

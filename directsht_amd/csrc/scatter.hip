@@ -82,6 +82,7 @@ __global__ __launch_bounds__(64) void k_scatter64(const PointRec* __restrict__ r
                                                   const int* __restrict__ seg_start, int nseg,
                                                   int chunk, int Nl,
                                                   const int* __restrict__ run_base,
+                                                  const double* __restrict__ wsort,
                                                   double* __restrict__ part,
                                                   int* __restrict__ run_seg) {
   const int c = blockIdx.x, mt = blockIdx.y, lane = threadIdx.x;
@@ -122,12 +123,13 @@ __global__ __launch_bounds__(64) void k_scatter64(const PointRec* __restrict__ r
     make_seeds<K, cplx>(rec, pb, nb, mt, [&](int j, int t, cplx z) { seeds[j][t] = z; });
     if (lane < nb) {
       const PointRec& pr = rec[pb + lane];
+      const double ws = wsort ? wsort[pb + lane] : 1.0;
       pdat[lane][0] = pr.ere;
       pdat[lane][1] = pr.eim;
-      pdat[lane][2] = pr.h0;
-      pdat[lane][3] = pr.h1;
-      pdat[lane][4] = pr.h2;
-      pdat[lane][5] = pr.h3;
+      pdat[lane][2] = ws * pr.h0;
+      pdat[lane][3] = ws * pr.h1;
+      pdat[lane][4] = ws * pr.h2;
+      pdat[lane][5] = ws * pr.h3;
       pseg[lane] = pr.seg;
     }
     __syncthreads();
@@ -171,6 +173,7 @@ __global__ __launch_bounds__(64) void k_scatter32(const PointRec* __restrict__ r
                                                   const int* __restrict__ seg_start, int nseg,
                                                   int chunk, int Nl,
                                                   const int* __restrict__ run_base,
+                                                  const double* __restrict__ wsort,
                                                   float* __restrict__ part,
                                                   int* __restrict__ run_seg) {
   const int c = blockIdx.x, mt = blockIdx.y, lane = threadIdx.x;
@@ -221,12 +224,13 @@ __global__ __launch_bounds__(64) void k_scatter32(const PointRec* __restrict__ r
                           [&](int j, int t, cplx z) { seeds[j][t] = make_float2((float)z.x, (float)z.y); });
     if (lane < nb) {
       const PointRec& pr = rec[pb + lane];
+      const double ws = wsort ? wsort[pb + lane] : 1.0;
       pdat[lane][0] = (float)pr.ere;
       pdat[lane][1] = (float)pr.eim;
-      pdat[lane][2] = (float)pr.h0;
-      pdat[lane][3] = (float)pr.h1;
-      pdat[lane][4] = (float)pr.h2;
-      pdat[lane][5] = (float)pr.h3;
+      pdat[lane][2] = (float)(ws * pr.h0);
+      pdat[lane][3] = (float)(ws * pr.h1);
+      pdat[lane][4] = (float)(ws * pr.h2);
+      pdat[lane][5] = (float)(ws * pr.h3);
       pseg[lane] = pr.seg;
     }
     __syncthreads();
@@ -267,20 +271,21 @@ __global__ __launch_bounds__(64) void k_scatter32(const PointRec* __restrict__ r
   flush();
 }
 
-int launch_scatter(Plan& pl, int64_t Np, hipStream_t s) {
+int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
-  if (pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX) return launch_scatter_mfma(pl, Np, s);
+  if (pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX)
+    return launch_scatter_mfma(pl, Np, wsort, s);
   if (pl.precision == DSHT_FP32) {
     constexpr int K = 8;
     dim3 grid(nchunks, (pl.Nl + 64 * K - 1) / (64 * K));
     k_scatter32<K><<<grid, 64, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
-                                       pl.ws.run_base, (float*)pl.ws.part, pl.ws.run_seg);
+                                       pl.ws.run_base, wsort, (float*)pl.ws.part, pl.ws.run_seg);
   } else {
     constexpr int K = 8;
     dim3 grid(nchunks, (pl.Nl + 64 * K - 1) / (64 * K));
     k_scatter64<K><<<grid, 64, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
-                                       pl.ws.run_base, (double*)pl.ws.part, pl.ws.run_seg);
+                                       pl.ws.run_base, wsort, (double*)pl.ws.part, pl.ws.run_seg);
   }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

@@ -39,11 +39,9 @@
 // r + 4 of the same lane.
 #include "common.h"
 #include "plan.h"
+#include "split.h"
 
 namespace dsht {
-
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -79,49 +77,6 @@ constexpr int kGroup = 0;
 constexpr int kRowB = kBlock + 8;   // halves per B row (68 dwords)
 constexpr int kRowE = kBlock + 4;   // floats per E row (132 dwords)
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// x = hi + lo for 8 values: hi = fp16(x) (round to nearest even, packed in
-// pairs), lo = fp16(x - hi) computed by a mixed-precision FMA that reads the
-// very hi register the MFMA will read (x - hi is exact in FP32, so lo is
-// correctly rounded).  One asm block keeps the compiler from deriving hi and
-// lo from differently rounded copies of x; the trailing s_nop covers the
-// VALU-write -> MFMA-read dependency, which the compiler's hazard tracking
-// does not see through inline assembly.
-__device__ inline void split8_f16(const float (&x)[8], half8& ahi, half8& alo) {
-  uint32_t h0, h1, h2, h3, l0, l1, l2, l3;
-  asm("v_cvt_pk_f16_f32 %0, %8, %9\n\t"
-      "v_cvt_pk_f16_f32 %1, %10, %11\n\t"
-      "v_cvt_pk_f16_f32 %2, %12, %13\n\t"
-      "v_cvt_pk_f16_f32 %3, %14, %15\n\t"
-      "v_fma_mixlo_f16 %4, -%0, 1.0, %8 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %4, -%0, 1.0, %9 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixlo_f16 %5, -%1, 1.0, %10 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %5, -%1, 1.0, %11 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixlo_f16 %6, -%2, 1.0, %12 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %6, -%2, 1.0, %13 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixlo_f16 %7, -%3, 1.0, %14 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %7, -%3, 1.0, %15 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\t"
-      "s_nop 4"
-      : "=&v"(h0), "=&v"(h1), "=&v"(h2), "=&v"(h3), "=&v"(l0), "=&v"(l1), "=&v"(l2), "=&v"(l3)
-      : "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "v"(x[4]), "v"(x[5]), "v"(x[6]), "v"(x[7]));
-  ahi = __builtin_bit_cast(half8, (u32x4){h0, h1, h2, h3});
-  alo = __builtin_bit_cast(half8, (u32x4){l0, l1, l2, l3});
-}
-
-// hi/lo of a complex value (re, im) for the B operand: packed (re | im << 16);
-// the result goes to LDS, so no MFMA hazard applies.
-__device__ inline void split2_f16(float re, float im, uint32_t& hi, uint32_t& lo) {
-  asm("v_cvt_pk_f16_f32 %0, %2, %3\n\t"
-      "v_fma_mixlo_f16 %1, -%0, 1.0, %2 op_sel_hi:[1,0,0]\n\t"
-      "v_fma_mixhi_f16 %1, -%0, 1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
-      : "=&v"(hi), "=&v"(lo)
-      : "v"(re), "v"(im));
-}
-
-__device__ inline _Float16 half_lo(uint32_t u) { return __builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu)); }
-__device__ inline _Float16 half_hi(uint32_t u) { return __builtin_bit_cast(_Float16, (uint16_t)(u >> 16)); }
-
 // A operand of one window pass: this lane's 8 points (K slice) of row
 // (k, m1, re|im): h_k(p) E(m1, p), with points outside [p0, p1) zeroed when
 // the window holds more than one run.
@@ -155,6 +110,7 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
                                                       const int* __restrict__ seg_start, int nseg,
                                                       int chunk, int Nl,
                                                       const int* __restrict__ run_base,
+                                                      const double* __restrict__ wsort,
                                                       float* __restrict__ part,
                                                       int* __restrict__ run_seg) {
   const int c = blockIdx.x, mt = blockIdx.y;
@@ -199,7 +155,8 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
       if (p < bn) {
         const PointRec& pr = rec[blk + p];
         e1 = cplx{pr.ere, pr.eim};
-        hval = (float)(g == 0 ? pr.h0 : g == 1 ? pr.h1 : g == 2 ? pr.h2 : pr.h3);
+        const double hk = g == 0 ? pr.h0 : g == 1 ? pr.h1 : g == 2 ? pr.h2 : pr.h3;
+        hval = (float)(wsort ? wsort[blk + p] * hk : hk);
         sg = pr.seg;
       }
       sH[g][p] = hval;
@@ -298,12 +255,12 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
   }
 }
 
-int launch_scatter_mfma(Plan& pl, int64_t Np, hipStream_t s) {
+int launch_scatter_mfma(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
   dim3 grid(nchunks, (pl.Nl + 1023) / 1024);
   k_scatter_mfma<<<grid, 512, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
-                                      pl.ws.run_base, (float*)pl.ws.part, pl.ws.run_seg);
+                                      pl.ws.run_base, wsort, (float*)pl.ws.part, pl.ws.run_seg);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

@@ -124,3 +124,14 @@ def lognormal_weights(theta, phi, seed, sigma=0.5, lmax=256, ntheta=512, nphi=10
     theta_r, g = grf_grid(lmax, ntheta, nphi, sigma, seed)
     gt = torch.from_numpy(g).to(theta.device)
     return torch.exp(_bilinear(gt, theta_r, theta, phi) - sigma ** 2 / 2)
+
+
+def lognormal_weight_sets(theta, phi, nsets, seed0=300, sigma=0.5, lmax=256, ntheta=512, nphi=1024):
+    """(nsets, N) weights: set 0 constant 1, set s > 0 = exp(g_s - sigma^2/2)
+    with an independent field g_s (seed seed0 + s) -- the many-weight-set
+    workload."""
+    W = torch.empty((nsets, theta.numel()), dtype=torch.float64, device=theta.device)
+    W[0] = 1.0
+    for s in range(1, nsets):
+        W[s] = lognormal_weights(theta, phi, seed0 + s, sigma, lmax, ntheta, nphi)
+    return W

@@ -88,11 +88,13 @@ class DirectSHT:
         other._Yv = other._Yd = None
         return other
 
-    def reserve(self, npoints):
-        """Size the workspace for transforms of up to ``npoints`` points now,
-        so later transforms neither allocate nor synchronize."""
+    def reserve(self, npoints, nsets=1):
+        """Size the workspace for transforms of up to ``npoints`` points (per
+        set) and ``nsets`` sets now, so later transforms neither allocate nor
+        synchronize."""
         with torch.cuda.device(self.device):
-            _lib.check(_lib.load().dsht_plan_reserve(self._plan, int(npoints)), "dsht_plan_reserve")
+            _lib.check(_lib.load().dsht_plan_reserve(self._plan, int(npoints), int(nsets)),
+                       "dsht_plan_reserve")
 
     def footprint(self):
         """HBM bytes held: tables (shared with clones) + own workspace."""
@@ -210,6 +212,58 @@ class DirectSHT:
                 self._status.data_ptr(), mode, stream.cuda_stream), "dsht_transform")
         return out, self._status
 
+    def transform_sets_device(self, theta, phi, W, out=None, mode=_lib.DSHT_MODE_CALL, stream=None,
+                              status=None):
+        """Enqueue transforms of many weight sets on shared points: W is a
+        (S, N) float64 CUDA tensor; returns (alm (S, Nlm) complex128, status
+        (S,) int32), nothing synchronized.  The points are binned once and the
+        tables read once per group of up to 64 sets."""
+        n = theta.numel()
+        S = W.shape[0]
+        W = W.contiguous()
+        if out is None:
+            out = torch.empty((S, self.Nlm), dtype=torch.complex128, device=self.device)
+        if status is None:
+            status = torch.zeros(S, dtype=torch.int32, device=self.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        lib = _lib.load()
+        with torch.cuda.device(self.device):
+            for s0 in range(0, S, _lib.MAX_SETS):
+                k = min(_lib.MAX_SETS, S - s0)
+                _lib.check(lib.dsht_transform_sets(
+                    self._plan, n, theta.data_ptr(), phi.data_ptr(), k, W[s0].data_ptr(), W.stride(0),
+                    out[s0].data_ptr(), status[s0:].data_ptr(), mode, stream.cuda_stream),
+                    "dsht_transform_sets")
+        return out, status
+
+    def transform_many_device(self, sets, out=None, mode=_lib.DSHT_MODE_CALL, stream=None,
+                              status=None):
+        """Enqueue transforms of many point sets at this (Nell, Nx): sets is a
+        list of (theta, phi, w) float64 CUDA tensors; returns (alm (S, Nlm),
+        status (S,)), nothing synchronized.  The table contraction and the
+        polar rotation run once per group of up to 64 sets."""
+        S = len(sets)
+        if out is None:
+            out = torch.empty((S, self.Nlm), dtype=torch.complex128, device=self.device)
+        if status is None:
+            status = torch.zeros(S, dtype=torch.int32, device=self.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        lib = _lib.load()
+        with torch.cuda.device(self.device):
+            for s0 in range(0, S, _lib.MAX_SETS):
+                grp = sets[s0:s0 + _lib.MAX_SETS]
+                k = len(grp)
+                Np = (ct.c_int64 * k)(*[g[0].numel() for g in grp])
+                th = (ct.c_void_p * k)(*[g[0].data_ptr() for g in grp])
+                ph = (ct.c_void_p * k)(*[g[1].data_ptr() for g in grp])
+                ww = (ct.c_void_p * k)(*[g[2].data_ptr() for g in grp])
+                _lib.check(lib.dsht_transform_many(self._plan, k, Np, th, ph, ww, out[s0].data_ptr(),
+                                                   status[s0:].data_ptr(), mode, stream.cuda_stream),
+                           "dsht_transform_many")
+        return out, status
+
     def _run(self, t, p, w, mode):
         on_device = isinstance(t, torch.Tensor) and t.is_cuda
         n = len(t)
@@ -220,7 +274,12 @@ class DirectSHT:
         return (alm if on_device else alm.cpu().numpy()), st
 
     def __call__(self, t, p, w):
-        """alm of points (t, p) with weights w -- sht/sht.py:112-150."""
+        """alm of points (t, p) with weights w -- sht/sht.py:112-150.
+
+        w may also be 2-D, shape (S, N): S weight sets on the same points,
+        returning alm of shape (S, Nlm) (one binning, one table read)."""
+        if getattr(w, "ndim", 1) == 2:
+            return self._call_sets(t, p, w)
         alm, st = self._run(t, p, w, _lib.DSHT_MODE_CALL)
         if st & _lib.DSHT_STATUS_POLAR_RANGE:
             # sht/sht.py:141-147: caught, printed, zeros returned
@@ -233,6 +292,21 @@ class DirectSHT:
         if st & _lib.DSHT_STATUS_EQU_RANGE:
             raise AssertionError("theta must be in [ACos[xmax],ACos[-xmax]).")
         return alm
+
+    def _call_sets(self, t, p, W):
+        on_device = isinstance(t, torch.Tensor) and t.is_cuda
+        n = len(t)
+        assert n == len(p) and W.shape[1] == n, "theta,phi,wt must be the same length."
+        td, pd = (_as_device(a, self.device) for a in (t, p))
+        Wd = W.to(self.device, torch.float64) if isinstance(W, torch.Tensor) else \
+            torch.from_numpy(np.ascontiguousarray(W, dtype=np.float64)).to(self.device)
+        alm, status = self.transform_sets_device(td, pd, Wd)
+        st = int(status.max().item()) if status.numel() else 0
+        if st & _lib.DSHT_STATUS_EQU_RANGE:
+            raise AssertionError("theta must be in [ACos[xmax],ACos[-xmax]).")
+        if st & _lib.DSHT_STATUS_POLAR_RANGE:
+            print("AssertionError in basic_sht: theta must be in [ACos[xmax],ACos[-xmax]).")
+        return alm if on_device else alm.cpu().numpy()
 
     def basic_sht(self, theta, phi, wt, reg_factor=1.0, verbose=False):
         """alm of points that all satisfy |cos theta| < xmax -- sht/sht.py:153-283."""

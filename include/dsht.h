@@ -97,11 +97,11 @@ int dsht_plan_destroy(dsht_plan* plan);
  * plan that shares them. */
 int dsht_plan_clone(const dsht_plan* src, dsht_plan** clone);
 
-/* Grow the plan's workspace now to hold a transform of Np points, so that
- * later dsht_transform calls with at most Np points allocate nothing and
- * never synchronize (what HIP graph capture and latency-bound callers need).
- * Growth waits for the device. */
-int dsht_plan_reserve(dsht_plan* plan, int64_t Np);
+/* Grow the plan's workspace now to hold a transform of Np points (per set)
+ * with up to nsets sets (1 for dsht_transform), so that later transforms
+ * within those sizes allocate nothing and never synchronize (what HIP graph
+ * capture and latency-bound callers need).  Growth waits for the device. */
+int dsht_plan_reserve(dsht_plan* plan, int64_t Np, int nsets);
 
 /* Bytes of HBM the plan holds: its tables (shared with its clones, see
  * dsht_plan_clone) + its own current workspace. */
@@ -126,6 +126,26 @@ int dsht_plan_get_tables(dsht_plan* plan, double* Yv, double* Yd);
  * by the caller (they share the plan's workspace); clones do not share it. */
 int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
                    const double* wt, double* alm, int* status, int mode, void* stream);
+
+/* Batched transforms (no counterpart in the reference, which transforms one
+ * weight vector of one point set per call; these share the resident tables
+ * and read them once for the whole batch).  At most 64 sets per call.
+ *
+ * Many weight sets on shared points: alm[s] = transform of (theta, phi,
+ * wt + s * ldw) for s < nsets (ldw >= Np).  The points are binned and sorted
+ * once.  alm: device array [nsets][Nlm] complex doubles; status: device
+ * [nsets] ints (may be NULL). */
+int dsht_transform_sets(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
+                        int nsets, const double* wt, int64_t ldw, double* alm, int* status,
+                        int mode, void* stream);
+
+/* Many independent point sets at the plan's (Nl, Nx): set s has Np[s] points
+ * theta[s], phi[s], wt[s] (host arrays of device pointers).  alm, status as
+ * for dsht_transform_sets.  The workspace is reused set after set; the table
+ * contraction and the polar rotation run once for all sets. */
+int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const double* const* theta,
+                        const double* const* phi, const double* const* wt, double* alm,
+                        int* status, int mode, void* stream);
 
 /* Tuning options (dsht_plan_configure).  Results do not depend on them beyond
  * rounding.
