@@ -15,7 +15,10 @@ Printed JSON line (stdout):  value = N * Nlm / seconds per step
 (points*modes/s); roofline of the dominant kernel from HIP events recorded
 on its stream during the timed steps; cpu_baseline = the reference's own C
 (csht, compiled into oracle/_ref) on a bounded sample of the same workload.
-Progress goes to stderr.
+Also (same line): "batched" -- many weight sets on shared points and an
+uneven batch of point sets, batched vs plain single calls, with errors --
+and "serving" -- p50/p99 latency and aggregate throughput of the transform
+server under Poisson arrivals.  Progress goes to stderr.
 """
 import argparse
 import json
@@ -90,6 +93,171 @@ def cpu_baseline(theta, phi, w, Nl, Nx, xmax=0.75):
     }
 
 
+def _timed(fn, reps, sync):
+    """Mean seconds per call of fn() over reps calls (after one untimed call)."""
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def _err(a, ref):
+    """max over sets of (max-rel, rms-rel) of a vs ref, both (S, Nlm)."""
+    d = (a - ref).abs()
+    mx = (d.amax(dim=1) / ref.abs().amax(dim=1)).max()
+    rms = (d.norm(dim=1) / ref.norm(dim=1)).max()
+    return {"max_rel": float(mx), "rms_rel": float(rms)}
+
+
+def bench_weight_sets(reps=3):
+    """Many weight sets on shared points: 2e6 uniform points, 64 weight sets
+    (set 0 constant, 63 lognormal-modulated with distinct seeds), Nl=768,
+    Nx=1536, FP32.  Plain = 64 single-set transforms; batched = one
+    dsht_transform_sets call (binning once, one table stream, fp16x3 MFMA
+    contraction; also with the FP32 GEMV contraction per set)."""
+    import torch
+
+    from directsht_amd import mocks
+    from directsht_amd.sht import DirectSHT
+    Nl, Nx, N, S = 768, 1536, 2_000_000, 64
+    log(f"weight sets: {N} uniform points, {S} weight sets, Nl={Nl}")
+    th, ph, _ = mocks.uniform_sphere(N, seed=3)
+    W = mocks.lognormal_weight_sets(th, ph, S, seed0=300)
+    sync = torch.cuda.synchronize
+    sht = DirectSHT(Nl, Nx, dflt_type="float32")
+    sht.reserve(N, S)
+    out1 = torch.empty(sht.Nlm, dtype=torch.complex128, device="cuda")
+
+    def plain():
+        for s in range(S):
+            sht.transform_device(th, ph, W[s], out=out1)
+
+    outS = torch.empty((S, sht.Nlm), dtype=torch.complex128, device="cuda")
+    t_plain = _timed(plain, reps, sync)
+    t_mat = _timed(lambda: sht.transform_sets_device(th, ph, W, out=outS), reps, sync)
+    a_mat = outS.clone()
+    sht.configure(contract="vector")
+    t_vec = _timed(lambda: sht.transform_sets_device(th, ph, W, out=outS), reps, sync)
+    a_vec = outS.clone()
+    del sht
+    torch.cuda.empty_cache()
+    s64 = DirectSHT(Nl, Nx, dflt_type="float64")
+    a64, _ = s64.transform_sets_device(th, ph, W)
+    sync()
+    Nlm = Nl * (Nl + 1) // 2
+    pm = N * Nlm * S
+    res = {"workload": f"{N} uniform points x {S} weight sets (1 constant, {S - 1} lognormal "
+                       f"sigma=0.5, seeds 301..), Nl={Nl}, Nx={Nx}, FP32",
+           "plain_ms": t_plain * 1e3, "batched_ms": t_mat * 1e3,
+           "batched_vector_contract_ms": t_vec * 1e3,
+           "plain_points_modes_per_s": pm / t_plain, "batched_points_modes_per_s": pm / t_mat,
+           "speedup": t_plain / t_mat,
+           "error_vs_fp64": {"mfma_fp16x3": _err(a_mat, a64), "fp32_gemv": _err(a_vec, a64)}}
+    del s64
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_uneven(nsets=200, reps=3):
+    """Uneven batch: 200 independent uniform point sets, sizes log-uniform in
+    [1e4, 5e6], Nl from {128, 256, 512, 1024}, Nx = 2 Nl, w = U[0.5, 1.5].
+    Plain = one DirectSHT call per set (each returns when its alm is ready);
+    batched = the autotuned BatchTransform graph (one launch)."""
+    import numpy as np
+    import torch
+
+    from directsht_amd.batch import BatchTransform
+    from directsht_amd.sht import DirectSHT
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(5e6), nsets)).astype(np.int64)
+    nls = rng.choice([128, 256, 512, 1024], nsets)
+    log(f"uneven batch: {nsets} sets, {int(sizes.sum())} points")
+    g = torch.Generator(device="cuda").manual_seed(4)
+    sets = []
+    for n, nl in zip(sizes, nls):
+        n = int(n)
+        th = torch.acos(2 * torch.rand(n, generator=g, device="cuda", dtype=torch.float64) - 1)
+        ph = 2 * math.pi * torch.rand(n, generator=g, device="cuda", dtype=torch.float64)
+        w = 0.5 + torch.rand(n, generator=g, device="cuda", dtype=torch.float64)
+        sets.append((th, ph, w, int(nl)))
+    sync = torch.cuda.synchronize
+    t0 = time.perf_counter()
+    bt = BatchTransform(sets, dflt_type="float32", nstreams=4)
+    t_setup = time.perf_counter() - t0
+    plans = {nl: bt.bases[(nl, 2 * nl)] for nl in set(int(x) for x in nls)}
+
+    def plain():
+        for th, ph, w, nl in sets:
+            plans[nl].transform_device(th, ph, w)[1].item()  # the call returns with its result
+
+    t_plain = _timed(plain, 1, sync)
+    t_batch = _timed(bt.run, reps, sync)
+    outs = [o.clone() for o in bt.run()]
+    sync()
+    worst = 0.0
+    for i in range(0, nsets, 25):  # spot check against single calls
+        th, ph, w, nl = sets[i]
+        ref = plans[nl].transform_device(th, ph, w)[0]
+        sync()
+        worst = max(worst, float((outs[i] - ref).abs().max() / ref.abs().max()))
+    pm = sum(int(n) * (int(nl) * (int(nl) + 1) // 2) for n, nl in zip(sizes, nls))
+    res = {"workload": f"{nsets} uniform point sets, N log-uniform [1e4, 5e6] "
+                       f"(total {int(sizes.sum())}), Nl in {{128,256,512,1024}}, Nx=2Nl, FP32",
+           "plain_ms": t_plain * 1e3, "batched_ms": t_batch * 1e3, "speedup": t_plain / t_batch,
+           "plain_points_modes_per_s": pm / t_plain, "batched_points_modes_per_s": pm / t_batch,
+           "setup_s": t_setup, "calls": len(bt.calls), "streams": bt.nstreams,
+           "max_rel_vs_single": worst}
+    del bt, plans, sets
+    torch.cuda.empty_cache()
+    return res
+
+
+def bench_serving(nreq=2100, warm=100, rate=50.0):
+    """Serving loop: Poisson arrivals (rate/s) of requests with N log-uniform
+    in [1e4, 5e7] and Nl in {256, 512, 1024, 1536} (Nx = 2 Nl), FP32 resident
+    tables; inputs are slices of a device-resident pool of uniform points;
+    latency = submit -> alm on the host."""
+    import numpy as np
+    import torch
+
+    from directsht_amd import mocks
+    from directsht_amd.serve import TransformServer
+    rng = np.random.default_rng(5)
+    Nmax = 50_000_000
+    th, ph, w = mocks.uniform_sphere(Nmax, seed=5)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(Nmax), nreq)).astype(np.int64)
+    nls = rng.choice([256, 512, 1024, 1536], nreq)
+    offs = [int(rng.integers(0, Nmax - n + 1)) for n in sizes]
+    gaps = rng.exponential(1.0 / rate, nreq)
+    log(f"serving: {nreq} requests ({warm} warm-up), Poisson {rate}/s")
+    with TransformServer(max_inflight=4, dflt_type="float32") as srv:
+        futs = []
+        t_next = time.perf_counter()
+        for i in range(nreq):
+            t_next += gaps[i]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            a, n = offs[i], int(sizes[i])
+            futs.append(srv.submit(th[a:a + n], ph[a:a + n], w[a:a + n], Nl=int(nls[i])))
+            if (i + 1) % 500 == 0:
+                log(f"serving: {i + 1} submitted")
+        for f in futs:
+            f.result(timeout=600)
+        done = srv.completed[warm:]
+        srv.completed = done
+        st = srv.stats()
+    st["workload"] = (f"Poisson {rate}/s, {nreq - warm} timed requests after {warm} warm-up, "
+                      f"N log-uniform [1e4, 5e7], Nl in {{256,512,1024,1536}}, Nx=2Nl, FP32, "
+                      f"device-resident inputs, alm returned to the host")
+    del th, ph, w
+    torch.cuda.empty_cache()
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +267,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true", help="skip the batched workloads")
+    ap.add_argument("--skip-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-requests", type=int, default=2100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run a 1 GiB device copy 3 times (the known-size "
                          "kernel that scales the PMC FETCH_SIZE/WRITE_SIZE counters)")
@@ -224,6 +395,23 @@ def main():
             result["cpu_baseline"] = None
         if result.get("cpu_baseline"):
             result["cpu_baseline"]["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    if not args.skip_batched:
+        result["batched"] = {}
+        for name, fn in (("weight_sets", bench_weight_sets), ("uneven", bench_uneven)):
+            try:
+                result["batched"][name] = fn()
+                log(f"{name}: {result['batched'][name]}")
+            except Exception as e:  # reported, never fatal for the headline number
+                log(f"{name} failed: {e!r}")
+                result["batched"][name] = {"error": repr(e)}
+    if not args.skip_serving:
+        try:
+            result["serving"] = bench_serving(nreq=args.serve_requests,
+                                              warm=min(100, args.serve_requests // 10))
+            log(f"serving: {result['serving']}")
+        except Exception as e:
+            log(f"serving failed: {e!r}")
+            result["serving"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 

@@ -428,6 +428,14 @@ int dsht_plan_configure(dsht_plan* p, int option, int value) {
     pl.scatter = value;
     return DSHT_OK;
   }
+  if (option == DSHT_OPT_CONTRACT) {
+    if (value != DSHT_CONTRACT_MATRIX && value != DSHT_CONTRACT_VECTOR) {
+      set_error("dsht_plan_configure: unknown contraction engine");
+      return DSHT_EINVAL;
+    }
+    pl.contract = value;
+    return DSHT_OK;
+  }
   if (option == DSHT_OPT_CHUNK) {
     if (value < 64 || value > 65536) {
       set_error("dsht_plan_configure: chunk must be in [64, 65536]");
@@ -547,6 +555,7 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.precision = s.precision;
   d.chunk = s.chunk;
   d.scatter = s.scatter;
+  d.contract = s.contract;
   d.ngroups = s.ngroups;
   d.groups = s.groups;
   d.mp_off = s.mp_off;

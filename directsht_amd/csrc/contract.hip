@@ -101,7 +101,8 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
 }
 
 int launch_contract(Plan& pl, int nsets, hipStream_t s) {
-  if (pl.precision == DSHT_FP32 && nsets >= kMfmaMinSets) return launch_contract_mfma(pl, nsets, s);
+  if (pl.precision == DSHT_FP32 && nsets >= kMfmaMinSets && pl.contract == DSHT_CONTRACT_MATRIX)
+    return launch_contract_mfma(pl, nsets, s);
   for (int set = 0; set < nsets; ++set) {
     if (pl.precision == DSHT_FP32) {
       k_contract32<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const float4*>(pl.tab32),

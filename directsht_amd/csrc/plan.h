@@ -64,6 +64,7 @@ struct Plan {
   // scatter decomposition
   int chunk = 2048;  // sorted points per scatter work item
   int scatter = DSHT_SCATTER_MATRIX;  // engine of FP32 plans
+  int contract = DSHT_CONTRACT_MATRIX;  // engine of batched FP32 contractions
   // contraction layout
   int ngroups = 0;
   GroupDesc* groups = nullptr;  // device [ngroups]

@@ -9,10 +9,19 @@ requested variance.  Points are then drawn by rejection with density
 proportional to exp(g - sigma^2/2) (bilinear lookup in the grid), on the GPU
 with torch's seeded generator.
 """
+import functools
 import math
 
 import numpy as np
 import torch
+
+
+@functools.lru_cache(maxsize=4)
+def _ylm_rings_cached(lmax, ntheta):
+    theta = (np.arange(ntheta) + 0.5) * math.pi / ntheta
+    Y = _ylm_rings(lmax, theta)
+    Y.setflags(write=False)
+    return Y
 
 
 def _ylm_rings(lmax, theta):
@@ -56,7 +65,7 @@ def grf_grid(lmax=256, ntheta=512, nphi=1024, sigma=0.5, seed=0,
             a[0, :] = rng.normal(size=n) * np.sqrt(c)
         else:
             a[m, m:] = (rng.normal(size=n) + 1j * rng.normal(size=n)) * np.sqrt(c[m:] / 2)
-    Y = _ylm_rings(lmax, theta)                       # [m][l][ring]
+    Y = _ylm_rings_cached(lmax, ntheta)               # [m][l][ring]
     F = np.einsum("ml,mlr->rm", a, Y)                 # F_m(theta) per ring
     spec = np.zeros((ntheta, nphi // 2 + 1), dtype=np.complex128)
     spec[:, :lmax + 1] = F

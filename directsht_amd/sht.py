@@ -158,11 +158,16 @@ class DirectSHT:
         return cl
 
     # ---- tuning -----------------------------------------------------------
-    def configure(self, scatter=None, chunk=None):
-        """Select the scatter engine ('matrix' | 'vector', FP32 plans) and the
-        sorted points per scatter work item.  Results do not depend on these
-        beyond rounding."""
+    def configure(self, scatter=None, chunk=None, contract=None):
+        """Select the scatter engine ('matrix' | 'vector', FP32 plans), the
+        sorted points per scatter work item, and the table contraction of
+        batched FP32 transforms ('matrix': fp16x3 MFMA GEMM over all sets,
+        'vector': FP32 GEMV per set).  Results do not depend on these beyond
+        rounding."""
         lib = _lib.load()
+        if contract is not None:
+            v = {"matrix": _lib.DSHT_CONTRACT_MATRIX, "vector": _lib.DSHT_CONTRACT_VECTOR}[contract]
+            _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_CONTRACT, v), "configure")
         if scatter is not None:
             v = {"matrix": _lib.DSHT_SCATTER_MATRIX, "vector": _lib.DSHT_SCATTER_VECTOR}[scatter]
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_SCATTER, v), "configure")

@@ -153,11 +153,18 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
  *                     DSHT_SCATTER_MATRIX (default) fp16x3 split-precision
  *                     MFMA GEMM, or DSHT_SCATTER_VECTOR FP32 VALU recurrence.
  *                     DSHT_FP64 plans always use the FP64 vector scatter.
- *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536).    */
+ *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536).
+ *   DSHT_OPT_CONTRACT table contraction of batched DSHT_FP32 transforms with
+ *                     4 or more sets: DSHT_CONTRACT_MATRIX (default) fp16x3
+ *                     matrix-core GEMM reading the tables once for all sets,
+ *                     or DSHT_CONTRACT_VECTOR FP32 GEMV once per set.      */
 #define DSHT_OPT_SCATTER 1
 #define DSHT_OPT_CHUNK 2
+#define DSHT_OPT_CONTRACT 3
 #define DSHT_SCATTER_MATRIX 0
 #define DSHT_SCATTER_VECTOR 1
+#define DSHT_CONTRACT_MATRIX 0
+#define DSHT_CONTRACT_VECTOR 1
 int dsht_plan_configure(dsht_plan* plan, int option, int value);
 
 /* Stage timing.  While enabled, every dsht_transform records HIP events on
