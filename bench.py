@@ -270,6 +270,8 @@ def main():
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched workloads")
     ap.add_argument("--skip-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2100)
+    ap.add_argument("--only", choices=("weight_sets", "uneven", "serving"),
+                    help="run only this workload and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run a 1 GiB device copy 3 times (the known-size "
                          "kernel that scales the PMC FETCH_SIZE/WRITE_SIZE counters)")
@@ -287,6 +289,12 @@ def main():
     from directsht_amd.sht import DirectSHT
 
     torch.cuda.set_device(0)
+    if args.only:
+        fn = {"weight_sets": bench_weight_sets, "uneven": bench_uneven,
+              "serving": lambda: bench_serving(nreq=args.serve_requests,
+                                               warm=min(100, args.serve_requests // 10))}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
     Nl, Nx, N = 1000, 2048, int(args.npoints)
     Nlm = Nl * (Nl + 1) // 2
     log(f"generating {N} lognormal cap points")

@@ -80,7 +80,10 @@ static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
   z.points = std::max<int64_t>(Np, 1);
   z.chunks = (Np + chunk - 1) / chunk;
   z.runs = z.chunks + num_segments(Nx) + 1;
-  z.tiles = std::max<int64_t>(1, (Np + tile - 1) / tile);
+  // tiles of the largest transform up to Np points: sort_tile() keeps every
+  // size at or below ~512 tiles, and sizes below 2^21 use ceil(Np / 4096)
+  z.tiles = std::max<int64_t>(1, std::max<int64_t>((Np + tile - 1) / tile,
+                                                   std::min<int64_t>((Np + 4095) / 4096, 512)));
   return z;
 }
 

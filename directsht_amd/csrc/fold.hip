@@ -32,7 +32,7 @@ __global__ __launch_bounds__(256) void k_seg_clear(int nseg, int* __restrict__ l
                                                    unsigned* __restrict__ rmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nseg) lo[s] = hi[s] = 0;
-  if (s == 0) *rmax = 0u;
+  if (s == 0 && rmax) *rmax = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_seg_runs(const int* __restrict__ run_seg,
@@ -123,15 +123,27 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
         *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
     }
   }
-  // per-set magnitude bound for the matrix-core contraction's scaling
-  for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-  if ((threadIdx.x & 63) == 0 && vmax > 0.f) atomicMax(rmax, __float_as_uint(vmax));
+  // per-set magnitude bound for the matrix-core contraction's scaling (only
+  // when that contraction follows): wavefront, then block, then one atomic
+  if (rmax) {
+    __shared__ float wmax[kFoldNp];
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float bm = wmax[0];
+      for (int i = 1; i < kFoldNp; ++i) bm = fmaxf(bm, wmax[i]);
+      if (bm > 0.f) atomicMax(rmax, __float_as_uint(bm));
+    }
+  }
 }
 
 int launch_fold(Plan& pl, int nsets, int set, hipStream_t s) {
   Workspace& ws = pl.ws;
   const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
-  unsigned* rmax = ws.rmax + set;
+  const bool scaled = pl.precision == DSHT_FP32 && nsets >= kMfmaMinSets &&
+                      pl.contract == DSHT_CONTRACT_MATRIX;
+  unsigned* rmax = scaled ? ws.rmax + set : nullptr;
   k_seg_clear<<<(pl.nseg + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi, rmax);
   DSHT_LAUNCH_CHECK();
   const int64_t max_runs = (int64_t)ws.cur_chunks + pl.nseg + 1;

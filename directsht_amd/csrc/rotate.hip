@@ -153,17 +153,134 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
   }
 }
 
+// Many sets: G sets per workgroup (up to 32), one thread per m (rounds of
+// 256), the recursion for each (l, m) generated once and applied to all G
+// sets.  The a'_{l m'} of the G sets are streamed through LDS in chunks of
+// kRotChunk values of m' (all threads walk m' = l .. 1 in the same order), so
+// the LDS footprint does not grow with l.
+constexpr int kRotChunk = 32;
+
 template <int G>
-static int launch_rotate_g(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
-                           int* status, hipStream_t s) {
-  const size_t lds = sizeof(double) * (2 + 2 * G) * (size_t)pl.Nl;
-  if (lds > 64 * 1024)
-    DSHT_HIP(hipFuncSetAttribute((const void*)k_rotate_combine<G>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+__global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ a_equ,
+                                                     const double* __restrict__ a_pol,
+                                                     double* __restrict__ alm, int Nl, int nsets,
+                                                     const int* __restrict__ flags, int flag_stride,
+                                                     int* __restrict__ status,
+                                                     const double* __restrict__ rot_top,
+                                                     const int* __restrict__ rot_exp,
+                                                     const double* __restrict__ rot_AB,
+                                                     const double* __restrict__ rot_iB) {
+  const int ell = blockIdx.x;
+  const int s0 = blockIdx.y * G;
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  __shared__ double sv[kRotChunk][G][2];  // a'_{l m'}, m' = hi - row
+  __shared__ double sAB[kRotChunk], sBi[kRotChunk];
+  __shared__ int smode[G];
+  if (threadIdx.x < G) {
+    const int set = s0 + threadIdx.x;
+    int md = 0;  // 0 not in block, 1 zeros, 2 equatorial only, 3 rotate and combine
+    if (set < nsets) {
+      const int fl = flags[set * flag_stride];
+      const int npol = flags[set * flag_stride + 1];
+      if (status && ell == 0) status[set] = fl;
+      md = (fl & DSHT_STATUS_POLAR_RANGE) ? 1 : (npol == 0 ? 2 : 3);
+    }
+    smode[threadIdx.x] = md;
+  }
+  __syncthreads();
+  bool any_rot = false;
+#pragma unroll
+  for (int g = 0; g < G; ++g) any_rot |= smode[g] == 3;
+  const int64_t tri = (int64_t)ell * (ell + 1) / 2;
+  for (int m0 = 0; m0 <= ell; m0 += blockDim.x) {
+    const int m = m0 + threadIdx.x;
+    const bool live = m <= ell;
+    double accE[G], accO[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) accE[g] = accO[g] = 0.0;
+    double cur = live ? rot_top[tri + m] : 0.0, nxt = 0.0;
+    int E = live ? rot_exp[tri + m] : 0;
+    const double twom = 2.0 * m;
+    const int cm = m & 1;  // component of even steps: re for even m, im for odd m
+    if (any_rot) {
+      for (int hi = ell; hi >= 1; hi -= kRotChunk) {
+        const int lo = max(1, hi - kRotChunk + 1);
+        const int n = hi - lo + 1;
+        __syncthreads();  // the previous chunk has been consumed
+        for (int e = threadIdx.x; e < n * G; e += blockDim.x) {
+          const int row = e / G, g = e % G;
+          const int64_t idx = hp_index(ell, hi - row, Nl);
+          const bool r = smode[g] == 3;
+          const double* ap = a_pol + ((int64_t)(s0 + g) * Nlm + idx) * 2;
+          sv[row][g][0] = r ? ap[0] : 0.0;
+          sv[row][g][1] = r ? ap[1] : 0.0;
+        }
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+          sAB[e] = rot_AB[tri + hi - e];
+          sBi[e] = rot_iB[tri + hi - e];
+        }
+        __syncthreads();
+        if (live) {
+          for (int row = 0; row < n; ++row) {
+            // step s = l - m' (m' = hi - row): d_{m',m} times Re a' if s + m is
+            // even and Im a' if odd; even steps accumulate in accE, odd in accO
+            const double dv = ldexp(cur, E);
+            if (((ell - hi + row) & 1) == 0) {  // uniform branch
+#pragma unroll
+              for (int g = 0; g < G; ++g) accE[g] = fma(dv, sv[row][g][cm], accE[g]);
+            } else {
+#pragma unroll
+              for (int g = 0; g < G; ++g) accO[g] = fma(dv, sv[row][g][cm ^ 1], accO[g]);
+            }
+            // d_{m'-1} = d_m' * (2m/B) - d_{m'+1} * (A/B); values stay below 2^1023
+            // since they are rescaled as soon as they pass 2^400
+            const double nw = fma(cur, twom * sBi[row], -nxt * sAB[row]);
+            nxt = cur;
+            cur = nw;
+            if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
+              cur *= kScaleLo;
+              nxt *= kScaleLo;
+              E += 400;
+            }
+          }
+        }
+      }
+    }
+    if (live) {
+      const double d0 = ldexp(cur, E);  // d_{0,m}
+      const int64_t idx = hp_index(ell, m, Nl);
+      const int64_t idx0 = hp_index(ell, 0, Nl);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int md = smode[g];
+        if (md == 0) continue;
+        const int64_t o = ((int64_t)(s0 + g) * Nlm + idx) * 2;
+        double re = 0.0, im = 0.0;
+        if (md >= 2) {
+          re = a_equ[o];
+          im = a_equ[o + 1];
+        }
+        if (md == 3) {
+          const double* a0 = a_pol + ((int64_t)(s0 + g) * Nlm + idx0) * 2;
+          const double sr = cm ? accO[g] : accE[g];
+          const double si = cm ? accE[g] : accO[g];
+          re += fma(d0, a0[0], 2.0 * sr);
+          im += fma(d0, a0[1], 2.0 * si);
+        }
+        alm[o] = re;
+        alm[o + 1] = im;
+      }
+    }
+  }
+}
+
+template <int G>
+static int launch_rotate_sets(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
+                              int* status, hipStream_t s) {
   dim3 grid(pl.Nl, (nsets + G - 1) / G);
-  k_rotate_combine<G><<<grid, 256, lds, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, nsets, flags,
-                                             flag_stride, status, pl.rot_top, pl.rot_exp, pl.rot_AB,
-                                             pl.rot_iB);
+  k_rotate_sets<G><<<grid, 256, 0, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, nsets, flags,
+                                        flag_stride, status, pl.rot_top, pl.rot_exp, pl.rot_AB,
+                                        pl.rot_iB);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
@@ -172,12 +289,18 @@ static int launch_rotate_g(Plan& pl, int nsets, const int* flags, int flag_strid
 // (may be NULL) receives the set's DSHT_STATUS_* bits.
 int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
                           int* status, hipStream_t s) {
-  const size_t per = sizeof(double) * (size_t)pl.Nl;  // LDS bytes per row of Nl doubles
-  const size_t cap = 160 * 1024;                       // LDS of one gfx950 workgroup
-  if (nsets >= 8 && 18 * per <= cap) return launch_rotate_g<8>(pl, nsets, flags, flag_stride, alm, status, s);
-  if (nsets >= 4 && 10 * per <= cap) return launch_rotate_g<4>(pl, nsets, flags, flag_stride, alm, status, s);
-  if (nsets >= 2 && 6 * per <= cap) return launch_rotate_g<2>(pl, nsets, flags, flag_stride, alm, status, s);
-  return launch_rotate_g<1>(pl, nsets, flags, flag_stride, alm, status, s);
+  if (nsets == 1) {
+    const size_t lds = sizeof(double) * 4 * (size_t)pl.Nl;
+    k_rotate_combine<1><<<pl.Nl, 256, lds, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, 1, flags,
+                                                flag_stride, status, pl.rot_top, pl.rot_exp,
+                                                pl.rot_AB, pl.rot_iB);
+    DSHT_LAUNCH_CHECK();
+    return DSHT_OK;
+  }
+  if (nsets > 16) return launch_rotate_sets<32>(pl, nsets, flags, flag_stride, alm, status, s);
+  if (nsets > 8) return launch_rotate_sets<16>(pl, nsets, flags, flag_stride, alm, status, s);
+  if (nsets > 4) return launch_rotate_sets<8>(pl, nsets, flags, flag_stride, alm, status, s);
+  return launch_rotate_sets<4>(pl, nsets, flags, flag_stride, alm, status, s);
 }
 
 }  // namespace dsht

@@ -2,33 +2,34 @@
 //   v_k(m, seg) = sum_{p in seg} h_k(p) e^{-i m phi_p},  k = 0..3,
 // the v's of sht/shared_interp_funcs.py:88-105, as a GEMM on the MFMA units.
 //
-// Factor m = 1024 mt + 32 m1 + m2 (mt = blockIdx.y, m1, m2 in [0, 32)):
-//   e^{-i m phi} = E(m1) B(m2),  E = e^{-i (1024 mt + 32 m1) phi},  B = e^{-i m2 phi}
+// Factor m = T mt + 32 m1 + m2 (mt = blockIdx.y, m1 < M1, m2 < 32, m tile
+// T = 32 M1 = 1024, 512 or 256 orders, the narrowest that covers Nl):
+//   e^{-i m phi} = E(m1) B(m2),  E = e^{-i (T mt + 32 m1) phi},  B = e^{-i m2 phi}
 // so for one run (stretch of one segment) of points p
 //   V_k(m) = sum_p [h_k(p) E_p(m1)] B_p(m2)
 // is a complex GEMM with rows (k, m1) and columns m2; written as one real GEMM
-// with rows (k, m1, re|im) = 256 and columns (re|im, m2) = 64:
+// with rows (k, m1, re|im) = 8 M1 and columns (re|im, m2) = 64:
 //   V_re = C[re][re] - C[im][im],  V_im = C[re][im] + C[im][re].
 // It runs as v_mfma_f32_32x32x16_f16 with each operand split into fp16
 // hi + lo (x = hi + lo to ~2^-22 relative, or 2^-25 absolute where lo is
 // subnormal, which the MFMA honours) and three products hi*hi + hi*lo + lo*hi
-// accumulated in FP32.  The MFMA adds every product to its accumulator with
-// an FP32 rounding, so a long run would accumulate ~3 roundings per point at
-// the size of the running sum; the accumulator is therefore restarted from
-// zero every 4 windows (64 points) and added into an outer FP32 sum.
+// accumulated in FP32 (the accumulator carries a whole run of at most one
+// chunk of points; kGroup > 0 restarts it every kGroup windows into an outer
+// FP32 sum).
 //
-// One workgroup (8 wavefronts) per (chunk of sorted points, m tile of 1024
-// orders).  The chunk is staged 128 points at a time: thread (point p, group
-// g = 0..3) generates B for m2 in [8g, 8g+8) and E for m1 in [8g, 8g+8) by
-// FP64 chains (steps e1 = e^{-i phi} and e1^32) into LDS, B split into fp16
-// hi/lo in [m2][point] order so every MFMA operand is one aligned 16-byte LDS
-// read.  The block is then consumed in 16-aligned windows (one MFMA K step).
-// A window inside one run (almost all of them) builds A = h_k E unmasked; a
-// window that contains a segment boundary is issued once per run, with the
-// points of other runs masked to zero in A.  Wave w owns row tile w (k = w/2,
-// 16 values of m1) and issues 6 MFMAs per window.  A run's result goes to its
-// own slot part[run][k][m][2] (FP32), as for the vector scatter; the 32 lanes
-// of a half-wave write 32 consecutive m (256 contiguous bytes).
+// One workgroup (M1/4 wavefronts, one per 32-row tile) per (chunk of sorted
+// points, m tile).  The chunk is staged KB points at a time (128 for the
+// 1024-wide tile, 64 otherwise): thread (point p, group g) generates B for a
+// range of m2 and E for a range of m1 by FP64 chains (steps e1 = e^{-i phi}
+// and e1^32) into LDS, B split into fp16 hi/lo in [m2][point] order so every
+// MFMA operand is one aligned 16-byte LDS read.  The block is then consumed in
+// 16-aligned windows (one MFMA K step).  A window inside one run (almost all
+// of them) builds A = h_k E unmasked; a window that contains a segment
+// boundary is issued once per run, with the points of other runs masked to
+// zero in A.  Wave w owns row tile w (row blocks 2w, 2w+1 of 8 values of m1
+// each) and issues 6 MFMAs per window.  A run's result goes to its own slot
+// part[run][k][m][2] (FP32), as for the vector scatter; the 32 lanes of a
+// half-wave write 32 consecutive m (256 contiguous bytes).
 //
 // Operand layouts of v_mfma_f32_32x32x16_f16 (checked on gfx950):
 //   A: lane l holds A[l%32][8*(l/32) + j],  j = 0..7
@@ -67,15 +68,13 @@ __device__ inline cplx cpow(cplx e, int n) {
 }  // namespace
 
 constexpr int kWin = 16;    // points per MFMA K step
-constexpr int kBlock = 128; // points staged in LDS per generation phase
 // Windows per MFMA accumulator restart into an outer FP32 sum; 0 = the MFMA
 // accumulator carries the whole run.
 constexpr int kGroup = 0;
-// LDS row strides padded to 4 (mod 32) dwords: the 32 lanes of a half-wave
-// read 32 different rows (m2 or m1) at the same column, and with this stride
-// each group of 8 lanes covers all 32 banks with its 16-byte reads.
-constexpr int kRowB = kBlock + 8;   // halves per B row (68 dwords)
-constexpr int kRowE = kBlock + 4;   // floats per E row (132 dwords)
+// LDS rows are padded to 4 (mod 32) dwords (KB + 8 halves, KB + 4 floats):
+// the 32 lanes of a half-wave read 32 different rows (m2 or m1) at the same
+// column, and with this stride each group of 8 lanes covers all 32 banks with
+// its 16-byte reads.
 
 // A operand of one window pass: this lane's 8 points (K slice) of row
 // (k, m1, re|im): h_k(p) E(m1, p), with points outside [p0, p1) zeroed when
@@ -106,13 +105,33 @@ __device__ inline void mfma6(const half8& ahi, const half8& alo, const half8& br
   acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bih, acc[1], 0, 0, 0);
 }
 
-__global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restrict__ rec,
-                                                      const int* __restrict__ seg_start, int nseg,
-                                                      int chunk, int Nl,
-                                                      const int* __restrict__ run_base,
-                                                      const double* __restrict__ wsort,
-                                                      float* __restrict__ part,
-                                                      int* __restrict__ run_seg) {
+__device__ inline cplx cpow2k(cplx x, int k) {  // x^(2^k)
+  for (int i = 0; i < k; ++i) x = csq(x);
+  return x;
+}
+
+template <int M1, int KB>
+__global__ __launch_bounds__(16 * M1, 4) void k_scatter_mfma(const PointRec* __restrict__ rec,
+                                                          const int* __restrict__ seg_start,
+                                                          int nseg, int chunk, int Nl,
+                                                          const int* __restrict__ run_base,
+                                                          const double* __restrict__ wsort,
+                                                          float* __restrict__ part,
+                                                          int* __restrict__ run_seg) {
+  constexpr int kWaves = M1 / 4;              // row tiles: 4 k * M1 m1 * (re, im) / 32
+  constexpr int kThreads = 64 * kWaves;
+  constexpr int kBlk = KB;                    // points staged per generation phase
+  constexpr int kRB = KB + 8;                 // padded row strides (4 mod 32 dwords)
+  constexpr int kRE = KB + 4;
+  constexpr int kGen = kThreads / kBlk;       // generation groups per point: 1, 2 or 4
+  constexpr int kBPer = 32 / kGen;            // m2 values per generation thread
+  constexpr int kEPer = M1 / kGen;            // m1 values per generation thread
+  constexpr int kHPer = 4 / kGen;             // Hermite weights per generation thread
+  constexpr int kQ = kBPer > kEPer ? kBPer : kEPer;
+  constexpr int kMT = 32 * M1;                // orders per m tile
+  constexpr int kMBlocks = M1 / 8;            // blocks of 8 m1 values per k
+  static_assert(M1 % 8 == 0 && kThreads % kBlk == 0 && kGen <= 4 && kEPer >= 1,
+                "unsupported m tile / staging block");
   const int c = blockIdx.x, mt = blockIdx.y;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int nvalid = seg_start[nseg];
@@ -121,16 +140,18 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
 
   // B = e^{-i m2 phi} as fp16 hi/lo, [re|im][hi|lo][m2][point]; E FP32
   // [re|im][m1][point]; h_k FP32 [k][point]; segment of each point
-  __shared__ __attribute__((aligned(16))) _Float16 sB[2][2][32][kRowB];
-  __shared__ __attribute__((aligned(16))) float sE[2][32][kRowE];
-  __shared__ __attribute__((aligned(16))) float sH[4][kBlock];
-  __shared__ int sSeg[kBlock];
+  __shared__ __attribute__((aligned(16))) _Float16 sB[2][2][32][kRB];
+  __shared__ __attribute__((aligned(16))) float sE[2][M1][kRE];
+  __shared__ __attribute__((aligned(16))) float sH[4][kBlk];
+  __shared__ int sSeg[kBlk];
 
-  // wave w owns row tile w: k = w/2, m1 = 16 (w&1) + 8g + e; rows i = 16g + 8c + e
+  // wave w owns row tile w; its rows i = 16g + 8c + e are row block 2w + g
+  // (k = (2w + g) / kMBlocks, m1 = 8 ((2w + g) % kMBlocks) + e), part c
   const int ia = l & 31, bq = l >> 5;
-  const int kA = w >> 1;
   const int gA = ia >> 4, cA = (ia >> 3) & 1;
-  const int m1A = 16 * (w & 1) + 8 * gA + (ia & 7);
+  const int bA = 2 * w + gA;
+  const int kA = bA / kMBlocks;
+  const int m1A = 8 * (bA % kMBlocks) + (ia & 7);
 
   floatx16 acc[2], out[2];  // [column tile re | im]
   const floatx16 zero = {};
@@ -142,44 +163,53 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
   int cur = __builtin_amdgcn_readfirstlane(rec[b].seg);
   int run_end = min(e, seg_start[cur + 1]);
 
-  for (int blk = b; blk < e; blk += kBlock) {
-    const int bn = min(kBlock, e - blk);
+  for (int blk = b; blk < e; blk += kBlk) {
+    const int bn = min(kBlk, e - blk);
     {
-      // generation: thread (point p, group g = 0..3, uniform per wavefront):
-      // B for m2 = 8g + q and E for m1 = 8g + q, q = 0..7, by FP64 chains
-      const int p = tid & (kBlock - 1);
-      const int g = __builtin_amdgcn_readfirstlane(tid >> 7);  // = w / 2
+      // generation: thread (point p, group g, uniform per wavefront): B for
+      // m2 in [kBPer g, kBPer (g+1)) and E for m1 in [kEPer g, kEPer (g+1))
+      // by FP64 chains
+      const int p = tid & (kBlk - 1);
+      const int g = __builtin_amdgcn_readfirstlane(tid / kBlk);
       cplx e1{1.0, 0.0};
-      float hval = 0.f;
       int sg = -1;
       if (p < bn) {
         const PointRec& pr = rec[blk + p];
         e1 = cplx{pr.ere, pr.eim};
-        const double hk = g == 0 ? pr.h0 : g == 1 ? pr.h1 : g == 2 ? pr.h2 : pr.h3;
-        hval = (float)(wsort ? wsort[blk + p] * hk : hk);
-        sg = pr.seg;
-      }
-      sH[g][p] = hval;
-      if (g == 0) sSeg[p] = sg;
-      const cplx e8 = csq(csq(csq(e1)));
-      const cplx e32 = csq(csq(e8));
-      const cplx e256 = csq(csq(csq(e32)));
-      cplx bz = cpow(e8, g);    // e1^(8g)
-      cplx ez = cpow(e256, g);  // e1^(256g)
-      if (mt > 0) ez = cmul(ez, cpow(csq(csq(e256)), mt));  // * e1^(1024mt)
+        const double ws = wsort ? wsort[blk + p] : 1.0;
+        const double hk[4] = {pr.h0, pr.h1, pr.h2, pr.h3};
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int m = 8 * g + q;
-        uint32_t hi, lo;
-        split2_f16((float)bz.x, (float)bz.y, hi, lo);
-        sB[0][0][m][p] = half_lo(hi);
-        sB[1][0][m][p] = half_hi(hi);
-        sB[0][1][m][p] = half_lo(lo);
-        sB[1][1][m][p] = half_hi(lo);
-        sE[0][m][p] = (float)ez.x;
-        sE[1][m][p] = (float)ez.y;
-        bz = cmul(bz, e1);
-        ez = cmul(ez, e32);
+        for (int j = 0; j < kHPer; ++j) sH[g * kHPer + j][p] = (float)(ws * hk[g * kHPer + j]);
+        sg = pr.seg;
+      } else {
+#pragma unroll
+        for (int j = 0; j < kHPer; ++j) sH[g * kHPer + j][p] = 0.f;
+      }
+      if (g == 0) sSeg[p] = sg;
+      const cplx e32 = cpow2k(e1, 5);
+      constexpr int lgB = kBPer == 8 ? 3 : kBPer == 16 ? 4 : 5;
+      constexpr int lgE = kEPer == 1 ? 0 : kEPer == 2 ? 1 : kEPer == 4 ? 2 : kEPer == 8 ? 3 : 4;
+      constexpr int lgM1 = M1 == 8 ? 3 : M1 == 16 ? 4 : 5;
+      cplx bz = cpow(cpow2k(e1, lgB), g);             // e1^(kBPer g)
+      cplx ez = cpow(cpow2k(e32, lgE), g);            // e1^(32 kEPer g)
+      if (mt > 0) ez = cmul(ez, cpow(cpow2k(e32, lgM1), mt));  // * e1^(kMT mt)
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) {
+        if (q < kBPer) {
+          const int m = kBPer * g + q;
+          uint32_t hi, lo;
+          split2_f16((float)bz.x, (float)bz.y, hi, lo);
+          sB[0][0][m][p] = half_lo(hi);
+          sB[1][0][m][p] = half_hi(hi);
+          sB[0][1][m][p] = half_lo(lo);
+          sB[1][1][m][p] = half_hi(lo);
+          bz = cmul(bz, e1);
+        }
+        if (q < kEPer) {
+          sE[0][kEPer * g + q][p] = (float)ez.x;
+          sE[1][kEPer * g + q][p] = (float)ez.y;
+          ez = cmul(ez, e32);
+        }
       }
     }
     __syncthreads();
@@ -222,21 +252,25 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
           wgroup = run_done ? 0 : 1;
         }
         if (run_done) {
-          // flush: lane holds (k = kA, m1 = 16(w&1) + 8g + 4(l/32) + d, m2 = l%32)
+          // flush: registers 8g + d (re row) and 8g + 4 + d (im row) hold row
+          // block 2w + g: k = (2w+g) / kMBlocks, m1 = 8 ((2w+g) % kMBlocks) + 4(l/32) + d
 #pragma unroll
-          for (int g = 0; g < 2; ++g)
+          for (int g = 0; g < 2; ++g) {
+            const int bb = 2 * w + g;
+            const int kk = bb / kMBlocks;
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
-              const int m = 1024 * mt + 32 * (16 * (w & 1) + 8 * g + 4 * bq + d) + (l & 31);
+              const int m = kMT * mt + 32 * (8 * (bb % kMBlocks) + 4 * bq + d) + (l & 31);
               if (m < Nl) {
                 const floatx16& r0 = kGroup > 0 ? out[0] : acc[0];
                 const floatx16& r1 = kGroup > 0 ? out[1] : acc[1];
                 const float vre = r0[8 * g + d] - r1[8 * g + 4 + d];
                 const float vim = r1[8 * g + d] + r0[8 * g + 4 + d];
-                *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + kA) * Nl + m) * 2) =
+                *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + kk) * Nl + m) * 2) =
                     make_float2(vre, vim);
               }
             }
+          }
           if (tid == 0 && mt == 0) run_seg[run] = cur;
           if constexpr (kGroup > 0) out[0] = out[1] = zero;
           if (p < e) {
@@ -255,14 +289,23 @@ __global__ __launch_bounds__(512, 4) void k_scatter_mfma(const PointRec* __restr
   }
 }
 
+template <int M1, int KB>
+static int launch_m1(Plan& pl, int nchunks, const double* wsort, hipStream_t s) {
+  dim3 grid(nchunks, (pl.Nl + 32 * M1 - 1) / (32 * M1));
+  k_scatter_mfma<M1, KB><<<grid, 16 * M1, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
+                                              pl.ws.run_base, wsort, (float*)pl.ws.part,
+                                              pl.ws.run_seg);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+// m tile of 1024, 512 or 256 orders: the narrowest that covers Nl, or 1024
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
-  dim3 grid(nchunks, (pl.Nl + 1023) / 1024);
-  k_scatter_mfma<<<grid, 512, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
-                                      pl.ws.run_base, wsort, (float*)pl.ws.part, pl.ws.run_seg);
-  DSHT_LAUNCH_CHECK();
-  return DSHT_OK;
+  if (pl.Nl <= 256) return launch_m1<8, 64>(pl, nchunks, wsort, s);
+  if (pl.Nl <= 512) return launch_m1<16, 64>(pl, nchunks, wsort, s);
+  return launch_m1<32, 128>(pl, nchunks, wsort, s);
 }
 
 }  // namespace dsht

@@ -237,6 +237,24 @@ def test_scatter_engines_agree(torch_cuda):
         assert e < 1e-6 and r < 1e-6, eng
 
 
+@pytest.mark.parametrize("Nl", [40, 256, 257, 512, 513])
+def test_matrix_scatter_tile_widths(torch_cuda, Nl):
+    """The matrix-core scatter's m tile (256, 512 or 1024 orders, chosen by Nl)
+    at and beside each width: FP32 matrix and vector engines against the FP64
+    path."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    Nx = 2 * Nl
+    rng = np.random.default_rng(Nl)
+    t, p, w = (torch.from_numpy(a).cuda() for a in _uniform(rng, 30000))
+    ref = DirectSHT(Nl, Nx, dflt_type="float64")(t, p, w).cpu().numpy()
+    for eng in ("matrix", "vector"):
+        s = DirectSHT(Nl, Nx, dflt_type="float32")
+        s.configure(scatter=eng)
+        a = s(t, p, w).cpu().numpy()
+        assert rel_max(a, ref) < 1e-6 and rel_rms(a, ref) < 1e-6, eng
+
+
 @pytest.mark.parametrize("Nl", [1100, 1536])
 def test_high_l_second_m_tile(torch_cuda, Nl):
     """Nl > 1024 uses a second m tile (matrix scatter) / more lanes (vector)."""
