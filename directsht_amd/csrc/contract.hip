@@ -20,6 +20,20 @@ namespace dsht {
 
 constexpr int kInner = 16;  // node pairs per FP32 partial sum
 
+// Table entries are read exactly once per transform: non-temporal loads keep
+// the stream from displacing the node coefficients (reused by every row
+// group of an (m, P)) in the caches.
+__device__ inline float4 stream_load(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline double4 stream_load(const double4* p) {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_double4(v.x, v.y, v.z, v.w);
+}
+
 __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ tab,
                                                    const float4* __restrict__ rhs,
                                                    const GroupDesc* __restrict__ groups, int Npair,
@@ -41,7 +55,7 @@ __global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ ta
     const int np1 = min(Npair, np0 + kInner);
 #pragma unroll 8
     for (int np = np0; np < np1; ++np) {
-      const float4 y = valid ? t[np * stride] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 y = valid ? stream_load(t + np * stride) : make_float4(0.f, 0.f, 0.f, 0.f);
       const float4 qer = q[np * qs + 0], qei = q[np * qs + 1];
       const float4 qpr = q[np * qs + 2], qpi = q[np * qs + 3];
       fer = fmaf(y.x, qer.x, fmaf(y.y, qer.y, fmaf(y.z, qer.z, fmaf(y.w, qer.w, fer))));
@@ -82,7 +96,7 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
   double aer = 0, aei = 0, apr = 0, api = 0;
 #pragma unroll 4
   for (int np = 0; np < Npair; ++np) {
-    const double4 y = valid ? t[np * stride] : make_double4(0, 0, 0, 0);
+    const double4 y = valid ? stream_load(t + np * stride) : make_double4(0, 0, 0, 0);
     const double4 qer = q[np * qs + 0], qei = q[np * qs + 1];
     const double4 qpr = q[np * qs + 2], qpi = q[np * qs + 3];
     aer = fma(y.x, qer.x, fma(y.y, qer.y, fma(y.z, qer.z, fma(y.w, qer.w, aer))));

@@ -20,12 +20,14 @@
 //
 // One workgroup (8 wavefronts) per row group.  Per chunk of 16 node pairs
 // (4 MFMA K steps) the workgroup stages the table block (64 rows x 64 K),
-// split into hi/lo, in LDS in the MFMA A-operand layout; wavefront w owns
-// column tile w (32 columns = 8 sets) and both row tiles, loads its B
-// operands straight from the node coefficients rhs[m][P][np][set][16] (32
-// consecutive columns = 512 contiguous bytes per node pair) and issues 6
-// MFMAs per K step.  Accumulators restart every chunk and are added into an
-// outer FP32 sum.
+// split into hi/lo, in LDS in the MFMA A-operand layout (double-buffered, one
+// barrier per chunk); wavefront w owns column tile w (32 columns = 8 sets)
+// and both row tiles, loads its B operands straight from the node
+// coefficients rhs[m][P][np][set][16] (32 consecutive columns = 512
+// contiguous bytes per node pair) and issues 6 MFMAs per K step.  The next
+// chunk's table entries and B operands are loaded into registers while the
+// current chunk is multiplied.  Accumulators restart every chunk and are
+// added into an outer FP32 sum.
 #include "common.h"
 #include "plan.h"
 #include "split.h"
@@ -41,8 +43,8 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
     const unsigned* __restrict__ rmax, double* __restrict__ a_equ, double* __restrict__ a_pol) {
   const GroupDesc gd = groups[blockIdx.x];
   if (gd.nrows == 0) return;  // padding of the XCD schedule
-  // [K step][row tile][hi|lo][lane]: 8 halves per lane, as one 16-byte read
-  __shared__ __attribute__((aligned(16))) u32x4 sA[4][2][2][64];
+  // [buffer][K step][row tile][hi|lo][lane]: 8 halves per lane, one 16-byte read
+  __shared__ __attribute__((aligned(16))) u32x4 sA[2][4][2][2][64];
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63, h = l >> 5;
   const int ncol = 4 * nsets, nct = (ncol + 31) >> 5;
   const bool active = w < nct;  // wave-uniform
@@ -62,9 +64,24 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
   const floatx16 zero = {};
   floatx16 out[2] = {zero, zero}, acc[2];
 
-  for (int c0 = 0; c0 < Npair; c0 += kCK) {
-    // this lane's B operands for the chunk: K step t, node pairs c0+4t+2h+{0,1}
-    float4 bq[4][2];
+  // loads of one chunk: this thread's two table entries (node pair c0 + e/64,
+  // row e%64, e = tid + 512 v) and this lane's B operands (K step t, node
+  // pairs c0 + 4t + 2h + {0, 1})
+  float4 ya[2], bq[4][2];
+  auto load = [&](int c0) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const int e = tid + kCThreads * v;
+      const int np = c0 + (e >> 6), r = e & 63;
+      if (np < Npair && r < gd.nrows) {  // read once: non-temporal
+        typedef float v4 __attribute__((ext_vector_type(4)));
+        const v4 t4 = __builtin_nontemporal_load(
+            reinterpret_cast<const v4*>(tab + gd.off + (int64_t)np * gd.nrows + r));
+        ya[v] = make_float4(t4.x, t4.y, t4.z, t4.w);
+      } else {
+        ya[v] = zero4;
+      }
+    }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -72,13 +89,16 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
         const int np = c0 + 4 * t + 2 * h + u;
         bq[t][u] = (colv && np < Npair) ? *reinterpret_cast<const float4*>(qb + np * qstride) : zero4;
       }
-    // stage the table block: entry e = (node pair c0 + e/64, row e%64)
+  };
+  load(0);
+  int buf = 0;
+  for (int c0 = 0; c0 < Npair; c0 += kCK, buf ^= 1) {
+    // stage this chunk's table block (split into hi/lo) into LDS buffer `buf`
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
       const int e = tid + kCThreads * v;
       const int u = e >> 6, r = e & 63;
-      const int np = c0 + u;
-      float4 y = (np < Npair && r < gd.nrows) ? tab[gd.off + (int64_t)np * gd.nrows + r] : zero4;
+      float4 y = ya[v];
       y.z *= dx;
       y.w *= dx;
       uint32_t h01, l01, h23, l23;
@@ -87,22 +107,35 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
       // K index within the step: 4 (u % 4) + q -> lane half (u % 4) / 2, slot 4 ((u % 4) % 2) + q
       const int t = u >> 2, hh = (u >> 1) & 1, half = u & 1;
       const int rt = r >> 5, lane = hh * 32 + (r & 31);
-      reinterpret_cast<u32x2*>(&sA[t][rt][0][lane])[half] = (u32x2){h01, h23};
-      reinterpret_cast<u32x2*>(&sA[t][rt][1][lane])[half] = (u32x2){l01, l23};
+      reinterpret_cast<u32x2*>(&sA[buf][t][rt][0][lane])[half] = (u32x2){h01, h23};
+      reinterpret_cast<u32x2*>(&sA[buf][t][rt][1][lane])[half] = (u32x2){l01, l23};
     }
+    // B of this chunk, scaled; then the next chunk's loads go in flight
+    float x[4][8];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      x[t][0] = bq[t][0].x * bsc;
+      x[t][1] = bq[t][0].y * bsc;
+      x[t][2] = bq[t][0].z * bscd;
+      x[t][3] = bq[t][0].w * bscd;
+      x[t][4] = bq[t][1].x * bsc;
+      x[t][5] = bq[t][1].y * bsc;
+      x[t][6] = bq[t][1].z * bscd;
+      x[t][7] = bq[t][1].w * bscd;
+    }
+    if (c0 + kCK < Npair) load(c0 + kCK);
+    // one barrier per chunk: every wave has finished the chunk that last
+    // used buffer `buf` before anyone stages into it again
     __syncthreads();
     if (active) {
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const float x[8] = {bq[t][0].x * bsc,  bq[t][0].y * bsc,  bq[t][0].z * bscd,
-                            bq[t][0].w * bscd, bq[t][1].x * bsc,  bq[t][1].y * bsc,
-                            bq[t][1].z * bscd, bq[t][1].w * bscd};
         half8 bhi, blo;
-        split8_f16(x, bhi, blo);
+        split8_f16(x[t], bhi, blo);
 #pragma unroll
         for (int rt = 0; rt < 2; ++rt) {
-          const half8 ahi = __builtin_bit_cast(half8, sA[t][rt][0][l]);
-          const half8 alo = __builtin_bit_cast(half8, sA[t][rt][1][l]);
+          const half8 ahi = __builtin_bit_cast(half8, sA[buf][t][rt][0][l]);
+          const half8 alo = __builtin_bit_cast(half8, sA[buf][t][rt][1][l]);
           acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bhi, t == 0 ? zero : acc[rt], 0, 0, 0);
           acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, blo, acc[rt], 0, 0, 0);
           acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bhi, acc[rt], 0, 0, 0);
@@ -111,7 +144,6 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
       out[0] += acc[0];
       out[1] += acc[1];
     }
-    __syncthreads();
   }
 
   if (colv) {
