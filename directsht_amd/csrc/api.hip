@@ -99,8 +99,9 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
   b += align256(z.runs * (int64_t)Nl * 8 * (precision == DSHT_FP32 ? 4 : 8));
   b += 6 * align256((nseg + 2) * sizeof(int));
   b += align256(z.tiles * (nseg + 1) * sizeof(int));
-  b += align256((int64_t)Nl * 2 * Npair * 16 * (precision == DSHT_FP32 ? 4 : 8));
+  b += align256((int64_t)Nl * 2 * Npair * 24 * (precision == DSHT_FP32 ? 4 : 8));
   b += 2 * align256(Nlm * 2 * sizeof(double));
+  b += align256((int64_t)kContractSplits * Nlm * 8 * sizeof(double));
   b += 256;
   return b;
 }
@@ -110,7 +111,7 @@ static void free_workspace(Workspace& ws) {
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
                   (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
                   (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.rmax, (void*)ws.wsort,
-                  (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.flags})
+                  (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.a_der, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
 }
@@ -131,17 +132,27 @@ static void free_workspace(Workspace& ws) {
 // sets, and the sorted-weight array if `wsort` is asked for.  Growth waits
 // for the device so buffers still in use by queued work are never freed
 // under it.
+// Node-coefficient elements per (m, P, node pair) for a transform of nsets
+// sets: one 24-value slot for the plain contraction, 16 per set for the
+// matrix-core contraction.
+static int64_t rhs_per_pair(const Plan& pl, int nsets) {
+  return use_mfma_contract(pl, nsets) ? std::max<int64_t>(24, 16 * (int64_t)nsets) : 24;
+}
+
 static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false) {
   const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
   Workspace& ws = pl.ws;
+  const int64_t rhs_need = (int64_t)pl.Nl * 2 * pl.Npair * rhs_per_pair(pl, nsets);
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
-      z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort))
+      z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort) &&
+      rhs_need <= ws.rhs_elems)
     return DSHT_OK;
   DSHT_HIP(hipDeviceSynchronize());
   WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
                std::max(z.runs, ws.cap_runs), std::max(z.tiles, ws.cap_tiles)};
   const int sets = std::max(nsets, ws.cap_sets);
   const bool with_w = wsort || ws.has_wsort;
+  const int64_t rhs_elems = std::max(rhs_need, ws.rhs_elems);
   free_workspace(ws);
   const int fsz = pl.precision == DSHT_FP32 ? 4 : 8;
   DSHT_ALLOC(ws.rec, want.points * sizeof(PointRec));
@@ -156,11 +167,12 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   DSHT_ALLOC(ws.seg_run_hi, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.ne_rank, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.thist, want.tiles * (pl.nseg + 1) * sizeof(int));
-  DSHT_ALLOC(ws.rhs, (int64_t)pl.Nl * 2 * pl.Npair * 16 * sets * fsz);
+  DSHT_ALLOC(ws.rhs, rhs_elems * fsz);
   DSHT_ALLOC(ws.rmax, sets * sizeof(unsigned));
   if (with_w) DSHT_ALLOC(ws.wsort, want.points * sizeof(double));
   DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sets * sizeof(double));
   DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sets * sizeof(double));
+  DSHT_ALLOC(ws.a_der, (int64_t)kContractSplits * pl.Nlm * 8 * sizeof(double));
   DSHT_ALLOC(ws.flags, 4 * sets * sizeof(int));
   ws.cap_points = want.points;
   ws.cap_chunks = want.chunks;
@@ -168,6 +180,7 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   ws.cap_tiles = want.tiles;
   ws.cap_sets = sets;
   ws.has_wsort = with_w;
+  ws.rhs_elems = rhs_elems;
   return DSHT_OK;
 }
 
@@ -205,8 +218,11 @@ static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int
   }
 }
 
+// alloc_tables: the plan's Y-only table (FP64 or FP32 by precision), built
+// if `build`; caller: instead an FP64 {Y, Y'} table that the caller's tables
+// are copied into (csht do_transform).
 static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool alloc_tables,
-                     bool build) {
+                     bool build, bool caller = false) {
   if (Nl < 2 || Nx < 2 || !(xmax > 0.0 && xmax < 1.0)) {
     set_error("dsht: need Nl >= 2, Nx >= 2 and 0 < xmax < 1");
     return DSHT_EINVAL;
@@ -240,8 +256,12 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   DSHT_ALLOC(pl.mp_off, mp_off.size() * sizeof(int64_t));
   DSHT_HIP(hipMemcpy(pl.groups, groups.data(), groups.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
   DSHT_HIP(hipMemcpy(pl.mp_off, mp_off.data(), mp_off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-  if (alloc_tables && precision == DSHT_FP64) DSHT_ALLOC(pl.tab64, pl.tab_vec4 * 4 * sizeof(double));
-  if (alloc_tables && precision == DSHT_FP32) DSHT_ALLOC(pl.tab32, pl.tab_vec4 * 4 * sizeof(float));
+  pl.caller_tables = caller;
+  if (caller) DSHT_ALLOC(pl.tab64, pl.tab_vec4 * 4 * sizeof(double));
+  if (alloc_tables && !caller && precision == DSHT_FP64)
+    DSHT_ALLOC(pl.tab64y, pl.tab_vec4 * 2 * sizeof(double));
+  if (alloc_tables && !caller && precision == DSHT_FP32)
+    DSHT_ALLOC(pl.tab32y, pl.tab_vec4 * 2 * sizeof(float));
   std::vector<double> top, A, iB;
   std::vector<int> ex;
   rotation_constants(Nl, top, ex, A, iB);
@@ -256,13 +276,44 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   pl.table_bytes = ws.bytes;
   ws.bytes = 0;
   pl.table_hold = std::make_shared<TableOwner>(
-      pl.device, std::vector<void*>{pl.groups, pl.mp_off, pl.tab64, pl.tab32, pl.rot_top,
-                                    pl.rot_exp, pl.rot_AB, pl.rot_iB});
-  if (alloc_tables && build) {
-    int rc = launch_tables(pl, precision == DSHT_FP64, precision == DSHT_FP32, nullptr, nullptr, 0);
+      pl.device, std::vector<void*>{pl.groups, pl.mp_off, pl.tab64y, pl.tab32y, pl.tab64,
+                                    pl.rot_top, pl.rot_exp, pl.rot_AB, pl.rot_iB});
+  if (alloc_tables && build && !caller) {
+    int rc = precision == DSHT_FP64 ? launch_tables(pl, 0, pl.tab64y, nullptr, 0)
+                                    : launch_tables(pl, 1, pl.tab32y, nullptr, 0);
     if (rc) return rc;
     DSHT_HIP(hipDeviceSynchronize());
   }
+  return DSHT_OK;
+}
+
+// The FP32 {Y, Y'} table of the matrix-core contraction: built on first use
+// (shared by the plan's clones); waits for the device once.
+static int ensure_full32(Plan& pl, const float** out) {
+  TableOwner& o = *pl.table_hold;
+  std::lock_guard<std::mutex> lock(o.mu);
+  if (!o.full32) {
+    const int64_t bytes = pl.tab_vec4 * 4 * (int64_t)sizeof(float);
+    float* t = nullptr;
+    hipError_t e = hipMalloc((void**)&t, (size_t)bytes);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc(") + std::to_string(bytes) + " bytes) for the {Y, Y'} table: " +
+                hipGetErrorString(e));
+      return DSHT_ENOMEM;
+    }
+    int rc = launch_tables(pl, 2, t, nullptr, 0);
+    if (rc == DSHT_OK && hipDeviceSynchronize() != hipSuccess) {
+      set_error("dsht: building the {Y, Y'} table failed");
+      rc = DSHT_EHIP;
+    }
+    if (rc) {
+      (void)hipFree(t);
+      return rc;
+    }
+    o.full32 = t;
+    o.full32_bytes = bytes;
+  }
+  *out = o.full32;
   return DSHT_OK;
 }
 
@@ -272,8 +323,9 @@ static void plan_free(Plan& pl) {
   if (pl.table_hold) {
     pl.table_hold.reset();  // frees the tables with the last plan sharing them
   } else {                  // plan_init failed before handing them to an owner
-    for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64, (void*)pl.tab32,
-                    (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB, (void*)pl.rot_iB})
+    for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64y, (void*)pl.tab32y,
+                    (void*)pl.tab64, (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB,
+                    (void*)pl.rot_iB})
       if (p) (void)hipFree(p);
   }
   pl = Plan{};
@@ -346,6 +398,9 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   }
   int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared);
   if (rc) return rc;
+  const bool mfma = use_mfma_contract(pl, a.nsets);
+  const float* full32 = nullptr;
+  if (mfma && (rc = ensure_full32(pl, &full32))) return rc;
   Workspace& ws = pl.ws;
   hipEvent_t* ev = nullptr;
   if (pl.timing && a.nsets == 1 && !a.shared) {
@@ -368,6 +423,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       if ((rc = launch_gather_weights(pl, Np, a.wsets + set * a.ldw, s))) return rc;
       if ((rc = launch_scatter(pl, Np, ws.wsort, s))) return rc;
       if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
+      if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
     }
   } else {
     for (int set = 0; set < a.nsets; ++set) {
@@ -380,9 +436,12 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       if ((rc = mark(2))) return rc;
       if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
       if ((rc = mark(3))) return rc;
+      if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
     }
   }
-  if ((rc = launch_contract(pl, a.nsets, s))) return rc;
+  // the plain contraction ran per set, right after its fold (one coefficient
+  // slot); the matrix-core contraction takes all slots at once
+  if (mfma && (rc = launch_contract_mfma(pl, full32, a.nsets, s))) return rc;
   if ((rc = mark(4))) return rc;
   if ((rc = launch_rotate_combine(pl, a.nsets, ws.flags, a.shared ? 0 : 4, alm, status, s))) return rc;
   if ((rc = mark(5))) return rc;
@@ -462,7 +521,7 @@ int dsht_plan_debug_rhs(dsht_plan* p, void* out, int64_t bytes) {
     return DSHT_EINVAL;
   }
   Plan& pl = p->pl;
-  const int64_t need = (int64_t)pl.Nl * 2 * pl.Npair * 16 * (pl.precision == DSHT_FP32 ? 4 : 8);
+  const int64_t need = (int64_t)pl.Nl * 2 * pl.Npair * 24 * (pl.precision == DSHT_FP32 ? 4 : 8);
   if (bytes < need || !pl.ws.rhs) {
     set_error("dsht_plan_debug_rhs: buffer too small or no transform yet");
     return DSHT_EINVAL;
@@ -563,8 +622,10 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.groups = s.groups;
   d.mp_off = s.mp_off;
   d.tab_vec4 = s.tab_vec4;
+  d.tab64y = s.tab64y;
+  d.tab32y = s.tab32y;
   d.tab64 = s.tab64;
-  d.tab32 = s.tab32;
+  d.caller_tables = s.caller_tables;
   d.table_bytes = s.table_bytes;
   d.rot_top = s.rot_top;
   d.rot_exp = s.rot_exp;
@@ -581,7 +642,12 @@ int dsht_plan_reserve(dsht_plan* p, int64_t Np, int nsets) {
     return DSHT_EINVAL;
   }
   DeviceGuard g(p->pl.device);
-  return ensure_workspace(p->pl, Np, nsets, nsets > 1);
+  int rc = ensure_workspace(p->pl, Np, nsets, nsets > 1);
+  if (rc == DSHT_OK && use_mfma_contract(p->pl, nsets)) {
+    const float* full32 = nullptr;
+    rc = ensure_full32(p->pl, &full32);  // so that later transforms allocate nothing
+  }
+  return rc;
 }
 
 int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {
@@ -589,7 +655,12 @@ int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {
     set_error("dsht_plan_footprint: null argument");
     return DSHT_EINVAL;
   }
-  *bytes = p->pl.table_bytes + p->pl.ws.bytes;
+  int64_t full = 0;
+  if (p->pl.table_hold) {
+    std::lock_guard<std::mutex> lock(p->pl.table_hold->mu);
+    full = p->pl.table_hold->full32_bytes;
+  }
+  *bytes = p->pl.table_bytes + full + p->pl.ws.bytes;
   return DSHT_OK;
 }
 
@@ -607,7 +678,7 @@ int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table
   groups = xcd_schedule(groups, kXcds);
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   if (table_bytes)
-    *table_bytes = vec4 * 4 * (precision == DSHT_FP32 ? 4 : 8) + Nlm * (3 * 8 + 4) +
+    *table_bytes = vec4 * 2 * (precision == DSHT_FP32 ? 4 : 8) + Nlm * (3 * 8 + 4) +
                    (int64_t)groups.size() * sizeof(GroupDesc) + (int64_t)mp_off.size() * 8;
   if (work_bytes) *work_bytes = workspace_bytes(Nl, Nx, precision, Np, 2048);
   return DSHT_OK;
@@ -627,7 +698,7 @@ int dsht_plan_get_tables(dsht_plan* p, double* Yv, double* Yd) {
     set_error("dsht_plan_get_tables: out of device memory");
     return DSHT_ENOMEM;
   }
-  int rc = launch_tables(pl, false, false, dv, dd, 0);
+  int rc = launch_tables(pl, 3, dv, dd, 0);
   if (rc == DSHT_OK) {
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess && Yv) e = hipMemcpy(Yv, dv, bytes, hipMemcpyDeviceToHost);
@@ -718,7 +789,7 @@ int make_table(int Nl, int Nx, double xmax, double Yv[], double Yd[]) {
       set_error("make_table: out of device memory");
       rc = DSHT_ENOMEM;
     } else {
-      rc = launch_tables(pl, false, false, dv, dd, 0);
+      rc = launch_tables(pl, 3, dv, dd, 0);
       if (rc == DSHT_OK) {
         if (hipDeviceSynchronize() != hipSuccess ||
             hipMemcpy(Yv, dv, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -740,7 +811,7 @@ int do_transform(int Nl, int Nx, double xmax, double Yv[], double Yd[], int Np, 
   static std::mutex mu;
   std::lock_guard<std::mutex> lock(mu);
   Plan pl;
-  int rc = plan_init(pl, Nl, Nx, xmax, DSHT_FP64, true, false);
+  int rc = plan_init(pl, Nl, Nx, xmax, DSHT_FP64, true, false, true);
   const size_t tb = (size_t)pl.Nlm * Nx * sizeof(double);
   double *dv = nullptr, *dd = nullptr, *dc = nullptr, *dp = nullptr, *dw = nullptr, *da = nullptr;
   std::vector<double> alm;

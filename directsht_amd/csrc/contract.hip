@@ -4,78 +4,180 @@
 // of sht/interp_funcs_py.py:46-60 / sht/interp_funcs_jax.py:31-48 and
 // csht/sht_helper.c:255-274, as one HBM-streaming GEMV.
 //
+// The plan's own tables hold Y only; the Y' term comes from the derivative
+// relation (the fold's coefficients f, g):
+//   sum_n Y'_l d_P = G_l sum_n Y_{l-1} g - l sum_n Y_l f_P,
+// so each row l accumulates S_c = sum Y_l c_P, S_f = sum Y_l f_P and
+// S_g = sum Y_l g (g for the row l+1 of the other parity); k_add_der then
+// forms a_lm = S_c - l S_f + G_l S_g(l-1).  Half the table bytes of a
+// {Y, Y'} table.
+//
 // One wavefront per row group (up to 64 rows l of one (m, P)), launched in an
 // XCD-aware order (xcd_schedule in api.hip); lane r owns row
-// l = m + P + 2(i0 + r) and walks the node pairs.  Each step loads one
-// 4-vector {Y(x_2np), Y(x_2np+1), Y'(x_2np), Y'(x_2np+1)} per lane (a
-// contiguous 1 KB (FP32) / 2 KB (FP64) block per wavefront) and multiplies it
-// with the wave-uniform node coefficients of that pair, which the compiler
-// keeps in scalar registers.  FP32 tables accumulate 16 node pairs in FP32 and
-// then in FP64; FP64 tables accumulate in FP64.  The result goes straight to
-// the healpy index of (l, m) (stage (d) pack).
+// l = m + P + 2(i0 + r) and walks the node pairs.  Each step loads one pair
+// {Y(x_2np), Y(x_2np+1)} per lane (a contiguous 512-byte (FP32) / 1 KB (FP64)
+// block per wavefront) and multiplies it with the wave-uniform coefficients of
+// that pair, which the compiler keeps in scalar registers.  FP32 tables
+// accumulate 16 node pairs in FP32 and then in FP64; FP64 tables accumulate
+// in FP64.  Results go straight to the healpy index of (l, m).
+//
+// Caller-provided {Y, Y'} tables (csht do_transform) are contracted as given
+// by k_contract64.
 #include "common.h"
 #include "plan.h"
 
 namespace dsht {
 
-constexpr int kInner = 16;  // node pairs per FP32 partial sum
-
 // Table entries are read exactly once per transform: non-temporal loads keep
 // the stream from displacing the node coefficients (reused by every row
 // group of an (m, P)) in the caches.
-__device__ inline float4 stream_load(const float4* p) {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
-  return make_float4(v.x, v.y, v.z, v.w);
-}
 __device__ inline double4 stream_load(const double4* p) {
   typedef double v4 __attribute__((ext_vector_type(4)));
   const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
   return make_double4(v.x, v.y, v.z, v.w);
 }
 
-__global__ __launch_bounds__(64) void k_contract32(const float4* __restrict__ tab,
-                                                   const float4* __restrict__ rhs,
+__device__ inline float2 stream_load(const float2* p) {
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ inline double2 stream_load(const double2* p) {
+  typedef double v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return make_double2(v.x, v.y);
+}
+
+template <typename T>
+struct Pair;
+template <>
+struct Pair<float> {
+  using type = float2;
+  static constexpr int kRing = 16;  // node pairs in flight per lane = pairs per FP32 partial sum
+};
+template <>
+struct Pair<double> {
+  using type = double2;
+  static constexpr int kRing = 8;
+};
+
+// Table loads run kRing node pairs ahead of the multiply (a ring of
+// registers), so that each wavefront keeps kRing loads in flight; lanes past
+// the group's rows load row 0 and discard it (no branch around the loads).
+// A scheduling fence after each pair keeps one pair's coefficients (24 scalar
+// registers) live at a time.
+//
+// Work item = (row group, node range): the node pairs are split into
+// gridDim.y ranges, so that the launch has several times more equal-length
+// wavefronts than the GPU holds at once (one wavefront per group would run
+// in ~1.4 rounds, the second mostly idle).  Each item writes its partial
+// sums part[range][h][8] = {S_c - l S_f (4), S_g (4)}; k_add_der adds the
+// ranges in order (deterministic).
+template <typename T>
+__global__ __launch_bounds__(64) void k_contract_y(const typename Pair<T>::type* __restrict__ tab,
+                                                   const T* __restrict__ rhs,
                                                    const GroupDesc* __restrict__ groups, int Npair,
-                                                   int Nl, int nsets, int set,
-                                                   double* __restrict__ a_equ,
-                                                   double* __restrict__ a_pol) {
+                                                   int Nl, int64_t Nlm, double* __restrict__ part) {
+  using T2 = typename Pair<T>::type;
+  constexpr int kRing = Pair<T>::kRing;
   const GroupDesc gd = groups[blockIdx.x];
   if (gd.nrows == 0) return;  // padding of the XCD schedule
+  const int nsplit = gridDim.y, split = blockIdx.y;
+  const int lo = (int)((int64_t)Npair * split / nsplit);
+  const int hi = (int)((int64_t)Npair * (split + 1) / nsplit);
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
-  const float4* t = tab + gd.off + r;
-  // the set's four coefficient vectors of node pair np start at q[np * qs]
-  const float4* q = rhs + ((int64_t)(gd.m * 2 + gd.P) * Npair * nsets + set) * 4;
-  const int64_t qs = 4 * (int64_t)nsets;
+  const T2* t = tab + gd.off + (valid ? r : 0);
+  // [c | f | g][er, ei, pr, pi][node] of node pair np at q[24 np]
+  const T* q = rhs + (int64_t)(gd.m * 2 + gd.P) * Npair * 24;
   const int64_t stride = gd.nrows;
-  double aer = 0, aei = 0, apr = 0, api = 0;
-  for (int np0 = 0; np0 < Npair; np0 += kInner) {
-    float fer = 0.f, fei = 0.f, fpr = 0.f, fpi = 0.f;
-    const int np1 = min(Npair, np0 + kInner);
-#pragma unroll 8
-    for (int np = np0; np < np1; ++np) {
-      const float4 y = valid ? stream_load(t + np * stride) : make_float4(0.f, 0.f, 0.f, 0.f);
-      const float4 qer = q[np * qs + 0], qei = q[np * qs + 1];
-      const float4 qpr = q[np * qs + 2], qpi = q[np * qs + 3];
-      fer = fmaf(y.x, qer.x, fmaf(y.y, qer.y, fmaf(y.z, qer.z, fmaf(y.w, qer.w, fer))));
-      fei = fmaf(y.x, qei.x, fmaf(y.y, qei.y, fmaf(y.z, qei.z, fmaf(y.w, qei.w, fei))));
-      fpr = fmaf(y.x, qpr.x, fmaf(y.y, qpr.y, fmaf(y.z, qpr.z, fmaf(y.w, qpr.w, fpr))));
-      fpi = fmaf(y.x, qpi.x, fmaf(y.y, qpi.y, fmaf(y.z, qpi.z, fmaf(y.w, qpi.w, fpi))));
+  T2 ring[kRing];
+#pragma unroll
+  for (int k = 0; k < kRing; ++k) ring[k] = stream_load(t + (int64_t)min(lo + k, hi - 1) * stride);
+  double S[3][4] = {};
+  auto pair = [&](T (&f)[3][4], T2 y, int np) {
+    if (!valid) y = T2{0, 0};
+    const T* qq = q + (int64_t)np * 24;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int o = 0; o < 4; ++o)
+        f[k][o] = fma(y.x, qq[8 * k + 2 * o], fma(y.y, qq[8 * k + 2 * o + 1], f[k][o]));
+  };
+  auto flush = [&](T (&f)[3][4]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int o = 0; o < 4; ++o) S[k][o] += (double)f[k][o];
+  };
+  int np0 = lo;
+  for (; np0 + kRing <= hi; np0 += kRing) {  // full rounds of the ring
+    T f[3][4] = {};
+#pragma unroll
+    for (int k = 0; k < kRing; ++k) {
+      // consume the slot, then refill the same registers kRing pairs ahead
+      pair(f, ring[k], np0 + k);
+      ring[k] = stream_load(t + (int64_t)min(np0 + kRing + k, hi - 1) * stride);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    aer += fer;
-    aei += fei;
-    apr += fpr;
-    api += fpi;
+    flush(f);
+  }
+  {  // the last hi - np0 < kRing pairs are already in the ring
+    T f[3][4] = {};
+#pragma unroll
+    for (int k = 0; k < kRing; ++k)
+      if (np0 + k < hi) pair(f, ring[k], np0 + k);
+    flush(f);
   }
   if (valid) {
     const int ell = gd.m + gd.P + 2 * (gd.i0 + r);
-    const int64_t idx = hp_index(ell, gd.m, Nl) + (int64_t)set * Nl * (Nl + 1) / 2;
-    a_equ[2 * idx] = aer;
-    a_equ[2 * idx + 1] = aei;
-    a_pol[2 * idx] = apr;
-    a_pol[2 * idx + 1] = api;
+    const int64_t h = hp_index(ell, gd.m, Nl);
+    double4* o = reinterpret_cast<double4*>(part + ((int64_t)split * Nlm + h) * 8);
+    o[0] = make_double4(S[0][0] - ell * S[1][0], S[0][1] - ell * S[1][1], S[0][2] - ell * S[1][2],
+                        S[0][3] - ell * S[1][3]);
+    o[1] = make_double4(S[2][0], S[2][1], S[2][2], S[2][3]);
   }
+}
+
+// a_lm = sum over ranges of (S_c - l S_f) + G_l sum over ranges of S_g(l-1, m)
+// (l > m), G_l = sqrt((l^2 - m^2)(2l+1)/(2l-1)): the partial sums of
+// k_contract_y combined with the rest of the Y' term (thread per (l, m))
+__global__ __launch_bounds__(256) void k_add_der(int Nl, int64_t Nlm, int nsplit, int64_t set_off,
+                                                 const double* __restrict__ part,
+                                                 double* __restrict__ a_equ,
+                                                 double* __restrict__ a_pol) {
+  const int ell = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (ell < m || ell >= Nl) return;
+  const int64_t h = hp_index(ell, m, Nl);
+  double4 a = make_double4(0, 0, 0, 0), d = make_double4(0, 0, 0, 0);
+  for (int sp = 0; sp < nsplit; ++sp) {
+    const double4 v = *reinterpret_cast<const double4*>(part + ((int64_t)sp * Nlm + h) * 8);
+    a.x += v.x;
+    a.y += v.y;
+    a.z += v.z;
+    a.w += v.w;
+    if (ell > m) {
+      const double4 u = *reinterpret_cast<const double4*>(part + ((int64_t)sp * Nlm + h - 1) * 8 + 4);
+      d.x += u.x;
+      d.y += u.y;
+      d.z += u.z;
+      d.w += u.w;
+    }
+  }
+  if (ell > m) {
+    const double G = sqrt((double)(ell - m) * (double)(ell + m) * (double)(2 * ell + 1) /
+                          (double)(2 * ell - 1));
+    a.x += G * d.x;
+    a.y += G * d.y;
+    a.z += G * d.z;
+    a.w += G * d.w;
+  }
+  const int64_t idx = h + set_off;
+  a_equ[2 * idx] = a.x;
+  a_equ[2 * idx + 1] = a.y;
+  a_pol[2 * idx] = a.z;
+  a_pol[2 * idx + 1] = a.w;
 }
 
 __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ tab,
@@ -94,7 +196,7 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
   const int64_t qs = 4 * (int64_t)nsets;
   const int64_t stride = gd.nrows;
   double aer = 0, aei = 0, apr = 0, api = 0;
-#pragma unroll 4
+#pragma unroll 2
   for (int np = 0; np < Npair; ++np) {
     const double4 y = valid ? stream_load(t + np * stride) : make_double4(0, 0, 0, 0);
     const double4 qer = q[np * qs + 0], qei = q[np * qs + 1];
@@ -114,21 +216,31 @@ __global__ __launch_bounds__(64) void k_contract64(const double4* __restrict__ t
   }
 }
 
-int launch_contract(Plan& pl, int nsets, hipStream_t s) {
-  if (pl.precision == DSHT_FP32 && nsets >= kMfmaMinSets && pl.contract == DSHT_CONTRACT_MATRIX)
-    return launch_contract_mfma(pl, nsets, s);
-  for (int set = 0; set < nsets; ++set) {
-    if (pl.precision == DSHT_FP32) {
-      k_contract32<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const float4*>(pl.tab32),
-                                             reinterpret_cast<const float4*>(pl.ws.rhs), pl.groups,
-                                             pl.Npair, pl.Nl, nsets, set, pl.ws.a_equ, pl.ws.a_pol);
-    } else {
-      k_contract64<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const double4*>(pl.tab64),
-                                             reinterpret_cast<const double4*>(pl.ws.rhs), pl.groups,
-                                             pl.Npair, pl.Nl, nsets, set, pl.ws.a_equ, pl.ws.a_pol);
-    }
+int launch_contract_set(Plan& pl, int nsets, int set, hipStream_t s) {
+  Workspace& ws = pl.ws;
+  if (pl.caller_tables) {
+    k_contract64<<<pl.ngroups, 64, 0, s>>>(reinterpret_cast<const double4*>(pl.tab64),
+                                           reinterpret_cast<const double4*>(ws.rhs), pl.groups,
+                                           pl.Npair, pl.Nl, nsets, set, ws.a_equ, ws.a_pol);
     DSHT_LAUNCH_CHECK();
+    return DSHT_OK;
   }
+  const int64_t set_off = (int64_t)set * pl.Nlm;
+  const int nsplit = contract_splits(pl);
+  dim3 cgrid(pl.ngroups, nsplit);
+  if (pl.precision == DSHT_FP32) {
+    k_contract_y<float><<<cgrid, 64, 0, s>>>(reinterpret_cast<const float2*>(pl.tab32y),
+                                             static_cast<const float*>(ws.rhs), pl.groups, pl.Npair,
+                                             pl.Nl, pl.Nlm, ws.a_der);
+  } else {
+    k_contract_y<double><<<cgrid, 64, 0, s>>>(reinterpret_cast<const double2*>(pl.tab64y),
+                                              static_cast<const double*>(ws.rhs), pl.groups, pl.Npair,
+                                              pl.Nl, pl.Nlm, ws.a_der);
+  }
+  DSHT_LAUNCH_CHECK();
+  dim3 grid((pl.Nl + 255) / 256, pl.Nl);
+  k_add_der<<<grid, 256, 0, s>>>(pl.Nl, pl.Nlm, nsplit, set_off, ws.a_der, ws.a_equ, ws.a_pol);
+  DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
 

@@ -164,13 +164,13 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
   }
 }
 
-int launch_contract_mfma(Plan& pl, int nsets, hipStream_t s) {
+int launch_contract_mfma(Plan& pl, const float* full32, int nsets, hipStream_t s) {
   if (4 * nsets > 32 * (kCThreads / 64)) {
     set_error("dsht: at most 64 sets per matrix-core contraction");
     return DSHT_EINVAL;
   }
   k_contract_mfma<<<pl.ngroups, kCThreads, 0, s>>>(
-      reinterpret_cast<const float4*>(pl.tab32), reinterpret_cast<const float*>(pl.ws.rhs),
+      reinterpret_cast<const float4*>(full32), reinterpret_cast<const float*>(pl.ws.rhs),
       pl.groups, pl.Npair, pl.Nl, nsets, (float)pl.dx, pl.ws.rmax, pl.ws.a_equ, pl.ws.a_pol);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

@@ -21,6 +21,19 @@
 //   branch b, offset 8b:  re: c(2np), c(2np+1), d(2np), d(2np+1);  im: same.
 // It also records max(|c|, |d/dx|) of the set (as float bits, atomicMax) for
 // the scaling of the matrix-core contraction.
+//
+// For the plain contraction, which streams Y only, the Y' term is rewritten
+// with the derivative relation of the tables (sht/legendre_py.py:54-56,
+// 111-117 in normalized form)
+//   Y'_l(x) = [G_l Y_{l-1}(x) - l x Y_l(x)] / (1 - x^2),
+//   G_l = sqrt((l^2 - m^2)(2l+1)/(2l-1)),
+// so that  sum_n Y'_l(x_n) d_P(n) = G_l sum_n Y_{l-1}(x_n) g(n) - l sum_n Y_l(x_n) f_P(n)
+// with f_P(n) = x_n d_P(n)/(1 - x_n^2) and g(n) = d_P(n)/(1 - x_n^2).  Row l-1
+// has the other parity, so the rows of parity P are contracted with c_P, f_P
+// and g = d_{1-P}/(1 - x^2) (for the row above them).  That fold writes one
+// slot rhs[m][P][np][24]: [c | f | g][er, ei, pr, pi][node 2np, 2np+1].
+#include <type_traits>
+
 #include "common.h"
 #include "plan.h"
 
@@ -60,7 +73,7 @@ struct Vec2<double> {
 constexpr int kFoldM = 64;   // orders per block (one wavefront)
 constexpr int kFoldNp = 4;   // node pairs per block (one wavefront each)
 
-template <typename TP, typename T>
+template <typename TP, typename T, bool kY>
 __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict__ part,
                                                            const int* __restrict__ seg_lo,
                                                            const int* __restrict__ seg_hi,
@@ -73,7 +86,9 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   const bool live = m < Nl && np < Npair;
   float vmax = 0.f;  // max |c|, |d / dx| written by this thread
   const V* pv = reinterpret_cast<const V*>(part);
-  T out[2][16];  // [P][16]
+  constexpr int kOut = kY ? 24 : 16;
+  T out[2][kOut];  // [P][kOut]
+  double dd[2][2][2][2] = {};  // d_P(n) of the Y-only form: [P][branch][re|im][node]
 #pragma unroll
   for (int br = 0; br < 2 && live; ++br) {
     // c, d per (node of the pair, hemisphere), complex
@@ -108,18 +123,38 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
         for (int odd = 0; odd < 2; ++odd) {
           const double cv = c[odd][0][ri] + sg * c[odd][1][ri];
           const double dv = d[odd][0][ri] + sg * d[odd][1][ri];
-          out[P][8 * br + 4 * ri + odd] = (T)cv;
-          out[P][8 * br + 4 * ri + 2 + odd] = (T)(dx * dv);
+          if constexpr (kY) {
+            out[P][2 * (2 * br + ri) + odd] = (T)cv;
+            dd[P][br][ri][odd] = dx * dv;
+          } else {
+            out[P][8 * br + 4 * ri + odd] = (T)cv;
+            out[P][8 * br + 4 * ri + 2 + odd] = (T)(dx * dv);
+          }
           vmax = fmaxf(vmax, (float)fmax(fabs(cv), fabs(dv)));
+        }
+    }
+  }
+  if constexpr (kY) {
+#pragma unroll
+    for (int odd = 0; odd < 2; ++odd) {
+      const double x = (double)(2 * np + odd) * dx;  // node x_n (d = 0 on padding nodes)
+      const double inv = 1.0 / (1.0 - x * x);
+#pragma unroll
+      for (int P = 0; P < 2; ++P)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          out[P][8 + 2 * q + odd] = (T)(x * inv * dd[P][q >> 1][q & 1][odd]);
+          out[P][16 + 2 * q + odd] = (T)(inv * dd[1 - P][q >> 1][q & 1][odd]);
         }
     }
   }
   if (live) {
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
-      T* o = rhs + ((((int64_t)m * 2 + P) * Npair + np) * nsets + set) * 16;
+      T* o = kY ? rhs + (((int64_t)m * 2 + P) * Npair + np) * 24
+                : rhs + ((((int64_t)m * 2 + P) * Npair + np) * nsets + set) * 16;
 #pragma unroll
-      for (int q = 0; q < 16; q += 2)
+      for (int q = 0; q < kOut; q += 2)
         *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
     }
   }
@@ -141,8 +176,8 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
 int launch_fold(Plan& pl, int nsets, int set, hipStream_t s) {
   Workspace& ws = pl.ws;
   const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
-  const bool scaled = pl.precision == DSHT_FP32 && nsets >= kMfmaMinSets &&
-                      pl.contract == DSHT_CONTRACT_MATRIX;
+  const bool scaled = use_mfma_contract(pl, nsets);
+  const bool yform = !scaled && !pl.caller_tables;  // plain contraction of Y-only tables
   unsigned* rmax = scaled ? ws.rmax + set : nullptr;
   k_seg_clear<<<(pl.nseg + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi, rmax);
   DSHT_LAUNCH_CHECK();
@@ -151,14 +186,18 @@ int launch_fold(Plan& pl, int nsets, int set, hipStream_t s) {
                                                           ws.seg_run_hi);
   DSHT_LAUNCH_CHECK();
   dim3 grid((pl.Npair + kFoldNp - 1) / kFoldNp, (pl.Nl + kFoldM - 1) / kFoldM);
+  auto go = [&](auto tp, auto yf) {
+    using T = decltype(tp);
+    k_fold<T, T, decltype(yf)::value><<<grid, kFoldM * kFoldNp, 0, s>>>(
+        (const T*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
+        set, (T*)ws.rhs, rmax);
+  };
   if (pl.precision == DSHT_FP32) {
-    k_fold<float, float><<<grid, kFoldM * kFoldNp, 0, s>>>(
-        (const float*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
-        set, (float*)ws.rhs, rmax);
+    if (yform) go(0.f, std::true_type{});
+    else go(0.f, std::false_type{});
   } else {
-    k_fold<double, double><<<grid, kFoldM * kFoldNp, 0, s>>>(
-        (const double*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
-        set, (double*)ws.rhs, rmax);
+    if (yform) go(0.0, std::true_type{});
+    else go(0.0, std::false_type{});
   }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -29,19 +30,29 @@ struct Workspace {
   int* seg_run_hi = nullptr;  // [nseg + 2]: one past its last run (0 if empty)
   int* ne_rank = nullptr;     // [nseg + 1]: non-empty segments before s
   int* thist = nullptr;       // [cap_tiles][nseg + 1]: tile histograms, then offsets
-  void* rhs = nullptr;        // [Nl][2][Npair][cap_sets][16] float or double
+  void* rhs = nullptr;        // node coefficients (float or double): plain contraction
+                              // [Nl][2][Npair][24] (one set at a time); matrix-core
+                              // contraction and caller tables [Nl][2][Npair][cap_sets][16]
+  int64_t rhs_elems = 0;      // capacity of rhs in elements
   unsigned* rmax = nullptr;   // [cap_sets]: max(|c|, |d/dx|) per set (float bits)
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
   double* a_pol = nullptr;    // [cap_sets][Nlm][2]
+  double* a_der = nullptr;    // [kContractSplits][Nlm][8]: partial sums of the plain
+                              // contraction per node range (one set at a time)
   int* flags = nullptr;       // [cap_sets][4]: status bits, n_polar, unused, unused
   int64_t bytes = 0;
 };
 
-// Frees a plan's table arrays (on the plan's device) when destroyed.
+// Frees a plan's table arrays (on the plan's device) when destroyed.  Also
+// holds the FP32 {Y, Y'} table of the matrix-core contraction, built on first
+// use and shared by every clone of the plan.
 struct TableOwner {
   int device;
   std::vector<void*> ptrs;
+  std::mutex mu;             // guards full32
+  float* full32 = nullptr;   // FP32 {Y, Y'} table (tab_vec4 4-vectors), or NULL
+  int64_t full32_bytes = 0;
   TableOwner(int dev, std::vector<void*> p) : device(dev), ptrs(std::move(p)) {}
   TableOwner(const TableOwner&) = delete;
   TableOwner& operator=(const TableOwner&) = delete;
@@ -51,6 +62,7 @@ struct TableOwner {
     if (prev != device) (void)hipSetDevice(device);
     for (void* q : ptrs)
       if (q) (void)hipFree(q);
+    if (full32) (void)hipFree(full32);
     if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
   }
 };
@@ -69,9 +81,14 @@ struct Plan {
   int ngroups = 0;
   GroupDesc* groups = nullptr;  // device [ngroups]
   int64_t* mp_off = nullptr;    // device [2*Nl]
-  int64_t tab_vec4 = 0;         // 4-vectors in one table copy
+  int64_t tab_vec4 = 0;         // entries (row, node pair) of one table copy
+  // Y-only tables {Y(x_2np), Y(x_2np+1)} per entry: FP64 plans tab64y, FP32
+  // plans tab32y.  tab64: FP64 {Y, Y'} copy of caller-provided tables (csht
+  // do_transform), contracted with the caller's Y' as given.
+  double* tab64y = nullptr;
+  float* tab32y = nullptr;
   double* tab64 = nullptr;
-  float* tab32 = nullptr;
+  bool caller_tables = false;
   int64_t table_bytes = 0;
   // polar rotation constants, per l at offset l(l+1)/2: d^l_{l,m}(pi/2) as
   // mantissa * 2^exp for m = 0..l, A(m')/B(m') and 1/B(m') of the recursion
@@ -102,7 +119,9 @@ std::vector<GroupDesc> xcd_schedule(const std::vector<GroupDesc>& groups, int nx
 int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk);
 
 // Kernel launchers (one per source file).
-int launch_tables(const Plan& pl, bool w64, bool w32, double* hpYv, double* hpYd, hipStream_t s);
+// out: 0 Y-only FP64 (dst double*), 1 Y-only FP32 (dst float*), 2 FP32 {Y, Y'}
+// (dst float*), 3 healpy rows (dst = Yv, hpYd = Yd)
+int launch_tables(const Plan& pl, int out, void* dst, double* hpYd, hipStream_t s);
 int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipStream_t s);
 // binning + counting sort; wt NULL = unit weights; flags: the set's 4 status ints
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
@@ -112,13 +131,26 @@ int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s);
 // wsort NULL: the records' own weights
 int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
-// node/parity fold of the last scatter into slot `set` of nsets
-int launch_fold(Plan& pl, int nsets, int set, hipStream_t s);
-// contraction of all nsets slots into a_equ / a_pol [set][Nlm][2]
 constexpr int kMfmaMinSets = 4;  // FP32 plans: matrix-core contraction from this many sets
 constexpr int kMaxSets = 64;     // sets per batched transform
-int launch_contract(Plan& pl, int nsets, hipStream_t s);
-int launch_contract_mfma(Plan& pl, int nsets, hipStream_t s);
+// the matrix-core contraction handles a transform of nsets sets on this plan
+inline bool use_mfma_contract(const Plan& pl, int nsets) {
+  return pl.precision == DSHT_FP32 && !pl.caller_tables && nsets >= kMfmaMinSets &&
+         pl.contract == DSHT_CONTRACT_MATRIX;
+}
+// node/parity fold of the last scatter: slot `set` of nsets for the
+// matrix-core contraction and caller tables, else the single plain slot
+int launch_fold(Plan& pl, int nsets, int set, hipStream_t s);
+// node ranges of the plain contraction (work items per row group)
+#ifndef DSHT_CONTRACT_SPLITS
+#define DSHT_CONTRACT_SPLITS 2
+#endif
+constexpr int kContractSplits = DSHT_CONTRACT_SPLITS;
+inline int contract_splits(const Plan& pl) { return pl.Npair < kContractSplits ? 1 : kContractSplits; }
+// plain contraction of the folded set `set` (of nsets) into a_equ / a_pol [set][Nlm][2]
+int launch_contract_set(Plan& pl, int nsets, int set, hipStream_t s);
+// matrix-core contraction of all nsets slots (FP32 {Y, Y'} table `full32`)
+int launch_contract_mfma(Plan& pl, const float* full32, int nsets, hipStream_t s);
 int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
                           int* status, hipStream_t s);
 

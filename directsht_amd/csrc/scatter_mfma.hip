@@ -14,8 +14,7 @@
 // hi + lo (x = hi + lo to ~2^-22 relative, or 2^-25 absolute where lo is
 // subnormal, which the MFMA honours) and three products hi*hi + hi*lo + lo*hi
 // accumulated in FP32 (the accumulator carries a whole run of at most one
-// chunk of points; kGroup > 0 restarts it every kGroup windows into an outer
-// FP32 sum).
+// chunk of points).
 //
 // One workgroup (M1/4 wavefronts, one per 32-row tile) per (chunk of sorted
 // points, m tile).  The chunk is staged KB points at a time (128 for the
@@ -27,7 +26,9 @@
 // of them) builds A = h_k E unmasked; a window that contains a segment
 // boundary is issued once per run, with the points of other runs masked to
 // zero in A.  Wave w owns row tile w (row blocks 2w, 2w+1 of 8 values of m1
-// each) and issues 6 MFMAs per window.  A run's result goes to its own slot
+// each) and issues 6 MFMAs per window; the next window's LDS loads and A build
+// are interleaved with them (a wavefront's VALU work overlaps its own MFMAs,
+// not those of other wavefronts on the SIMD).  A run's result goes to its own slot
 // part[run][k][m][2] (FP32), as for the vector scatter; the 32 lanes of a
 // half-wave write 32 consecutive m (256 contiguous bytes).
 //
@@ -38,6 +39,8 @@
 // Rows of a 32-row tile are ordered i = 16g + 8c + e (g: k parity, c: re|im,
 // e: m1 offset), so the re and im rows of one (k, m1) sit in registers r and
 // r + 4 of the same lane.
+#include <type_traits>
+
 #include "common.h"
 #include "plan.h"
 #include "split.h"
@@ -68,9 +71,6 @@ __device__ inline cplx cpow(cplx e, int n) {
 }  // namespace
 
 constexpr int kWin = 16;    // points per MFMA K step
-// Windows per MFMA accumulator restart into an outer FP32 sum; 0 = the MFMA
-// accumulator carries the whole run.
-constexpr int kGroup = 0;
 // LDS rows are padded to 4 (mod 32) dwords (KB + 8 halves, KB + 4 floats):
 // the 32 lanes of a half-wave read 32 different rows (m2 or m1) at the same
 // column, and with this stride each group of 8 lanes covers all 32 banks with
@@ -110,8 +110,19 @@ __device__ inline cplx cpow2k(cplx x, int k) {  // x^(2^k)
   return x;
 }
 
+// Empty register pins: a value pinned at a scheduling fence is computed on
+// that side of it (no hoisting above, no sinking into later blocks).
+template <typename T>
+__device__ inline void pin(T& x) {
+  asm volatile("" : "+v"(x));
+}
+__device__ inline void pin(cplx& z) {
+  pin(z.x);
+  pin(z.y);
+}
+
 template <int M1, int KB>
-__global__ __launch_bounds__(16 * M1, 4) void k_scatter_mfma(const PointRec* __restrict__ rec,
+__global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __restrict__ rec,
                                                           const int* __restrict__ seg_start,
                                                           int nseg, int chunk, int Nl,
                                                           const int* __restrict__ run_base,
@@ -120,30 +131,36 @@ __global__ __launch_bounds__(16 * M1, 4) void k_scatter_mfma(const PointRec* __r
                                                           int* __restrict__ run_seg) {
   constexpr int kWaves = M1 / 4;              // row tiles: 4 k * M1 m1 * (re, im) / 32
   constexpr int kThreads = 64 * kWaves;
-  constexpr int kBlk = KB;                    // points staged per generation phase
+  constexpr int kBlk = KB;                    // points per staged block
   constexpr int kRB = KB + 8;                 // padded row strides (4 mod 32 dwords)
   constexpr int kRE = KB + 4;
   constexpr int kGen = kThreads / kBlk;       // generation groups per point: 1, 2 or 4
   constexpr int kBPer = 32 / kGen;            // m2 values per generation thread
   constexpr int kEPer = M1 / kGen;            // m1 values per generation thread
   constexpr int kHPer = 4 / kGen;             // Hermite weights per generation thread
-  constexpr int kQ = kBPer > kEPer ? kBPer : kEPer;
+  constexpr int kNWin = KB / kWin;            // windows per staged block
+  constexpr int kBStep = kBPer / kNWin;       // chain steps per window
+  constexpr int kEStep = kEPer / kNWin;
   constexpr int kMT = 32 * M1;                // orders per m tile
   constexpr int kMBlocks = M1 / 8;            // blocks of 8 m1 values per k
-  static_assert(M1 % 8 == 0 && kThreads % kBlk == 0 && kGen <= 4 && kEPer >= 1,
+  static_assert(M1 % 8 == 0 && kThreads % kBlk == 0 && kGen <= 4 && kGen >= 1,
                 "unsupported m tile / staging block");
+  static_assert(kBStep * kNWin == kBPer && kEStep * kNWin == kEPer && kEStep >= 1,
+                "chain steps must spread evenly over the windows of a block");
   const int c = blockIdx.x, mt = blockIdx.y;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
   const int nvalid = seg_start[nseg];
   const int b = c * chunk, e = min(nvalid, b + chunk);
   if (b >= e) return;
 
-  // B = e^{-i m2 phi} as fp16 hi/lo, [re|im][hi|lo][m2][point]; E FP32
-  // [re|im][m1][point]; h_k FP32 [k][point]; segment of each point
-  __shared__ __attribute__((aligned(16))) _Float16 sB[2][2][32][kRB];
-  __shared__ __attribute__((aligned(16))) float sE[2][M1][kRE];
-  __shared__ __attribute__((aligned(16))) float sH[4][kBlk];
-  __shared__ int sSeg[kBlk];
+  // two staging buffers: block i is multiplied out of one while block i+1 is
+  // generated into the other.  B = e^{-i m2 phi} as fp16 hi/lo,
+  // [re|im][hi|lo][m2][point]; E FP32 [re|im][m1][point]; h_k FP32
+  // [k][point]; segment of each point
+  __shared__ __attribute__((aligned(16))) _Float16 sB[2][2][2][32][kRB];
+  __shared__ __attribute__((aligned(16))) float sE[2][2][M1][kRE];
+  __shared__ __attribute__((aligned(16))) float sH[2][4][kBlk];
+  __shared__ int sSeg[2][kBlk];
 
   // wave w owns row tile w; its rows i = 16g + 8c + e are row block 2w + g
   // (k = (2w + g) / kMBlocks, m1 = 8 ((2w + g) % kMBlocks) + e), part c
@@ -153,139 +170,283 @@ __global__ __launch_bounds__(16 * M1, 4) void k_scatter_mfma(const PointRec* __r
   const int kA = bA / kMBlocks;
   const int m1A = 8 * (bA % kMBlocks) + (ia & 7);
 
-  floatx16 acc[2], out[2];  // [column tile re | im]
+  floatx16 acc[2];  // [column tile re | im], accumulating the current run
   const floatx16 zero = {};
-  acc[0] = acc[1] = out[0] = out[1] = zero;
-  int wgroup = 0;
+  acc[0] = acc[1] = zero;
 
   // run tracking is wave-uniform (scalar registers, scalar branches)
   int run = __builtin_amdgcn_readfirstlane(run_base[c]);
   int cur = __builtin_amdgcn_readfirstlane(rec[b].seg);
   int run_end = min(e, seg_start[cur + 1]);
+  int blk = b, bn = 0, buf = 0;  // staged block: first point, size, buffer
 
-  for (int blk = b; blk < e; blk += kBlk) {
-    const int bn = min(kBlk, e - blk);
-    {
-      // generation: thread (point p, group g, uniform per wavefront): B for
-      // m2 in [kBPer g, kBPer (g+1)) and E for m1 in [kEPer g, kEPer (g+1))
-      // by FP64 chains
-      const int p = tid & (kBlk - 1);
-      const int g = __builtin_amdgcn_readfirstlane(tid / kBlk);
-      cplx e1{1.0, 0.0};
-      int sg = -1;
-      if (p < bn) {
-        const PointRec& pr = rec[blk + p];
-        e1 = cplx{pr.ere, pr.eim};
-        const double ws = wsort ? wsort[blk + p] : 1.0;
-        const double hk[4] = {pr.h0, pr.h1, pr.h2, pr.h3};
+  // ---- generation: thread (point gp, group gg) makes B for m2 = gg + kGen q
+  // (q < kBPer) and E for m1 = gg + kGen q (q < kEPer) by FP64 chains with
+  // steps e1^kGen and e1^(32 kGen), e1 = e^{-i phi}
+  const int gp = tid % kBlk;
+  const int gg = __builtin_amdgcn_readfirstlane(tid / kBlk);
+  cplx gb{1.0, 0.0}, ge{1.0, 0.0}, gbs{1.0, 0.0}, ges{1.0, 0.0};  // chain values and steps
+  // record of this thread's point in a later block, loaded one block ahead
+  double pf_ere = 1.0, pf_eim = 0.0, pf_w = 0.0, pf_h[kHPer];
+  int pf_seg = -1;
 #pragma unroll
-        for (int j = 0; j < kHPer; ++j) sH[g * kHPer + j][p] = (float)(ws * hk[g * kHPer + j]);
-        sg = pr.seg;
-      } else {
+  for (int j = 0; j < kHPer; ++j) pf_h[j] = 0.0;
+  auto prefetch = [&](int pblk) {
+    const int p = pblk + gp;
+    if (p < e) {
+      // loaded into registers and first used a block later: no value is
+      // selected here (that would wait for the loads)
+      const PointRec& pr = rec[p];
+      pf_ere = pr.ere;
+      pf_eim = pr.eim;
+      const double* hk = &pr.h0 + gg * kHPer;  // h0..h3 are consecutive
 #pragma unroll
-        for (int j = 0; j < kHPer; ++j) sH[g * kHPer + j][p] = 0.f;
-      }
-      if (g == 0) sSeg[p] = sg;
-      const cplx e32 = cpow2k(e1, 5);
-      constexpr int lgB = kBPer == 8 ? 3 : kBPer == 16 ? 4 : 5;
-      constexpr int lgE = kEPer == 1 ? 0 : kEPer == 2 ? 1 : kEPer == 4 ? 2 : kEPer == 8 ? 3 : 4;
-      constexpr int lgM1 = M1 == 8 ? 3 : M1 == 16 ? 4 : 5;
-      cplx bz = cpow(cpow2k(e1, lgB), g);             // e1^(kBPer g)
-      cplx ez = cpow(cpow2k(e32, lgE), g);            // e1^(32 kEPer g)
-      if (mt > 0) ez = cmul(ez, cpow(cpow2k(e32, lgM1), mt));  // * e1^(kMT mt)
+      for (int j = 0; j < kHPer; ++j) pf_h[j] = hk[j];
+      pf_seg = pr.seg;
+      pf_w = wsort ? wsort[p] : 1.0;
+    } else {
+      pf_ere = 1.0;
+      pf_eim = 0.0;
+      pf_w = 0.0;
+      pf_seg = -1;
+    }
+  };
+  // chain start for the prefetched block, h and segments into buffer nbuf
+  auto gen_setup = [&](int nbuf) {
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) {
-        if (q < kBPer) {
-          const int m = kBPer * g + q;
-          uint32_t hi, lo;
-          split2_f16((float)bz.x, (float)bz.y, hi, lo);
-          sB[0][0][m][p] = half_lo(hi);
-          sB[1][0][m][p] = half_hi(hi);
-          sB[0][1][m][p] = half_lo(lo);
-          sB[1][1][m][p] = half_hi(lo);
-          bz = cmul(bz, e1);
-        }
-        if (q < kEPer) {
-          sE[0][kEPer * g + q][p] = (float)ez.x;
-          sE[1][kEPer * g + q][p] = (float)ez.y;
-          ez = cmul(ez, e32);
+    for (int j = 0; j < kHPer; ++j) sH[nbuf][gg * kHPer + j][gp] = (float)(pf_w * pf_h[j]);
+    if (gg == 0) sSeg[nbuf][gp] = pf_seg;
+    const cplx e1{pf_ere, pf_eim};
+    const cplx e2 = csq(e1), e4 = csq(e2), e32 = cpow2k(e4, 3);
+    const cplx e64 = csq(e32), e128 = csq(e64);
+    gbs = kGen == 4 ? e4 : kGen == 2 ? e2 : e1;                 // e1^kGen
+    ges = kGen == 4 ? e128 : kGen == 2 ? e64 : e32;             // e1^(32 kGen)
+    gb = gg == 0 ? cplx{1.0, 0.0} : gg == 1 ? e1 : gg == 2 ? e2 : cmul(e2, e1);        // e1^gg
+    ge = gg == 0 ? cplx{1.0, 0.0} : gg == 1 ? e32 : gg == 2 ? e64 : cmul(e64, e32);    // e1^(32 gg)
+    if (mt > 0) {
+      constexpr int lgT = M1 == 8 ? 3 : M1 == 16 ? 4 : 5;     // kMT = 32 * 2^lgT
+      ge = cmul(ge, cpow(cpow2k(e32, lgT), mt));               // * e1^(kMT mt)
+    }
+  };
+  // chain steps of window j of the block being generated into buffer nbuf
+  auto gen_slice_b = [&](int j, int nbuf) {
+    pin(gb);
+#pragma unroll
+    for (int t = 0; t < kBStep; ++t) {
+      const int m2 = gg + kGen * (j * kBStep + t);
+      uint32_t hi, lo;
+      split2_f16((float)gb.x, (float)gb.y, hi, lo);
+      sB[nbuf][0][0][m2][gp] = half_lo(hi);
+      sB[nbuf][1][0][m2][gp] = half_hi(hi);
+      sB[nbuf][0][1][m2][gp] = half_lo(lo);
+      sB[nbuf][1][1][m2][gp] = half_hi(lo);
+      gb = cmul(gb, gbs);
+    }
+    pin(gb);
+  };
+  auto gen_slice_e = [&](int j, int nbuf) {
+    pin(ge);
+#pragma unroll
+    for (int t = 0; t < kEStep; ++t) {
+      const int m1 = gg + kGen * (j * kEStep + t);
+      sE[nbuf][0][m1][gp] = (float)ge.x;
+      sE[nbuf][1][m1][gp] = (float)ge.y;
+      ge = cmul(ge, ges);
+    }
+    pin(ge);
+  };
+  auto gen_slice = [&](int j, int nbuf) {
+    gen_slice_b(j, nbuf);
+    gen_slice_e(j, nbuf);
+  };
+
+  // ---- windows
+  // this lane's E and h slices of window wo (8 points from wo + 8 (l/32))
+  typedef float f32x4 __attribute__((ext_vector_type(4)));  // 16-byte aligned LDS reads
+  auto load_eh = [&](int wo, float (&ev)[8], float (&hv)[8]) {
+    const int po = wo + 8 * bq;
+    const f32x4 E0 = *reinterpret_cast<const f32x4*>(&sE[buf][cA][m1A][po]);
+    const f32x4 E1 = *reinterpret_cast<const f32x4*>(&sE[buf][cA][m1A][po + 4]);
+    const f32x4 H0 = *reinterpret_cast<const f32x4*>(&sH[buf][kA][po]);
+    const f32x4 H1 = *reinterpret_cast<const f32x4*>(&sH[buf][kA][po + 4]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      ev[j] = E0[j];
+      ev[4 + j] = E1[j];
+      hv[j] = H0[j];
+      hv[4 + j] = H1[j];
+    }
+  };
+  // this lane's B operands of window wo: re hi, re lo, im hi, im lo
+  auto load_b = [&](int wo, half8 (&bv)[4]) {
+    const int po = wo + 8 * bq;
+    bv[0] = *reinterpret_cast<const half8*>(&sB[buf][0][0][l & 31][po]);
+    bv[1] = *reinterpret_cast<const half8*>(&sB[buf][0][1][l & 31][po]);
+    bv[2] = *reinterpret_cast<const half8*>(&sB[buf][1][0][l & 31][po]);
+    bv[3] = *reinterpret_cast<const half8*>(&sB[buf][1][1][l & 31][po]);
+  };
+  // the current run ends at point p: write its slot, clear the accumulators,
+  // move to the next run
+  auto finish_run = [&](int p) {
+    // registers 8g + d (re row) and 8g + 4 + d (im row) hold row block
+    // 2w + g: k = (2w+g) / kMBlocks, m1 = 8 ((2w+g) % kMBlocks) + 4(l/32) + d
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int bb = 2 * w + g;
+      const int kk = bb / kMBlocks;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const int m = kMT * mt + 32 * (8 * (bb % kMBlocks) + 4 * bq + d) + (l & 31);
+        if (m < Nl) {
+          const float vre = acc[0][8 * g + d] - acc[1][8 * g + 4 + d];
+          const float vim = acc[1][8 * g + d] + acc[0][8 * g + 4 + d];
+          *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + kk) * Nl + m) * 2) =
+              make_float2(vre, vim);
         }
       }
     }
-    __syncthreads();
-    for (int wo = 0; wo < bn; wo += kWin) {
-      const int whi = blk + min(wo + kWin, bn);
+    acc[0] = acc[1] = zero;
+    if (tid == 0 && mt == 0) run_seg[run] = cur;
+    if (p < e) {
+      ++run;
+      const int off = p - blk;  // point p may lie past this block
+      if (off < bn)
+        cur = __builtin_amdgcn_readfirstlane(sSeg[buf][off]);
+      else
+        cur = __builtin_amdgcn_readfirstlane(rec[p].seg);
+      run_end = min(e, seg_start[cur + 1]);
+    }
+  };
+  // window wo: MFMAs with its operands (cb, ca); interleaved with them, the
+  // next window's operands (nb, na), if kNext, and, if kGenNext, slice
+  // wo / kWin of the next block's generation
+  auto step = [&](auto gen_next, auto next, int wo, half8 (&cb)[4], half8 (&ca)[2],
+                  half8 (&nb)[4], half8 (&na)[2]) {
+    constexpr bool kGenNext = decltype(gen_next)::value;
+    constexpr bool kNext = decltype(next)::value;
+    const int whi = blk + min(wo + kWin, bn);
+    const int j = wo / kWin;
+    float ev[8], hv[8];
+    if (whi <= run_end) {  // uniform: the whole window lies in the current run
+      // explicit order (scheduling fences): the LDS loads first, then each
+      // MFMA followed by a share of the VALU work, so that the wavefront
+      // issues that work while its MFMAs execute
+      if constexpr (kNext) {
+        load_eh(wo + kWin, ev, hv);
+        load_b(wo + kWin, nb);
+      }
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ca[0], cb[0], acc[0], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (kGenNext) gen_slice_b(j, buf ^ 1);
+      __builtin_amdgcn_sched_barrier(0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ca[0], cb[2], acc[1], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (kGenNext) gen_slice_e(j, buf ^ 1);
+      __builtin_amdgcn_sched_barrier(0);
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ca[0], cb[1], acc[0], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      uint32_t nh[4], nl[4];
+      if constexpr (kNext) {  // next A, hi parts: products and their fp16 rounding
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float2v v = {hv[2 * q] * ev[2 * q], hv[2 * q + 1] * ev[2 * q + 1]};
+          nh[q] = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, half2v));
+          pin(nh[q]);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ca[0], cb[3], acc[1], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (kNext) {  // next A, lo parts: fp16(x - hi), x - hi exact in FP32
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const half2v hh = __builtin_bit_cast(half2v, nh[q]);
+          half2v ll;
+          ll[0] = (_Float16)__builtin_fmaf((float)hh[0], -1.0f, hv[2 * q] * ev[2 * q]);
+          ll[1] = (_Float16)__builtin_fmaf((float)hh[1], -1.0f, hv[2 * q + 1] * ev[2 * q + 1]);
+          nl[q] = __builtin_bit_cast(uint32_t, ll);
+          pin(nl[q]);
+        }
+        na[0] = __builtin_bit_cast(half8, (u32x4){nh[0], nh[1], nh[2], nh[3]});
+        na[1] = __builtin_bit_cast(half8, (u32x4){nl[0], nl[1], nl[2], nl[3]});
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ca[1], cb[0], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ca[1], cb[2], acc[1], 0, 0, 0);
+      if (whi == run_end) finish_run(whi);
+    } else {
+      // a segment boundary inside the window: one masked pass per run
+      load_eh(wo, ev, hv);
+      const int pg0 = blk + wo + 8 * bq;
       int p = blk + wo;
-      // operands of this window that do not depend on the run split
-      const int po = wo + 8 * bq;  // first point of this lane's K slice
-      const float4 E0 = *reinterpret_cast<const float4*>(&sE[cA][m1A][po]);
-      const float4 E1 = *reinterpret_cast<const float4*>(&sE[cA][m1A][po + 4]);
-      const float4 H0 = *reinterpret_cast<const float4*>(&sH[kA][po]);
-      const float4 H1 = *reinterpret_cast<const float4*>(&sH[kA][po + 4]);
-      const half8 brh = *reinterpret_cast<const half8*>(&sB[0][0][l & 31][po]);
-      const half8 brl = *reinterpret_cast<const half8*>(&sB[0][1][l & 31][po]);
-      const half8 bih = *reinterpret_cast<const half8*>(&sB[1][0][l & 31][po]);
-      const half8 bil = *reinterpret_cast<const half8*>(&sB[1][1][l & 31][po]);
-      const float ev[8] = {E0.x, E0.y, E0.z, E0.w, E1.x, E1.y, E1.z, E1.w};
-      const float hv[8] = {H0.x, H0.y, H0.z, H0.w, H1.x, H1.y, H1.z, H1.w};
       while (p < whi) {
         const int vhi = min(whi, run_end);  // points [p, vhi) belong to run `cur`
-        half8 ahi, alo;
-        const int pg0 = blk + po;
-        if (vhi == whi && p == blk + wo)  // uniform: the whole window is one run
-          build_a<false>(hv, ev, pg0, p, vhi, ahi, alo);
-        else
-          build_a<true>(hv, ev, pg0, p, vhi, ahi, alo);
-        if (wgroup == 0)  // uniform
-          mfma6<true>(ahi, alo, brh, brl, bih, bil, acc);
-        else
-          mfma6<false>(ahi, alo, brh, brl, bih, bil, acc);
+        half8 mhi, mlo;
+        build_a<true>(hv, ev, pg0, p, vhi, mhi, mlo);
+        mfma6<false>(mhi, mlo, cb[0], cb[1], cb[2], cb[3], acc);
         p = vhi;
-        const bool run_done = p == run_end;
-        if constexpr (kGroup > 0) {
-          if (++wgroup == kGroup || run_done) {
-            out[0] += acc[0];
-            out[1] += acc[1];
-            wgroup = 0;
-          }
-        } else {
-          wgroup = run_done ? 0 : 1;
-        }
-        if (run_done) {
-          // flush: registers 8g + d (re row) and 8g + 4 + d (im row) hold row
-          // block 2w + g: k = (2w+g) / kMBlocks, m1 = 8 ((2w+g) % kMBlocks) + 4(l/32) + d
-#pragma unroll
-          for (int g = 0; g < 2; ++g) {
-            const int bb = 2 * w + g;
-            const int kk = bb / kMBlocks;
-#pragma unroll
-            for (int d = 0; d < 4; ++d) {
-              const int m = kMT * mt + 32 * (8 * (bb % kMBlocks) + 4 * bq + d) + (l & 31);
-              if (m < Nl) {
-                const floatx16& r0 = kGroup > 0 ? out[0] : acc[0];
-                const floatx16& r1 = kGroup > 0 ? out[1] : acc[1];
-                const float vre = r0[8 * g + d] - r1[8 * g + 4 + d];
-                const float vim = r1[8 * g + d] + r0[8 * g + 4 + d];
-                *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + kk) * Nl + m) * 2) =
-                    make_float2(vre, vim);
-              }
-            }
-          }
-          if (tid == 0 && mt == 0) run_seg[run] = cur;
-          if constexpr (kGroup > 0) out[0] = out[1] = zero;
-          if (p < e) {
-            ++run;
-            const int off = p - blk;  // point p may lie past this block
-            if (off < bn)
-              cur = __builtin_amdgcn_readfirstlane(sSeg[off]);
-            else
-              cur = __builtin_amdgcn_readfirstlane(rec[p].seg);
-            run_end = min(e, seg_start[cur + 1]);
-          }
-        }
+        if (p == run_end) finish_run(p);
+      }
+      if constexpr (kGenNext) gen_slice(j, buf ^ 1);
+      if constexpr (kNext) {
+        load_eh(wo + kWin, ev, hv);
+        load_b(wo + kWin, nb);
+        build_a<false>(hv, ev, 0, 0, 0, na[0], na[1]);
       }
     }
+  };
+  auto first_window_at = [&](int wo, half8 (&bv)[4], half8 (&av)[2]) {
+    float ev[8], hv[8];
+    load_eh(wo, ev, hv);
+    load_b(wo, bv);
+    build_a<false>(hv, ev, 0, 0, 0, av[0], av[1]);
+  };
+  auto first_window = [&](half8 (&bv)[4], half8 (&av)[2]) { first_window_at(0, bv, av); };
+  // a full block followed by another: kNWin windows, fully unrolled (the
+  // operand register sets alternate), generating the next block
+  auto windows_full = [&]() {
+    half8 bx[4], by[4], ax[2], ay[2];
+    first_window(bx, ax);
+#pragma unroll
+    for (int j = 0; j < kNWin; j += 2) {
+      step(std::true_type{}, std::true_type{}, j * kWin, bx, ax, by, ay);
+      if (j + 2 < kNWin)
+        step(std::true_type{}, std::true_type{}, (j + 1) * kWin, by, ay, bx, ax);
+      else
+        step(std::true_type{}, std::false_type{}, (j + 1) * kWin, by, ay, bx, ax);
+    }
+  };
+  // the last block of the chunk (possibly partial): nothing to generate
+  auto windows_last = [&]() {
+    half8 bx[4], ax[2];
+#pragma unroll 1
+    for (int wo = 0; wo < bn; wo += kWin) {
+      first_window_at(wo, bx, ax);
+      step(std::false_type{}, std::false_type{}, wo, bx, ax, bx, ax);
+    }
+  };
+
+  // block 0 is generated up front; then block i+1 is generated while block i
+  // is multiplied out.  A block that is followed by another in this chunk is
+  // full (kBlk points, kNWin windows), so every slice of the next block's
+  // generation gets its window.
+  prefetch(b);
+  gen_setup(0);
+#pragma unroll 1
+  for (int j = 0; j < kNWin; ++j) gen_slice(j, 0);
+  if (b + kBlk < e) prefetch(b + kBlk);
+  for (; blk < e; blk += kBlk, buf ^= 1) {
+    // one barrier per block: block i is complete in `buf`, and every read of
+    // the other buffer (block i-1) is done before block i+1 is written there
     __syncthreads();
+    bn = min(kBlk, e - blk);
+    if (blk + kBlk < e) {  // uniform
+      gen_setup(buf ^ 1);
+      if (blk + 2 * kBlk < e) prefetch(blk + 2 * kBlk);
+      windows_full();
+    } else {
+      windows_last();
+    }
   }
 }
 

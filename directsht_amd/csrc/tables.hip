@@ -16,15 +16,19 @@
 // reference's numpy tables bit for bit.
 //
 // The per-column recurrence is inherently sequential in l; the parallelism is
-// over Nl * Nx columns.  Output goes either to the contraction layout
-// (GroupDesc in common.h; FP64 and/or FP32 copies) or, for DirectSHT.Yv/Yd,
-// to the healpy row layout [Nlm][Nx].
+// over Nl * Nx columns.  Output goes to the contraction layout (GroupDesc in
+// common.h), either Y only (FP64 or FP32: {Y(x_2np), Y(x_2np+1)} per row and
+// node pair; the contraction derives the Y' terms from neighbouring rows) or
+// Y and Y' (FP32, {Y(2np), Y(2np+1), Y'(2np), Y'(2np+1)}: the matrix-core
+// contraction), or, for DirectSHT.Yv/Yd, to the healpy row layout [Nlm][Nx].
 #include "common.h"
 #include "plan.h"
 
 namespace dsht {
 
-template <bool kW64, bool kW32, bool kHP>
+enum TableOut { kOutY64 = 0, kOutY32 = 1, kOutFull32 = 2, kOutHP = 3 };
+
+template <int kOut>
 __global__ __launch_bounds__(256) void k_tables(int Nl, int Nx, int Npair, double dx,
                                                 const int64_t* __restrict__ mp_off,
                                                 double* __restrict__ tab64,
@@ -45,7 +49,7 @@ __global__ __launch_bounds__(256) void k_tables(int Nl, int Nx, int Npair, doubl
     const double norm = sqrt((double)(2 * ell + 1) / 4.0 / M_PI);
     double y = phantom ? 0.0 : P * norm;
     double yd = phantom ? 0.0 : D * norm;
-    if (kHP) {
+    if constexpr (kOut == kOutHP) {
       if (!phantom) {
         int64_t row = hp_index(ell, m, Nl);
         hpYv[row * Nx + n] = y;
@@ -56,14 +60,12 @@ __global__ __launch_bounds__(256) void k_tables(int Nl, int Nx, int Npair, doubl
       const int g = i / kGroupRows, r = i % kGroupRows;
       const int R = rows_mp(Nl, m, P2);
       const int nrows = min(kGroupRows, R - g * kGroupRows);
-      const int64_t idx4 = mp_off[2 * m + P2] + (int64_t)g * kGroupRows * Npair + (int64_t)np * nrows + r;
-      if (kW64) {
-        tab64[idx4 * 4 + odd] = y;
-        tab64[idx4 * 4 + 2 + odd] = yd;
-      }
-      if (kW32) {
-        tab32[idx4 * 4 + odd] = (float)y;
-        tab32[idx4 * 4 + 2 + odd] = (float)yd;
+      const int64_t e = mp_off[2 * m + P2] + (int64_t)g * kGroupRows * Npair + (int64_t)np * nrows + r;
+      if constexpr (kOut == kOutY64) tab64[e * 2 + odd] = y;
+      if constexpr (kOut == kOutY32) tab32[e * 2 + odd] = (float)y;
+      if constexpr (kOut == kOutFull32) {
+        tab32[e * 4 + odd] = (float)y;
+        tab32[e * 4 + 2 + odd] = (float)yd;
       }
     }
   };
@@ -136,21 +138,27 @@ int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipS
   return DSHT_OK;
 }
 
-int launch_tables(const Plan& pl, bool w64, bool w32, double* hpYv, double* hpYd, hipStream_t s) {
+int launch_tables(const Plan& pl, int out, void* dst, double* hpYd, hipStream_t s) {
   dim3 block(256);
   dim3 grid((2 * pl.Npair + 255) / 256, pl.Nl);
-  if (hpYv) {
-    k_tables<false, false, true><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
-                                                         nullptr, nullptr, hpYv, hpYd);
-  } else if (w64 && w32) {
-    k_tables<true, true, false><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
-                                                        pl.tab64, pl.tab32, nullptr, nullptr);
-  } else if (w64) {
-    k_tables<true, false, false><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
-                                                         pl.tab64, nullptr, nullptr, nullptr);
-  } else {
-    k_tables<false, true, false><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off,
-                                                         nullptr, pl.tab32, nullptr, nullptr);
+  double* d64 = static_cast<double*>(dst);
+  float* d32 = static_cast<float*>(dst);
+  switch (out) {
+    case kOutY64:
+      k_tables<kOutY64><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off, d64, nullptr,
+                                               nullptr, nullptr);
+      break;
+    case kOutY32:
+      k_tables<kOutY32><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off, nullptr, d32,
+                                               nullptr, nullptr);
+      break;
+    case kOutFull32:
+      k_tables<kOutFull32><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off, nullptr,
+                                                  d32, nullptr, nullptr);
+      break;
+    default:
+      k_tables<kOutHP><<<grid, block, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.dx, pl.mp_off, nullptr,
+                                              nullptr, d64, hpYd);
   }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;

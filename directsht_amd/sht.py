@@ -189,11 +189,12 @@ class DirectSHT:
         return dict(zip(_lib.STAGES, list(ms))), n.value
 
     def debug_node_coefficients(self):
-        """Node coefficients of the last transform, shape (Nl, 2, ceil(Nx/2), 16)
-        (see dsht_plan_debug_rhs in include/dsht.h); for stage-level tests."""
+        """Node coefficients of the last single-set transform, shape
+        (Nl, 2, ceil(Nx/2), 24): [m][parity][node pair][kind*8 + q*2 + node],
+        kinds c, f, g (see dsht_plan_debug_rhs in include/dsht.h)."""
         npair = (self.Nx + 1) // 2
         dt = np.float32 if self._precision == _lib.DSHT_FP32 else np.float64
-        out = np.empty((self.Nell, 2, npair, 16), dtype=dt)
+        out = np.empty((self.Nell, 2, npair, 24), dtype=dt)
         _lib.check(_lib.load().dsht_plan_debug_rhs(self._plan, out.ctypes.data, out.nbytes),
                    "dsht_plan_debug_rhs")
         return out

@@ -100,15 +100,22 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** clone);
 /* Grow the plan's workspace now to hold a transform of Np points (per set)
  * with up to nsets sets (1 for dsht_transform), so that later transforms
  * within those sizes allocate nothing and never synchronize (what HIP graph
- * capture and latency-bound callers need).  Growth waits for the device. */
+ * capture and latency-bound callers need).  Growth waits for the device.
+ * On an FP32 plan with nsets >= 4 and the matrix-core contraction this also
+ * builds the {Y, Y'} table that contraction reads (otherwise built by the
+ * first such transform). */
 int dsht_plan_reserve(dsht_plan* plan, int64_t Np, int nsets);
 
 /* Bytes of HBM the plan holds: its tables (shared with its clones, see
- * dsht_plan_clone) + its own current workspace. */
+ * dsht_plan_clone; the Y-only table, plus the FP32 {Y, Y'} table once a
+ * batched matrix-core contraction has built it) + its own current
+ * workspace. */
 int dsht_plan_footprint(const dsht_plan* plan, int64_t* bytes);
 
 /* HBM a plan would need for its tables and for a transform of Np points
- * (what a server admits against). */
+ * (what a server admits against).  table_bytes covers the Y-only table the
+ * single-set contraction streams; a batched FP32 transform with the
+ * matrix-core contraction adds a {Y, Y'} table of twice its size. */
 int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table_bytes,
                        int64_t* work_bytes);
 
@@ -177,12 +184,13 @@ int dsht_plan_configure(dsht_plan* plan, int option, int value);
 int dsht_plan_set_timing(dsht_plan* plan, int enable);
 int dsht_plan_get_timing(dsht_plan* plan, double stage_ms[DSHT_NSTAGES], int64_t* ntransforms);
 
-/* Diagnostic: copy the node coefficients of the plan's last transform (the
- * right-hand side of the contraction, after the scatter and fold) to `out`:
- * Nl * 2 * ceil(Nx/2) * 16 values, float for DSHT_FP32 plans, double for
- * DSHT_FP64, layout [m][parity][node pair][branch*8 + {c re, c re', d re,
- * d re', c im, c im', d im, d im'}] (primes: the odd node of the pair).
- * Waits for the device.  Used by the stage-level tests. */
+/* Diagnostic: copy the node coefficients of the plan's last single-set
+ * transform (the right-hand side of the contraction, after the scatter and
+ * fold) to `out`: Nl * 2 * ceil(Nx/2) * 24 values, float for DSHT_FP32 plans,
+ * double for DSHT_FP64, layout [m][parity P][node pair][kind*8 + q*2 + node]
+ * with kind c (the Y coefficients c_P), f (x d_P / (1 - x^2)) and
+ * g (d_{1-P} / (1 - x^2)), d the Y' coefficients, q = equatorial re, im,
+ * polar re, im, node = even, odd node of the pair.  Waits for the device. */
 int dsht_plan_debug_rhs(dsht_plan* plan, void* out, int64_t bytes);
 
 /* Message describing the calling thread's last error ("" if none). */

@@ -50,6 +50,6 @@ def test_estimate_without_gpu():
     lib = _lib.load()
     tb, wb = ct.c_int64(), ct.c_int64()
     assert lib.dsht_plan_estimate(1000, 2048, _lib.DSHT_FP32, 10_000_000, ct.byref(tb), ct.byref(wb)) == 0
-    # FP32 tables: 2 * Nlm * Nx * 4 bytes (+ small metadata)
-    assert abs(tb.value - 2 * 500500 * 2048 * 4) / (2 * 500500 * 2048 * 4) < 0.01
+    # FP32 Y-only table: Nlm * Nx * 4 bytes (+ small metadata)
+    assert abs(tb.value - 500500 * 2048 * 4) / (500500 * 2048 * 4) < 0.01
     assert wb.value > 10_000_000 * 52
