@@ -89,7 +89,7 @@ static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
 
 int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
   const WsSizes z = ws_sizes(Nx, Np, chunk);
-  const int Npair = (Nx + 1) / 2;
+  const int Npair = node_pairs(Nx);
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   const int nseg = num_segments(Nx);
   int64_t b = 0;
@@ -238,7 +238,7 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   DSHT_HIP(hipGetDevice(&pl.device));
   pl.Nl = Nl;
   pl.Nx = Nx;
-  pl.Npair = (Nx + 1) / 2;
+  pl.Npair = node_pairs(Nx);
   pl.nseg = num_segments(Nx);
   pl.Nlm = (int64_t)Nl * (Nl + 1) / 2;
   pl.xmax = xmax;
@@ -673,7 +673,7 @@ int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table
   std::vector<GroupDesc> groups;
   std::vector<int64_t> mp_off;
   int64_t vec4 = 0;
-  const int Npair = (Nx + 1) / 2;
+  const int Npair = node_pairs(Nx);
   build_layout(Nl, Npair, groups, mp_off, vec4);
   groups = xcd_schedule(groups, kXcds);
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;

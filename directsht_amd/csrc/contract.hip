@@ -37,72 +37,74 @@ __device__ inline double4 stream_load(const double4* p) {
   return make_double4(v.x, v.y, v.z, v.w);
 }
 
-__device__ inline float2 stream_load(const float2* p) {
-  typedef float v2 __attribute__((ext_vector_type(2)));
-  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
-  return make_float2(v.x, v.y);
-}
-__device__ inline double2 stream_load(const double2* p) {
-  typedef double v2 __attribute__((ext_vector_type(2)));
-  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
-  return make_double2(v.x, v.y);
+__device__ inline float4 stream_load(const float4* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Y-only tables are read in couples: {Y(x_4c), Y(x_4c+1), Y(x_4c+2),
+// Y(x_4c+3)} of one row = node pairs 2c and 2c+1 (one 16-byte (FP32) /
+// 32-byte (FP64) load per lane, a contiguous 1 KB / 2 KB block per wavefront).
 template <typename T>
-struct Pair;
+struct Couple;
 template <>
-struct Pair<float> {
-  using type = float2;
-  static constexpr int kRing = 16;  // node pairs in flight per lane = pairs per FP32 partial sum
+struct Couple<float> {
+  using type = float4;
+  static constexpr int kRing = 8;  // couples in flight per lane = 16 pairs per FP32 partial sum
 };
 template <>
-struct Pair<double> {
-  using type = double2;
-  static constexpr int kRing = 8;
+struct Couple<double> {
+  using type = double4;
+  static constexpr int kRing = 4;
 };
 
-// Table loads run kRing node pairs ahead of the multiply (a ring of
-// registers), so that each wavefront keeps kRing loads in flight; lanes past
-// the group's rows load row 0 and discard it (no branch around the loads).
-// A scheduling fence after each pair keeps one pair's coefficients (24 scalar
-// registers) live at a time.
+// Table loads run kRing couples ahead of the multiply (a ring of registers),
+// so that each wavefront keeps kRing loads in flight; lanes past the group's
+// rows read row 0 and discard their sums (no branch anywhere).  Scheduling
+// fences keep one node pair's coefficients (24 scalar registers) live at a
+// time, plus, for FP32, the next pair's, loaded ahead.
 //
-// Work item = (row group, node range): the node pairs are split into
+// Work item = (row group, range of couples): the couples are split into
 // gridDim.y ranges, so that the launch has several times more equal-length
 // wavefronts than the GPU holds at once (one wavefront per group would run
 // in ~1.4 rounds, the second mostly idle).  Each item writes its partial
 // sums part[range][h][8] = {S_c - l S_f (4), S_g (4)}; k_add_der adds the
 // ranges in order (deterministic).
 template <typename T>
-__global__ __launch_bounds__(64) void k_contract_y(const typename Pair<T>::type* __restrict__ tab,
+__global__ __launch_bounds__(64) void k_contract_y(const typename Couple<T>::type* __restrict__ tab,
                                                    const T* __restrict__ rhs,
                                                    const GroupDesc* __restrict__ groups, int Npair,
                                                    int Nl, int64_t Nlm, double* __restrict__ part) {
-  using T2 = typename Pair<T>::type;
-  constexpr int kRing = Pair<T>::kRing;
+  using T4 = typename Couple<T>::type;
+  constexpr int kRing = Couple<T>::kRing;
   const GroupDesc gd = groups[blockIdx.x];
   if (gd.nrows == 0) return;  // padding of the XCD schedule
+  const int ncpl = Npair / 2;
   const int nsplit = gridDim.y, split = blockIdx.y;
-  const int lo = (int)((int64_t)Npair * split / nsplit);
-  const int hi = (int)((int64_t)Npair * (split + 1) / nsplit);
+  const int lo = (int)((int64_t)ncpl * split / nsplit);
+  const int hi = (int)((int64_t)ncpl * (split + 1) / nsplit);
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
-  const T2* t = tab + gd.off + (valid ? r : 0);
+  const T4* t = tab + gd.off / 2 + (valid ? r : 0);  // gd.off is even (Npair is)
   // [c | f | g][er, ei, pr, pi][node] of node pair np at q[24 np]
   const T* q = rhs + (int64_t)(gd.m * 2 + gd.P) * Npair * 24;
   const int64_t stride = gd.nrows;
-  T2 ring[kRing];
+  T4 ring[kRing];
 #pragma unroll
   for (int k = 0; k < kRing; ++k) ring[k] = stream_load(t + (int64_t)min(lo + k, hi - 1) * stride);
   double S[3][4] = {};
-  auto pair = [&](T (&f)[3][4], T2 y, int np) {
-    if (!valid) y = T2{0, 0};
+  auto coef = [&](int np, T (&c)[24]) {
     const T* qq = q + (int64_t)np * 24;
+#pragma unroll
+    for (int i = 0; i < 24; ++i) c[i] = qq[i];
+  };
+  auto mul = [&](T (&f)[3][4], T y0, T y1, const T (&c)[24]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int o = 0; o < 4; ++o)
-        f[k][o] = fma(y.x, qq[8 * k + 2 * o], fma(y.y, qq[8 * k + 2 * o + 1], f[k][o]));
+        f[k][o] = fma(y0, c[8 * k + 2 * o], fma(y1, c[8 * k + 2 * o + 1], f[k][o]));
   };
   auto flush = [&](T (&f)[3][4]) {
 #pragma unroll
@@ -110,23 +112,45 @@ __global__ __launch_bounds__(64) void k_contract_y(const typename Pair<T>::type*
 #pragma unroll
       for (int o = 0; o < 4; ++o) S[k][o] += (double)f[k][o];
   };
-  int np0 = lo;
-  for (; np0 + kRing <= hi; np0 += kRing) {  // full rounds of the ring
+  // couple c: pairs 2c (coefficients in ca) and 2c+1 (cb)
+  constexpr bool kPre = sizeof(T) == 4;  // FP32: next pair's coefficients loaded ahead
+  T ca[24], cb[24];
+  if constexpr (kPre) coef(2 * lo, ca);
+  // (lanes past the group's rows compute on row 0 and never store)
+  auto couple = [&](T (&f)[3][4], T4 y, int c) {
+    if constexpr (kPre) {
+      coef(2 * c + 1, cb);
+      __builtin_amdgcn_sched_barrier(0);  // issue the loads before the multiplies
+      mul(f, y.x, y.y, ca);
+      __builtin_amdgcn_sched_barrier(0);
+      coef(min(2 * c + 2, 2 * hi - 1), ca);
+      __builtin_amdgcn_sched_barrier(0);
+      mul(f, y.z, y.w, cb);
+    } else {
+      coef(2 * c, ca);
+      mul(f, y.x, y.y, ca);
+      __builtin_amdgcn_sched_barrier(0);
+      coef(2 * c + 1, cb);
+      mul(f, y.z, y.w, cb);
+    }
+  };
+  int c0 = lo;
+  for (; c0 + kRing <= hi; c0 += kRing) {  // full rounds of the ring
     T f[3][4] = {};
 #pragma unroll
     for (int k = 0; k < kRing; ++k) {
-      // consume the slot, then refill the same registers kRing pairs ahead
-      pair(f, ring[k], np0 + k);
-      ring[k] = stream_load(t + (int64_t)min(np0 + kRing + k, hi - 1) * stride);
+      // consume the slot, then refill the same registers kRing couples ahead
+      couple(f, ring[k], c0 + k);
+      ring[k] = stream_load(t + (int64_t)min(c0 + kRing + k, hi - 1) * stride);
       __builtin_amdgcn_sched_barrier(0);
     }
     flush(f);
   }
-  {  // the last hi - np0 < kRing pairs are already in the ring
+  {  // the last hi - c0 < kRing couples are already in the ring
     T f[3][4] = {};
 #pragma unroll
     for (int k = 0; k < kRing; ++k)
-      if (np0 + k < hi) pair(f, ring[k], np0 + k);
+      if (c0 + k < hi) couple(f, ring[k], c0 + k);
     flush(f);
   }
   if (valid) {
@@ -229,11 +253,11 @@ int launch_contract_set(Plan& pl, int nsets, int set, hipStream_t s) {
   const int nsplit = contract_splits(pl);
   dim3 cgrid(pl.ngroups, nsplit);
   if (pl.precision == DSHT_FP32) {
-    k_contract_y<float><<<cgrid, 64, 0, s>>>(reinterpret_cast<const float2*>(pl.tab32y),
+    k_contract_y<float><<<cgrid, 64, 0, s>>>(reinterpret_cast<const float4*>(pl.tab32y),
                                              static_cast<const float*>(ws.rhs), pl.groups, pl.Npair,
                                              pl.Nl, pl.Nlm, ws.a_der);
   } else {
-    k_contract_y<double><<<cgrid, 64, 0, s>>>(reinterpret_cast<const double2*>(pl.tab64y),
+    k_contract_y<double><<<cgrid, 64, 0, s>>>(reinterpret_cast<const double4*>(pl.tab64y),
                                               static_cast<const double*>(ws.rhs), pl.groups, pl.Npair,
                                               pl.Nl, pl.Nlm, ws.a_der);
   }

@@ -137,8 +137,9 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   if constexpr (kY) {
 #pragma unroll
     for (int odd = 0; odd < 2; ++odd) {
-      const double x = (double)(2 * np + odd) * dx;  // node x_n (d = 0 on padding nodes)
-      const double inv = 1.0 / (1.0 - x * x);
+      const int n = 2 * np + odd;
+      const double x = (double)n * dx;  // node x_n
+      const double inv = n < Nx ? 1.0 / (1.0 - x * x) : 0.0;  // padding nodes: d = 0
 #pragma unroll
       for (int P = 0; P < 2; ++P)
 #pragma unroll

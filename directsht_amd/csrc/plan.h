@@ -9,6 +9,11 @@
 
 namespace dsht {
 
+// Node pairs of the contraction layout: an even number covering Nx nodes
+// (nodes >= Nx are padding with zero table values and coefficients), so that
+// the Y-only table can hold two pairs of a row in one 16-byte vector.
+inline int node_pairs(int Nx) { return 2 * ((Nx + 3) / 4); }
+
 struct Workspace {
   int64_t cap_points = 0;  // capacity of the sorted-point arrays
   int64_t cap_chunks = 0;  // capacity of per-chunk arrays
@@ -146,7 +151,7 @@ int launch_fold(Plan& pl, int nsets, int set, hipStream_t s);
 #define DSHT_CONTRACT_SPLITS 2
 #endif
 constexpr int kContractSplits = DSHT_CONTRACT_SPLITS;
-inline int contract_splits(const Plan& pl) { return pl.Npair < kContractSplits ? 1 : kContractSplits; }
+inline int contract_splits(const Plan& pl) { return pl.Npair / 2 < kContractSplits ? 1 : kContractSplits; }
 // plain contraction of the folded set `set` (of nsets) into a_equ / a_pol [set][Nlm][2]
 int launch_contract_set(Plan& pl, int nsets, int set, hipStream_t s);
 // matrix-core contraction of all nsets slots (FP32 {Y, Y'} table `full32`)

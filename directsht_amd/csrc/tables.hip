@@ -18,7 +18,8 @@
 // The per-column recurrence is inherently sequential in l; the parallelism is
 // over Nl * Nx columns.  Output goes to the contraction layout (GroupDesc in
 // common.h), either Y only (FP64 or FP32: {Y(x_2np), Y(x_2np+1)} per row and
-// node pair; the contraction derives the Y' terms from neighbouring rows) or
+// node pair, the pairs 2c and 2c+1 of a row adjacent; the contraction derives
+// the Y' terms from neighbouring rows) or
 // Y and Y' (FP32, {Y(2np), Y(2np+1), Y'(2np), Y'(2np+1)}: the matrix-core
 // contraction), or, for DirectSHT.Yv/Yd, to the healpy row layout [Nlm][Nx].
 #include "common.h"
@@ -60,9 +61,13 @@ __global__ __launch_bounds__(256) void k_tables(int Nl, int Nx, int Npair, doubl
       const int g = i / kGroupRows, r = i % kGroupRows;
       const int R = rows_mp(Nl, m, P2);
       const int nrows = min(kGroupRows, R - g * kGroupRows);
-      const int64_t e = mp_off[2 * m + P2] + (int64_t)g * kGroupRows * Npair + (int64_t)np * nrows + r;
-      if constexpr (kOut == kOutY64) tab64[e * 2 + odd] = y;
-      if constexpr (kOut == kOutY32) tab32[e * 2 + odd] = (float)y;
+      const int64_t gbase = mp_off[2 * m + P2] + (int64_t)g * kGroupRows * Npair;
+      const int64_t e = gbase + (int64_t)np * nrows + r;
+      // Y-only: the two pairs 2c, 2c+1 of a row side by side,
+      // [c][row][pair parity][node parity] (Npair is even)
+      const int64_t ey = gbase + (int64_t)(np >> 1) * 2 * nrows + 2 * r + (np & 1);
+      if constexpr (kOut == kOutY64) tab64[ey * 2 + odd] = y;
+      if constexpr (kOut == kOutY32) tab32[ey * 2 + odd] = (float)y;
       if constexpr (kOut == kOutFull32) {
         tab32[e * 4 + odd] = (float)y;
         tab32[e * 4 + 2 + odd] = (float)yd;

@@ -190,9 +190,10 @@ class DirectSHT:
 
     def debug_node_coefficients(self):
         """Node coefficients of the last single-set transform, shape
-        (Nl, 2, ceil(Nx/2), 24): [m][parity][node pair][kind*8 + q*2 + node],
+        (Nl, 2, Npair, 24) with Npair = 2 ceil(Nx/4) (nodes past Nx are padding):
+        [m][parity][node pair][kind*8 + q*2 + node],
         kinds c, f, g (see dsht_plan_debug_rhs in include/dsht.h)."""
-        npair = (self.Nx + 1) // 2
+        npair = 2 * ((self.Nx + 3) // 4)
         dt = np.float32 if self._precision == _lib.DSHT_FP32 else np.float64
         out = np.empty((self.Nell, 2, npair, 24), dtype=dt)
         _lib.check(_lib.load().dsht_plan_debug_rhs(self._plan, out.ctypes.data, out.nbytes),
