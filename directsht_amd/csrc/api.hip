@@ -396,7 +396,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     }
     maxNp = std::max(maxNp, n);
   }
-  int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared);
+  int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl));
   if (rc) return rc;
   const bool mfma = use_mfma_contract(pl, a.nsets);
   const float* full32 = nullptr;
@@ -419,9 +419,15 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     const int64_t Np = a.Np[0];
     ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
     if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
+    const bool gathers = scatter_gathers(pl);
     for (int set = 0; set < a.nsets; ++set) {
-      if ((rc = launch_gather_weights(pl, Np, a.wsets + set * a.ldw, s))) return rc;
-      if ((rc = launch_scatter(pl, Np, ws.wsort, s))) return rc;
+      const double* w = a.wsets + set * a.ldw;
+      if (gathers) {
+        if ((rc = launch_scatter_mfma(pl, Np, w, true, s))) return rc;
+      } else {
+        if ((rc = launch_gather_weights(pl, Np, w, s))) return rc;
+        if ((rc = launch_scatter(pl, Np, ws.wsort, s))) return rc;
+      }
       if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
       if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
     }
@@ -642,7 +648,7 @@ int dsht_plan_reserve(dsht_plan* p, int64_t Np, int nsets) {
     return DSHT_EINVAL;
   }
   DeviceGuard g(p->pl.device);
-  int rc = ensure_workspace(p->pl, Np, nsets, nsets > 1);
+  int rc = ensure_workspace(p->pl, Np, nsets, nsets > 1 && !scatter_gathers(p->pl));
   if (rc == DSHT_OK && use_mfma_contract(p->pl, nsets)) {
     const float* full32 = nullptr;
     rc = ensure_full32(p->pl, &full32);  // so that later transforms allocate nothing

@@ -135,7 +135,13 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
 int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s);
 // wsort NULL: the records' own weights
 int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
-int launch_scatter_mfma(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
+// gather: w holds a weight set in the caller's input order (read through the
+// records' idx); otherwise w is wsort as for launch_scatter
+int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s);
+// the plan's scatter gathers weight sets itself (no launch_gather_weights)
+inline bool scatter_gathers(const Plan& pl) {
+  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX;
+}
 constexpr int kMfmaMinSets = 4;  // FP32 plans: matrix-core contraction from this many sets
 constexpr int kMaxSets = 64;     // sets per batched transform
 // the matrix-core contraction handles a transform of nsets sets on this plan

@@ -121,7 +121,10 @@ __device__ inline void pin(cplx& z) {
   pin(z.y);
 }
 
-template <int M1, int KB>
+// kGather: the weights are w[rec.idx] of a weight set in the caller's input
+// order (wsort = w), gathered here; otherwise wsort[sorted position] or, if
+// NULL, 1 (the records' own weights).
+template <int M1, int KB, bool kGather>
 __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __restrict__ rec,
                                                           const int* __restrict__ seg_start,
                                                           int nseg, int chunk, int Nl,
@@ -188,9 +191,14 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
   cplx gb{1.0, 0.0}, ge{1.0, 0.0}, gbs{1.0, 0.0}, ges{1.0, 0.0};  // chain values and steps
   // record of this thread's point in a later block, loaded one block ahead
   double pf_ere = 1.0, pf_eim = 0.0, pf_w = 0.0, pf_h[kHPer];
-  int pf_seg = -1;
+  int pf_seg = -1, pf_idx = 0;
+  bool pf_ok = false;
 #pragma unroll
   for (int j = 0; j < kHPer; ++j) pf_h[j] = 0.0;
+  // h and weight of this thread's point in the block being generated; the
+  // weight (gathered: loaded at gen_setup) is combined into sH at the end of
+  // the current block's windows
+  double hold_h[kHPer], hold_w = 0.0, hold_ok = 0.0;
   auto prefetch = [&](int pblk) {
     const int p = pblk + gp;
     if (p < e) {
@@ -203,18 +211,27 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
 #pragma unroll
       for (int j = 0; j < kHPer; ++j) pf_h[j] = hk[j];
       pf_seg = pr.seg;
-      pf_w = wsort ? wsort[p] : 1.0;
+      if constexpr (kGather) pf_idx = pr.idx;
+      else pf_w = wsort ? wsort[p] : 1.0;
+      pf_ok = true;
     } else {
       pf_ere = 1.0;
       pf_eim = 0.0;
       pf_w = 0.0;
       pf_seg = -1;
+      pf_idx = 0;
+      pf_ok = false;
     }
   };
   // chain start for the prefetched block, h and segments into buffer nbuf
   auto gen_setup = [&](int nbuf) {
 #pragma unroll
-    for (int j = 0; j < kHPer; ++j) sH[nbuf][gg * kHPer + j][gp] = (float)(pf_w * pf_h[j]);
+    for (int j = 0; j < kHPer; ++j) hold_h[j] = pf_h[j];
+    // unconditional load (index 0 for padding points, scaled by hold_ok = 0):
+    // nothing waits for it until write_h
+    if constexpr (kGather) hold_w = wsort[pf_idx];
+    else hold_w = pf_w;
+    hold_ok = pf_ok ? 1.0 : 0.0;
     if (gg == 0) sSeg[nbuf][gp] = pf_seg;
     const cplx e1{pf_ere, pf_eim};
     const cplx e2 = csq(e1), e4 = csq(e2), e32 = cpow2k(e4, 3);
@@ -227,6 +244,11 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
       constexpr int lgT = M1 == 8 ? 3 : M1 == 16 ? 4 : 5;     // kMT = 32 * 2^lgT
       ge = cmul(ge, cpow(cpow2k(e32, lgT), mt));               // * e1^(kMT mt)
     }
+  };
+  auto write_h = [&](int nbuf) {
+#pragma unroll
+    for (int j = 0; j < kHPer; ++j)
+      sH[nbuf][gg * kHPer + j][gp] = (float)(hold_ok * hold_w * hold_h[j]);
   };
   // chain steps of window j of the block being generated into buffer nbuf
   auto gen_slice_b = [&](int j, int nbuf) {
@@ -432,6 +454,7 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
   // generation gets its window.
   prefetch(b);
   gen_setup(0);
+  write_h(0);
 #pragma unroll 1
   for (int j = 0; j < kNWin; ++j) gen_slice(j, 0);
   if (b + kBlk < e) prefetch(b + kBlk);
@@ -444,6 +467,7 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
       gen_setup(buf ^ 1);
       if (blk + 2 * kBlk < e) prefetch(blk + 2 * kBlk);
       windows_full();
+      write_h(buf ^ 1);
     } else {
       windows_last();
     }
@@ -451,22 +475,27 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
 }
 
 template <int M1, int KB>
-static int launch_m1(Plan& pl, int nchunks, const double* wsort, hipStream_t s) {
+static int launch_m1(Plan& pl, int nchunks, const double* w, bool gather, hipStream_t s) {
   dim3 grid(nchunks, (pl.Nl + 32 * M1 - 1) / (32 * M1));
-  k_scatter_mfma<M1, KB><<<grid, 16 * M1, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
-                                              pl.ws.run_base, wsort, (float*)pl.ws.part,
-                                              pl.ws.run_seg);
+  if (gather)
+    k_scatter_mfma<M1, KB, true><<<grid, 16 * M1, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg,
+                                                         pl.chunk, pl.Nl, pl.ws.run_base, w,
+                                                         (float*)pl.ws.part, pl.ws.run_seg);
+  else
+    k_scatter_mfma<M1, KB, false><<<grid, 16 * M1, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg,
+                                                          pl.chunk, pl.Nl, pl.ws.run_base, w,
+                                                          (float*)pl.ws.part, pl.ws.run_seg);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
 
-// m tile of 1024, 512 or 256 orders: the narrowest that covers Nl, or 1024
-int launch_scatter_mfma(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
+// m tile of 256, 512 or 1024 orders: the narrowest that covers Nl, or 1024
+int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
-  if (pl.Nl <= 256) return launch_m1<8, 64>(pl, nchunks, wsort, s);
-  if (pl.Nl <= 512) return launch_m1<16, 64>(pl, nchunks, wsort, s);
-  return launch_m1<32, 128>(pl, nchunks, wsort, s);
+  if (pl.Nl <= 256) return launch_m1<8, 64>(pl, nchunks, w, gather, s);
+  if (pl.Nl <= 512) return launch_m1<16, 64>(pl, nchunks, w, gather, s);
+  return launch_m1<32, 128>(pl, nchunks, w, gather, s);
 }
 
 }  // namespace dsht
