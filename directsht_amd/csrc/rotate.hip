@@ -173,7 +173,9 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
   const int ell = blockIdx.x;
   const int s0 = blockIdx.y * G;
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
-  __shared__ double sv[kRotChunk][G][2];  // a'_{l m'}, m' = hi - row
+  // a'_{l m'}, m' = hi - row, as [row][re|im][set]: a lane reads its
+  // component for two sets per 16-byte LDS read
+  __shared__ __attribute__((aligned(16))) double sv[kRotChunk][2][G];
   __shared__ double sAB[kRotChunk], sBi[kRotChunk];
   __shared__ int smode[G];
   if (threadIdx.x < G) {
@@ -212,8 +214,8 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
           const int64_t idx = hp_index(ell, hi - row, Nl);
           const bool r = smode[g] == 3;
           const double* ap = a_pol + ((int64_t)(s0 + g) * Nlm + idx) * 2;
-          sv[row][g][0] = r ? ap[0] : 0.0;
-          sv[row][g][1] = r ? ap[1] : 0.0;
+          sv[row][0][g] = r ? ap[0] : 0.0;
+          sv[row][1][g] = r ? ap[1] : 0.0;
         }
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
           sAB[e] = rot_AB[tri + hi - e];
@@ -225,12 +227,23 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
             // step s = l - m' (m' = hi - row): d_{m',m} times Re a' if s + m is
             // even and Im a' if odd; even steps accumulate in accE, odd in accO
             const double dv = ldexp(cur, E);
+            typedef double d2 __attribute__((ext_vector_type(2)));
             if (((ell - hi + row) & 1) == 0) {  // uniform branch
+              const d2* v = reinterpret_cast<const d2*>(&sv[row][cm][0]);
 #pragma unroll
-              for (int g = 0; g < G; ++g) accE[g] = fma(dv, sv[row][g][cm], accE[g]);
+              for (int g = 0; g < G; g += 2) {
+                const d2 a = v[g / 2];
+                accE[g] = fma(dv, a.x, accE[g]);
+                accE[g + 1] = fma(dv, a.y, accE[g + 1]);
+              }
             } else {
+              const d2* v = reinterpret_cast<const d2*>(&sv[row][cm ^ 1][0]);
 #pragma unroll
-              for (int g = 0; g < G; ++g) accO[g] = fma(dv, sv[row][g][cm ^ 1], accO[g]);
+              for (int g = 0; g < G; g += 2) {
+                const d2 a = v[g / 2];
+                accO[g] = fma(dv, a.x, accO[g]);
+                accO[g + 1] = fma(dv, a.y, accO[g + 1]);
+              }
             }
             // d_{m'-1} = d_m' * (2m/B) - d_{m'+1} * (A/B); values stay below 2^1023
             // since they are rescaled as soon as they pass 2^400
@@ -297,7 +310,8 @@ int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride
     DSHT_LAUNCH_CHECK();
     return DSHT_OK;
   }
-  if (nsets > 16) return launch_rotate_sets<32>(pl, nsets, flags, flag_stride, alm, status, s);
+  // at most 16 sets per workgroup: 32 would need two waves per SIMD (FP64
+  // accumulators) and runs slower than recomputing the d-columns per 16 sets
   if (nsets > 8) return launch_rotate_sets<16>(pl, nsets, flags, flag_stride, alm, status, s);
   if (nsets > 4) return launch_rotate_sets<8>(pl, nsets, flags, flag_stride, alm, status, s);
   return launch_rotate_sets<4>(pl, nsets, flags, flag_stride, alm, status, s);
