@@ -21,7 +21,7 @@ DSHT_MODE_CALL, DSHT_MODE_BASIC = 0, 1
 DSHT_STATUS_EQU_RANGE, DSHT_STATUS_POLAR_RANGE = 1, 2
 STAGES = ("points", "scatter", "fold", "contract", "rotate")
 DSHT_OPT_SCATTER, DSHT_OPT_CHUNK, DSHT_OPT_CONTRACT = 1, 2, 3
-DSHT_SCATTER_MATRIX, DSHT_SCATTER_VECTOR = 0, 1
+DSHT_SCATTER_MATRIX, DSHT_SCATTER_VECTOR, DSHT_SCATTER_SPREAD = 0, 1, 2
 DSHT_CONTRACT_MATRIX, DSHT_CONTRACT_VECTOR = 0, 1
 MAX_SETS = 64  # sets per batched call (dsht_transform_sets / dsht_transform_many)
 

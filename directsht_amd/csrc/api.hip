@@ -218,6 +218,51 @@ static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int
   }
 }
 
+// Spreading-scatter constants: n = 2^logn >= 4 Nl (at least 64), beta =
+// 2.3 W, the twiddles e^{-2 pi i k / n} and 1/psihat(m) for m < Nl, with
+// psihat(m) = (W/2) int_{-1}^{1} exp(beta (sqrt(1 - z^2) - 1)) cos(pi m W z / n) dz
+// by 128-point Gauss-Legendre quadrature (FP64).
+static void spread_constants(int Nl, int& logn, double& beta, std::vector<float2>& tw,
+                             std::vector<float>& ipsi) {
+  logn = 6;
+  while ((1 << logn) < 4 * Nl) ++logn;
+  const int n = 1 << logn;
+  const double W = (double)kSpreadW;
+  beta = 2.3 * W;
+  tw.resize(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = 2.0 * M_PI * (double)k / (double)n;
+    tw[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  constexpr int kq = 128;
+  double zq[kq], wq[kq];
+  for (int i = 0; i < kq; ++i) {  // Legendre roots by Newton's method
+    double z = std::cos(M_PI * (i + 0.75) / (kq + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= kq; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = kq * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    zq[i] = z;
+    wq[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+  ipsi.resize(Nl);
+  for (int m = 0; m < Nl; ++m) {
+    double sum = 0.0;
+    for (int i = 0; i < kq; ++i)
+      sum += wq[i] * std::exp(beta * (std::sqrt(1.0 - zq[i] * zq[i]) - 1.0)) *
+             std::cos(M_PI * m * W * zq[i] / n);
+    ipsi[m] = (float)(1.0 / (0.5 * W * sum));
+  }
+}
+
 // alloc_tables: the plan's Y-only table (FP64 or FP32 by precision), built
 // if `build`; caller: instead an FP64 {Y, Y'} table that the caller's tables
 // are copied into (csht do_transform).
@@ -273,11 +318,21 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   DSHT_HIP(hipMemcpy(pl.rot_exp, ex.data(), ex.size() * sizeof(int), hipMemcpyHostToDevice));
   DSHT_HIP(hipMemcpy(pl.rot_AB, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
   DSHT_HIP(hipMemcpy(pl.rot_iB, iB.data(), iB.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (precision == DSHT_FP32) {
+    std::vector<float2> tw;
+    std::vector<float> ipsi;
+    spread_constants(Nl, pl.spread_logn, pl.spread_beta, tw, ipsi);
+    DSHT_ALLOC(pl.spread_tw, tw.size() * sizeof(float2));
+    DSHT_ALLOC(pl.spread_ipsi, ipsi.size() * sizeof(float));
+    DSHT_HIP(hipMemcpy(pl.spread_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    DSHT_HIP(hipMemcpy(pl.spread_ipsi, ipsi.data(), ipsi.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
   pl.table_bytes = ws.bytes;
   ws.bytes = 0;
   pl.table_hold = std::make_shared<TableOwner>(
       pl.device, std::vector<void*>{pl.groups, pl.mp_off, pl.tab64y, pl.tab32y, pl.tab64,
-                                    pl.rot_top, pl.rot_exp, pl.rot_AB, pl.rot_iB});
+                                    pl.rot_top, pl.rot_exp, pl.rot_AB, pl.rot_iB, pl.spread_tw,
+                                    pl.spread_ipsi});
   if (alloc_tables && build && !caller) {
     int rc = precision == DSHT_FP64 ? launch_tables(pl, 0, pl.tab64y, nullptr, 0)
                                     : launch_tables(pl, 1, pl.tab32y, nullptr, 0);
@@ -325,7 +380,7 @@ static void plan_free(Plan& pl) {
   } else {                  // plan_init failed before handing them to an owner
     for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64y, (void*)pl.tab32y,
                     (void*)pl.tab64, (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB,
-                    (void*)pl.rot_iB})
+                    (void*)pl.rot_iB, (void*)pl.spread_tw, (void*)pl.spread_ipsi})
       if (p) (void)hipFree(p);
   }
   pl = Plan{};
@@ -423,7 +478,9 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     for (int set = 0; set < a.nsets; ++set) {
       const double* w = a.wsets + set * a.ldw;
       if (gathers) {
-        if ((rc = launch_scatter_mfma(pl, Np, w, true, s))) return rc;
+        rc = uses_spread(pl) ? launch_scatter_spread(pl, Np, w, 2, s)
+                             : launch_scatter_mfma(pl, Np, w, true, s);
+        if (rc) return rc;
       } else {
         if ((rc = launch_gather_weights(pl, Np, w, s))) return rc;
         if ((rc = launch_scatter(pl, Np, ws.wsort, s))) return rc;
@@ -489,7 +546,8 @@ int dsht_plan_configure(dsht_plan* p, int option, int value) {
   }
   Plan& pl = p->pl;
   if (option == DSHT_OPT_SCATTER) {
-    if (value != DSHT_SCATTER_MATRIX && value != DSHT_SCATTER_VECTOR) {
+    if (value != DSHT_SCATTER_MATRIX && value != DSHT_SCATTER_VECTOR &&
+        value != DSHT_SCATTER_SPREAD) {
       set_error("dsht_plan_configure: unknown scatter engine");
       return DSHT_EINVAL;
     }
@@ -637,6 +695,10 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.rot_exp = s.rot_exp;
   d.rot_AB = s.rot_AB;
   d.rot_iB = s.rot_iB;
+  d.spread_logn = s.spread_logn;
+  d.spread_beta = s.spread_beta;
+  d.spread_tw = s.spread_tw;
+  d.spread_ipsi = s.spread_ipsi;
   d.table_hold = s.table_hold;
   *out = p;
   return DSHT_OK;

@@ -92,6 +92,8 @@ struct alignas(64) PointRec {
                            // the basis alone (w = 1) for weight-set transforms
   int seg;                 // segment id
   int idx;                 // the point's position in the caller's input arrays
+  double u;                // phi / 2pi in [0, 1), phi in the point's branch frame
+                           // (written for the spreading scatter only)
 };
 static_assert(sizeof(PointRec) == 64, "PointRec must be one 64-byte line");
 

@@ -95,6 +95,13 @@ struct Plan {
   double* tab64 = nullptr;
   bool caller_tables = false;
   int64_t table_bytes = 0;
+  // spreading scatter (DSHT_SCATTER_SPREAD): grid of n = 2^spread_logn >= 4 Nl
+  // points, kernel parameter beta, tw[k] = e^{-2 pi i k / n} (k < n), and
+  // 1/psihat(m) (m < Nl); built for every FP32 plan
+  int spread_logn = 0;
+  double spread_beta = 0;
+  float2* spread_tw = nullptr;
+  float* spread_ipsi = nullptr;
   // polar rotation constants, per l at offset l(l+1)/2: d^l_{l,m}(pi/2) as
   // mantissa * 2^exp for m = 0..l, A(m')/B(m') and 1/B(m') of the recursion
   double* rot_top = nullptr;
@@ -135,12 +142,20 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
 int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s);
 // wsort NULL: the records' own weights
 int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
+// spreading scatter; wmode 0: the records' own weights, 1: w[sorted
+// position], 2: w[rec.idx] (a weight set in the caller's input order)
+constexpr int kSpreadW = 8;  // kernel taps (beta = 2.3 W)
+int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
+inline bool uses_spread(const Plan& pl) {
+  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_SPREAD;
+}
 // gather: w holds a weight set in the caller's input order (read through the
 // records' idx); otherwise w is wsort as for launch_scatter
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s);
 // the plan's scatter gathers weight sets itself (no launch_gather_weights)
 inline bool scatter_gathers(const Plan& pl) {
-  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX;
+  return pl.precision == DSHT_FP32 &&
+         (pl.scatter == DSHT_SCATTER_MATRIX || pl.scatter == DSHT_SCATTER_SPREAD);
 }
 constexpr int kMfmaMinSets = 4;  // FP32 plans: matrix-core contraction from this many sets
 constexpr int kMaxSets = 64;     // sets per batched transform

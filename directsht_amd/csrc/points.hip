@@ -23,6 +23,7 @@ namespace dsht {
 
 struct PointParams {
   int Nx, nseg, mode;
+  int want_u;  // also record u = phi / 2pi (spreading scatter)
   double xmax, dx, acos_xmax, acos_mxmax;
 };
 
@@ -31,6 +32,7 @@ struct Classified {
   int polar;     // 1 if the point went through the rotated branch
   int flag;      // DSHT_STATUS_* bits raised by this point
   double ere, eim, h0, h1, h2, h3;
+  double u;      // phi / 2pi in [0, 1) (when PointParams.want_u)
 };
 
 __device__ inline Classified classify(double theta, double phi, double w, const PointParams& p) {
@@ -41,6 +43,7 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
   c.flag = 0;
   c.ere = 1.0; c.eim = 0.0;
   c.h0 = c.h1 = c.h2 = c.h3 = 0.0;
+  c.u = 0.0;
   double st, ct;
   double xu, sp, cp;
   int branch;
@@ -82,6 +85,13 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
     return c;  // NaN cos(theta): dropped
   }
   if (xu != xu) return c;  // NaN after rotation: dropped
+  if (p.want_u) {
+    // the azimuth from the branch's (cos phi, sin phi), so that it is exact
+    // whatever the caller's phi range: u = atan2(sin, cos) / 2pi mod 1
+    double u = atan2(-c.eim, c.ere) * (0.5 / M_PI);
+    if (u < 0.0) u += 1.0;
+    c.u = u < 1.0 ? u : 0.0;
+  }
   const int h = xu < 0.0 ? 1 : 0;
   const double ax = fabs(xu);
   // np.digitize(ax, x_grid) - 1 with x_grid[j] = j * dx (exact FP64 products)
@@ -182,6 +192,7 @@ __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restr
     d[1] = make_double2(c.h0, c.h1);
     d[2] = make_double2(c.h2, c.h3);
     reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)i);
+    if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
   }
 }
 
@@ -347,6 +358,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
         d[1] = make_double2(c.h0, c.h1);
         d[2] = make_double2(c.h2, c.h3);
         reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)(i0 + (int64_t)u * kSortThreads));
+        if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
       }
     }
   }
@@ -379,7 +391,8 @@ int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s) 
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, int* flags, hipStream_t s) {
   Workspace& ws = pl.ws;
-  PointParams p{pl.Nx, pl.nseg, mode, pl.xmax, pl.dx, pl.acos_xmax, pl.acos_mxmax};
+  PointParams p{pl.Nx, pl.nseg, mode, uses_spread(pl) ? 1 : 0, pl.xmax, pl.dx, pl.acos_xmax,
+                pl.acos_mxmax};
   // counters are cleared by a kernel, not a memset node, so the transform can
   // be captured in a HIP graph and replayed
   k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, flags, 4);

@@ -276,6 +276,7 @@ int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
   if (nchunks == 0) return DSHT_OK;
   if (pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX)
     return launch_scatter_mfma(pl, Np, wsort, false, s);
+  if (uses_spread(pl)) return launch_scatter_spread(pl, Np, wsort, wsort ? 1 : 0, s);
   if (pl.precision == DSHT_FP32) {
     constexpr int K = 8;
     dim3 grid(nchunks, (pl.Nl + 64 * K - 1) / (64 * K));

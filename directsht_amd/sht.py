@@ -159,17 +159,20 @@ class DirectSHT:
 
     # ---- tuning -----------------------------------------------------------
     def configure(self, scatter=None, chunk=None, contract=None):
-        """Select the scatter engine ('matrix' | 'vector', FP32 plans), the
-        sorted points per scatter work item, and the table contraction of
-        batched FP32 transforms ('matrix': fp16x3 MFMA GEMM over all sets,
-        'vector': FP32 GEMV per set).  Results do not depend on these beyond
-        rounding."""
+        """Select the scatter engine of FP32 plans ('matrix': fp16x3 MFMA
+        direct sums, 'vector': FP32 VALU direct sums, 'spread': per-run
+        non-uniform FFT, ~2e-8 relative), the sorted points per scatter work
+        item, and the table contraction of batched FP32 transforms ('matrix':
+        fp16x3 MFMA GEMM over all sets, 'vector': FP32 GEMV per set).
+        Results do not depend on these beyond rounding (and the spreading
+        kernel's ~2e-8)."""
         lib = _lib.load()
         if contract is not None:
             v = {"matrix": _lib.DSHT_CONTRACT_MATRIX, "vector": _lib.DSHT_CONTRACT_VECTOR}[contract]
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_CONTRACT, v), "configure")
         if scatter is not None:
-            v = {"matrix": _lib.DSHT_SCATTER_MATRIX, "vector": _lib.DSHT_SCATTER_VECTOR}[scatter]
+            v = {"matrix": _lib.DSHT_SCATTER_MATRIX, "vector": _lib.DSHT_SCATTER_VECTOR,
+                 "spread": _lib.DSHT_SCATTER_SPREAD}[scatter]
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_SCATTER, v), "configure")
         if chunk is not None:
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_CHUNK, int(chunk)), "configure")

@@ -158,7 +158,10 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
  * rounding.
  *   DSHT_OPT_SCATTER  scatter engine of DSHT_FP32 plans:
  *                     DSHT_SCATTER_MATRIX (default) fp16x3 split-precision
- *                     MFMA GEMM, or DSHT_SCATTER_VECTOR FP32 VALU recurrence.
+ *                     MFMA GEMM, DSHT_SCATTER_VECTOR FP32 VALU recurrence,
+ *                     or DSHT_SCATTER_SPREAD: each run's sums as a type-1
+ *                     non-uniform FFT (kernel spreading onto a 4x oversampled
+ *                     phi grid, FFT, deconvolution; ~2e-8 relative).
  *                     DSHT_FP64 plans always use the FP64 vector scatter.
  *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536).
  *   DSHT_OPT_CONTRACT table contraction of batched DSHT_FP32 transforms with
@@ -170,6 +173,7 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
 #define DSHT_OPT_CONTRACT 3
 #define DSHT_SCATTER_MATRIX 0
 #define DSHT_SCATTER_VECTOR 1
+#define DSHT_SCATTER_SPREAD 2
 #define DSHT_CONTRACT_MATRIX 0
 #define DSHT_CONTRACT_VECTOR 1
 int dsht_plan_configure(dsht_plan* plan, int option, int value);

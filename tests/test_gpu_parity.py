@@ -240,18 +240,20 @@ def test_scatter_engines_agree(torch_cuda):
 @pytest.mark.parametrize("Nl", [40, 256, 257, 512, 513])
 def test_matrix_scatter_tile_widths(torch_cuda, Nl):
     """The matrix-core scatter's m tile (256, 512 or 1024 orders, chosen by Nl)
-    at and beside each width: FP32 matrix and vector engines against the FP64
-    path."""
+    and the spreading scatter's grid (n = 4 Nl rounded up to a power of two;
+    odd and even log2 n) at and beside each width: FP32 matrix, vector and
+    spread engines against the FP64 path."""
     torch = torch_cuda
     from directsht_amd.sht import DirectSHT
     Nx = 2 * Nl
     rng = np.random.default_rng(Nl)
     t, p, w = (torch.from_numpy(a).cuda() for a in _uniform(rng, 30000))
     ref = DirectSHT(Nl, Nx, dflt_type="float64")(t, p, w).cpu().numpy()
-    for eng in ("matrix", "vector"):
+    for eng in ("matrix", "vector", "spread"):
         s = DirectSHT(Nl, Nx, dflt_type="float32")
         s.configure(scatter=eng)
         a = s(t, p, w).cpu().numpy()
+        print(f"{eng}: max {rel_max(a, ref):.2e} rms {rel_rms(a, ref):.2e}")
         assert rel_max(a, ref) < 1e-6 and rel_rms(a, ref) < 1e-6, eng
 
 
@@ -264,6 +266,10 @@ def test_high_l_second_m_tile(torch_cuda, Nl):
     t, p, w = (torch.from_numpy(a).cuda() for a in _uniform(rng, 20000))
     ref = DirectSHT(Nl, 2 * Nl, dflt_type="float64")(t, p, w).cpu().numpy()
     a = DirectSHT(Nl, 2 * Nl, dflt_type="float32")(t, p, w).cpu().numpy()
+    assert rel_max(a, ref) < 1e-6 and rel_rms(a, ref) < 1e-6
+    s = DirectSHT(Nl, 2 * Nl, dflt_type="float32")
+    s.configure(scatter="spread")  # grid n = 8192 (128 KB of LDS)
+    a = s(t, p, w).cpu().numpy()
     assert rel_max(a, ref) < 1e-6 and rel_rms(a, ref) < 1e-6
 
 
@@ -279,7 +285,7 @@ def test_single_ring_accumulation(torch_cuda, n):
     p = rng.uniform(0, 2 * np.pi, n)
     w = rng.uniform(0.5, 1.5, n)
     ref = DirectSHT(Nl, Nx, dflt_type="float64")(t, p, w)
-    for eng in ("matrix", "vector"):
+    for eng in ("matrix", "vector", "spread"):
         s = DirectSHT(Nl, Nx, dflt_type="float32")
         s.configure(scatter=eng)
         a = s(t, p, w)
