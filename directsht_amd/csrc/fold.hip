@@ -93,26 +93,47 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   for (int br = 0; br < 2 && live; ++br) {
     // c, d per (node of the pair, hemisphere), complex
     double c[2][2][2] = {}, d[2][2][2] = {};
+    // the 8 (node, hemisphere, half) slots of this branch: their run ranges,
+    // then the first run of every slot loaded at once (16 loads in flight),
+    // then the sums in slot order (a segment's further runs, rare, after its
+    // first)
+    int r0[8], r1[8];
 #pragma unroll
-    for (int odd = 0; odd < 2; ++odd) {
-      const int n = 2 * np + odd;
+    for (int q = 0; q < 8; ++q) {
+      const int odd = q >> 2, h = (q >> 1) & 1, half = q & 1;
+      const int j = 2 * np + odd - half;  // bin n (v0, v1) or bin n-1 (v2, v3)
+      r0[q] = r1[q] = 0;
+      if (j >= 0 && j <= Nx - 2) {
+        const int s = (br * 2 + h) * (Nx - 1) + j;
+        r0[q] = seg_lo[s];
+        r1[q] = seg_hi[s];
+      }
+    }
+    V fc[8], fd[8];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+    for (int q = 0; q < 8; ++q) {
+      const int half = q & 1;
+      const bool any = r0[q] < r1[q];
+      const int64_t r = any ? r0[q] : 0;
+      fc[q] = pv[(r * 4 + 2 * half) * Nl + m];
+      fd[q] = pv[(r * 4 + 2 * half + 1) * Nl + m];
+      if (!any) fc[q] = fd[q] = V{0, 0};
+    }
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          const int j = n - half;  // bin n (v0, v1) or bin n-1 (v2, v3)
-          if (j < 0 || j > Nx - 2) continue;
-          const int s = (br * 2 + h) * (Nx - 1) + j;
-          const int r0 = seg_lo[s], r1 = seg_hi[s];
-          for (int r = r0; r < r1; ++r) {
-            const V vc = pv[((int64_t)r * 4 + 2 * half) * Nl + m];
-            const V vd = pv[((int64_t)r * 4 + 2 * half + 1) * Nl + m];
-            c[odd][h][0] += (double)vc.x;
-            c[odd][h][1] += (double)vc.y;
-            d[odd][h][0] += (double)vd.x;
-            d[odd][h][1] += (double)vd.y;
-          }
-        }
+    for (int q = 0; q < 8; ++q) {
+      const int odd = q >> 2, h = (q >> 1) & 1, half = q & 1;
+      c[odd][h][0] += (double)fc[q].x;
+      c[odd][h][1] += (double)fc[q].y;
+      d[odd][h][0] += (double)fd[q].x;
+      d[odd][h][1] += (double)fd[q].y;
+      for (int r = r0[q] + 1; r < r1[q]; ++r) {
+        const V vc = pv[((int64_t)r * 4 + 2 * half) * Nl + m];
+        const V vd = pv[((int64_t)r * 4 + 2 * half + 1) * Nl + m];
+        c[odd][h][0] += (double)vc.x;
+        c[odd][h][1] += (double)vc.y;
+        d[odd][h][0] += (double)vd.x;
+        d[odd][h][1] += (double)vd.y;
+      }
     }
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
