@@ -110,7 +110,8 @@ static void free_workspace(Workspace& ws) {
   for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
                   (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
-                  (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.rmax, (void*)ws.wsort,
+                  (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.rmax, (void*)ws.wmax, (void*)ws.wsort,
+                  (void*)ws.sidx, (void*)ws.wsf,
                   (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.a_der, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
@@ -139,18 +140,20 @@ static int64_t rhs_per_pair(const Plan& pl, int nsets) {
   return use_mfma_contract(pl, nsets) ? std::max<int64_t>(24, 16 * (int64_t)nsets) : 24;
 }
 
-static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false) {
+static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false,
+                            int part_sets = 1) {
   const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
   Workspace& ws = pl.ws;
   const int64_t rhs_need = (int64_t)pl.Nl * 2 * pl.Npair * rhs_per_pair(pl, nsets);
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
       z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort) &&
-      rhs_need <= ws.rhs_elems)
+      rhs_need <= ws.rhs_elems && part_sets <= ws.cap_part_sets)
     return DSHT_OK;
   DSHT_HIP(hipDeviceSynchronize());
   WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
                std::max(z.runs, ws.cap_runs), std::max(z.tiles, ws.cap_tiles)};
   const int sets = std::max(nsets, ws.cap_sets);
+  const int psets = std::max(part_sets, ws.cap_part_sets);
   const bool with_w = wsort || ws.has_wsort;
   const int64_t rhs_elems = std::max(rhs_need, ws.rhs_elems);
   free_workspace(ws);
@@ -159,7 +162,7 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   DSHT_ALLOC(ws.run_count, (want.chunks + 1) * sizeof(int));
   DSHT_ALLOC(ws.run_base, (want.chunks + 1) * sizeof(int));
   DSHT_ALLOC(ws.run_seg, want.runs * sizeof(int));
-  DSHT_ALLOC(ws.part, want.runs * (int64_t)pl.Nl * 8 * fsz);
+  DSHT_ALLOC(ws.part, psets * want.runs * (int64_t)pl.Nl * 8 * fsz);
   DSHT_ALLOC(ws.seg_count, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
   DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
@@ -169,6 +172,12 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   DSHT_ALLOC(ws.thist, want.tiles * (pl.nseg + 1) * sizeof(int));
   DSHT_ALLOC(ws.rhs, rhs_elems * fsz);
   DSHT_ALLOC(ws.rmax, sets * sizeof(unsigned));
+  DSHT_ALLOC(ws.wmax, sets * sizeof(unsigned));
+  if (psets > 1) {
+    ws.ldws = (want.points + 63) / 64 * 64 + 64;
+    DSHT_ALLOC(ws.sidx, want.points * sizeof(int));
+    DSHT_ALLOC(ws.wsf, psets * ws.ldws * sizeof(float));
+  }
   if (with_w) DSHT_ALLOC(ws.wsort, want.points * sizeof(double));
   DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sets * sizeof(double));
   DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sets * sizeof(double));
@@ -179,6 +188,7 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   ws.cap_runs = want.runs;
   ws.cap_tiles = want.tiles;
   ws.cap_sets = sets;
+  ws.cap_part_sets = psets;
   ws.has_wsort = with_w;
   ws.rhs_elems = rhs_elems;
   return DSHT_OK;
@@ -451,7 +461,9 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     }
     maxNp = std::max(maxNp, n);
   }
-  int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl));
+  const bool sets_scatter = a.shared && uses_sets_scatter(pl, a.nsets);
+  int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl),
+                            sets_scatter ? a.nsets : 1);
   if (rc) return rc;
   const bool mfma = use_mfma_contract(pl, a.nsets);
   const float* full32 = nullptr;
@@ -475,8 +487,15 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
     if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
     const bool gathers = scatter_gathers(pl);
+    if (sets_scatter && (rc = launch_scatter_sets(pl, Np, a.wsets, a.ldw, a.nsets, s))) return rc;
     for (int set = 0; set < a.nsets; ++set) {
       const double* w = a.wsets + set * a.ldw;
+      if (sets_scatter) {
+        const float* part = (const float*)ws.part + set * part_set_stride(pl);
+        if ((rc = launch_fold(pl, a.nsets, set, s, part))) return rc;
+        if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
+        continue;
+      }
       if (gathers) {
         rc = uses_spread(pl) ? launch_scatter_spread(pl, Np, w, 2, s)
                              : launch_scatter_mfma(pl, Np, w, true, s);
@@ -710,7 +729,8 @@ int dsht_plan_reserve(dsht_plan* p, int64_t Np, int nsets) {
     return DSHT_EINVAL;
   }
   DeviceGuard g(p->pl.device);
-  int rc = ensure_workspace(p->pl, Np, nsets, nsets > 1 && !scatter_gathers(p->pl));
+  int rc = ensure_workspace(p->pl, Np, nsets, nsets > 1 && !scatter_gathers(p->pl),
+                            uses_sets_scatter(p->pl, nsets) ? nsets : 1);
   if (rc == DSHT_OK && use_mfma_contract(p->pl, nsets)) {
     const float* full32 = nullptr;
     rc = ensure_full32(p->pl, &full32);  // so that later transforms allocate nothing

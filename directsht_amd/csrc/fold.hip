@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   }
 }
 
-int launch_fold(Plan& pl, int nsets, int set, hipStream_t s) {
+int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part) {
   Workspace& ws = pl.ws;
   const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
   const bool scaled = use_mfma_contract(pl, nsets);
@@ -211,7 +211,7 @@ int launch_fold(Plan& pl, int nsets, int set, hipStream_t s) {
   auto go = [&](auto tp, auto yf) {
     using T = decltype(tp);
     k_fold<T, T, decltype(yf)::value><<<grid, kFoldM * kFoldNp, 0, s>>>(
-        (const T*)ws.part, ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
+        (const T*)(part ? part : ws.part), ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
         set, (T*)ws.rhs, rmax);
   };
   if (pl.precision == DSHT_FP32) {

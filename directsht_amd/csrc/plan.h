@@ -26,8 +26,11 @@ struct Workspace {
   int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
   int* run_base = nullptr;    // [cap_chunks + 1]: exclusive scan of run_count
   int* run_seg = nullptr;     // [cap_runs]: segment of each run
-  void* part = nullptr;       // [cap_runs][4][Nl][2]: per-run v_k(m), k = 0..3, complex;
-                              // double (FP64 scatter) or float (FP32 scatter)
+  void* part = nullptr;       // [cap_part_sets][cap_runs][4][Nl][2]: per-run v_k(m),
+                              // k = 0..3, complex; double (FP64 scatter) or float
+                              // (FP32 scatter); one slot per weight set for the
+                              // weight-set scatter, else one
+  int cap_part_sets = 0;      // slots of part
   int* seg_count = nullptr;   // [nseg + 2]
   int* seg_start = nullptr;   // [nseg + 2]
   int* seg_cursor = nullptr;  // [nseg + 2]
@@ -40,6 +43,12 @@ struct Workspace {
                               // contraction and caller tables [Nl][2][Npair][cap_sets][16]
   int64_t rhs_elems = 0;      // capacity of rhs in elements
   unsigned* rmax = nullptr;   // [cap_sets]: max(|c|, |d/dx|) per set (float bits)
+  unsigned* wmax = nullptr;   // [cap_sets]: max |w| per weight set (float bits)
+  int* sidx = nullptr;        // [cap_points]: input index of each sorted point
+                              // (weight-set scatter)
+  float* wsf = nullptr;       // [cap_part_sets][ldws]: weight sets in sorted order,
+                              // FP32, zero past the valid points (weight-set scatter)
+  int64_t ldws = 0;           // row stride of wsf: cap_points rounded up to 64, + 64
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
   double* a_pol = nullptr;    // [cap_sets][Nlm][2]
@@ -164,9 +173,21 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
   return pl.precision == DSHT_FP32 && !pl.caller_tables && nsets >= kMfmaMinSets &&
          pl.contract == DSHT_CONTRACT_MATRIX;
 }
-// node/parity fold of the last scatter: slot `set` of nsets for the
-// matrix-core contraction and caller tables, else the single plain slot
-int launch_fold(Plan& pl, int nsets, int set, hipStream_t s);
+// weight sets on shared points (DSHT_FP32, matrix scatter engine, from
+// kSetsScatterMin sets): one scatter of all sets, set s's run partials at
+// part + s * part_set_stride(pl)
+constexpr int kSetsScatterMin = 2;
+inline bool uses_sets_scatter(const Plan& pl, int nsets) {
+  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX && nsets >= kSetsScatterMin;
+}
+inline int64_t part_set_stride(const Plan& pl) { return pl.ws.cap_runs * (int64_t)pl.Nl * 8; }
+// w: nsets rows of ldw weights in the caller's input order
+int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                        hipStream_t s);
+// node/parity fold of the run partials `part` (NULL: the workspace's first
+// slot): slot `set` of nsets for the matrix-core contraction and caller
+// tables, else the single plain slot
+int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part = nullptr);
 // node ranges of the plain contraction (work items per row group)
 #ifndef DSHT_CONTRACT_SPLITS
 #define DSHT_CONTRACT_SPLITS 2

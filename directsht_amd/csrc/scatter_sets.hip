@@ -1,0 +1,376 @@
+// Stage (b), part 2, for many weight sets on shared points (DSHT_FP32 plans,
+// dsht_transform_sets): the scatter of all sets at once,
+//   v^s_k(m, run) = sum_{p in run} w_s(p) h_k(p) e^{-i m phi_p},
+// the v's of sht/shared_interp_funcs.py:88-105 once per weight set, as one
+// matrix-core GEMM per run with the sets as rows:
+//   C[s][(k, m, re|im)] = sum_p W[s][p] B[p][(k, m, re|im)],
+//   W[s][p] = w_s(p),  B[p][(k, m, c)] = h_k(p) (Re | Im) e^{-i m phi_p}.
+// Each B entry is formed once for all sets (and each weight split once for
+// all orders), so the vector work per matrix-core product is ~4 instructions
+// (against ~10 for the per-set scatter), and the real-times-complex product
+// is 2 flop per (set, point, m, k) (the per-set scatter's real GEMM form of a
+// complex product takes 4).  Both operands are split into fp16 hi + lo and
+// three products hi*hi + hi*lo + lo*hi are accumulated in FP32 by
+// v_mfma_f32_32x32x16_f16, as in the per-set scatter.
+//
+// e^{-i m phi} = E(m1) B(m2), m = 32 m1 + m2, with E and B from FP64 chains
+// (steps e1 = e^{-i phi} and e1^32) rounded to FP32; their FP32 complex
+// product is within ~1e-7 of e^{-i m phi}.
+//
+// Each weight set is scaled by a power of two 2^-e_s that brings max |w_s|
+// into [0.5, 1) (exact; undone at the run's write), so the fp16 split keeps
+// ~2^-22 relative precision whatever the weights' magnitude.
+//
+// The weights are first put in sorted order once (k_sort_weights: FP32 rows
+// wsf[s][pos], one random-gather pass per set, which also finds max |w_s|),
+// so the scatter reads them coalesced.
+//
+// Workgroups: (chunk of sorted points, block of 64 orders), the order blocks
+// of a chunk on one XCD.  One workgroup of 4 wavefronts, two per CU (each
+// wavefront's 128 accumulator registers use half of a SIMD's register file);
+// wavefront w owns 16 orders m = 64 my + 16 w + (0..15), i.e. four 32-column
+// tiles (one per k, columns (m, re|im)), and RT row tiles of 32 sets.  The
+// chunk is staged kBlk points at a time: the sets' weights (scaled, split
+// into fp16 hi/lo), E for the workgroup's two m1, B for all 32 m2, and h_k;
+// one workgroup's staging overlaps the other's products.  A 16-point window
+// (one MFMA K step) is issued once per run it meets (once for almost all
+// windows), with the points outside the run zeroed.  A run's result goes to
+// its own slot part[set][run][k][m][2] (FP32), the layout the fold reads for
+// each set.
+#include "common.h"
+#include "plan.h"
+#include "split.h"
+
+namespace dsht {
+
+namespace {
+
+struct cplx {
+  double x, y;
+};
+__device__ inline cplx cmul(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline cplx csq(cplx a) { return {a.x * a.x - a.y * a.y, 2.0 * a.x * a.y}; }
+
+constexpr int kBlk = 64;         // points per staged block (a chunk is a multiple of it)
+constexpr int kRow = kBlk + 8;   // fp16 rows: 4 (mod 32) dwords
+constexpr int kRowF = kBlk + 4;  // FP32 rows: 4 (mod 32) dwords
+constexpr int kThreads = 256;    // 4 wavefronts: 64 orders
+constexpr int kOrders = 64;      // orders per workgroup
+constexpr int kWin = 16;         // points per MFMA K step
+
+}  // namespace
+
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// sidx[pos] = input index of sorted point pos (valid points only)
+__global__ __launch_bounds__(256) void k_sorted_idx(int64_t Np, const int* __restrict__ nvalid,
+                                                    const PointRec* __restrict__ rec,
+                                                    int* __restrict__ sidx) {
+  const int64_t pos = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (pos < Np && pos < *nvalid) sidx[pos] = rec[pos].idx;
+}
+
+// Weight set s (blockIdx.y) in sorted order, FP32, zero past the valid points
+// up to n: wsf[s][pos] = w_s[sidx[pos]], and smax[s] = max |w_s| as float bits
+// (non-negative floats order like their bit patterns; smax cleared first).
+// The sets are the slow grid index, so the random gathers of one set's row
+// run together and hit the caches.  One block covers kSortPer * 256
+// positions and issues one atomic (same-address atomics serialize).
+constexpr int kSortPer = 8;
+__global__ __launch_bounds__(256) void k_sort_weights(const double* __restrict__ w, int64_t ldw,
+                                                      const int* __restrict__ nvalid,
+                                                      const int* __restrict__ sidx, int64_t n,
+                                                      float* __restrict__ wsf, int64_t ldws,
+                                                      unsigned* __restrict__ smax) {
+  __shared__ float red[4];
+  const int s = blockIdx.y;
+  const int nv = *nvalid;
+  const double* ws = w + (int64_t)s * ldw;
+  float* out = wsf + (int64_t)s * ldws;
+  float mx = 0.f;
+#pragma unroll
+  for (int i = 0; i < kSortPer; ++i) {
+    const int64_t pos = ((int64_t)blockIdx.x * kSortPer + i) * 256 + threadIdx.x;
+    float v = 0.f;
+    if (pos < nv) v = (float)ws[sidx[pos]];
+    if (pos < n) out[pos] = v;
+    mx = fmaxf(mx, fabsf(v));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(smax + s, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+}
+
+// Weight-set scatter, RT row tiles of 32 sets (RT = 2: up to 64 sets).
+template <int RT>
+__global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
+    const PointRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg, int chunk, int Nl,
+    const int* __restrict__ run_base, const float* __restrict__ wsf, int64_t ldws, int nsets,
+    const unsigned* __restrict__ smax, int nchunks, int nmy, float* __restrict__ part,
+    int64_t set_stride, int* __restrict__ run_seg) {
+  // workgroup b runs on XCD b % 8; the nmy order blocks of one chunk go to
+  // one XCD, one after another, so they share its L2 for the chunk's records
+  // and weights
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int c = 8 * (slot / nmy) + xcd, my = slot % nmy;
+  if (c >= nchunks) return;
+  const int tid = threadIdx.x, l = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nvalid = seg_start[nseg];
+  const int b = c * chunk, e = min(nvalid, b + chunk);
+  if (b >= e) return;
+
+  // double-buffered copies of a block's weights (FP32, unscaled) and records,
+  // filled by LDS DMA (global_load_lds) while the previous block is
+  // multiplied out; z, h and segments of the current block
+  __shared__ __attribute__((aligned(16))) float sWr[2][32 * RT][kRowF];  // [buf][set][point]
+  __shared__ PointRec sRec[2][kBlk];
+  __shared__ __attribute__((aligned(16))) float sZ[2 * kOrders][kRowF];  // [(m, re|im)][point]
+  __shared__ __attribute__((aligned(16))) float sH[4][kBlk];
+  __shared__ int sSeg[kBlk];
+  __shared__ float sInv[32 * RT];  // 2^e_s
+
+  // weight set s is scaled by 2^-e_s, max |w_s| 2^-e_s in [0.5, 1); this
+  // lane's A rows are sets 32 rt + l % 32
+  float asc[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    const int s = 32 * rt + (l & 31);
+    int ex = 0;
+    if (s < nsets) {
+      const float mx = __uint_as_float(smax[s]);
+      if (mx > 0.f && mx < 3.0e38f) (void)frexpf(mx, &ex);
+    }
+    asc[rt] = ldexpf(1.0f, -ex);
+    if (wv == 0 && l < 32) sInv[s] = ldexpf(1.0f, ex);
+  }
+
+  int blk = b, bn = 0;  // current block and its number of points
+
+  // LDS DMA of the block at pblk into buffer nb.  Records: wavefront wv
+  // copies records 16 wv .. 16 wv + 15 (lane l: 16 bytes, quarter l % 4 of
+  // record l / 4; padding records past e repeat record pblk).  Weights: one
+  // 256-byte row per instruction, sets wv + 4 t below nsets (rows of other
+  // sets are never used: their results are not written).  wsf rows are long
+  // enough for a whole block.
+  auto dma = [&](int pblk, int nb) {
+    const int ri = pblk + 16 * wv + (l >> 2);
+    const char* src = reinterpret_cast<const char*>(rec + (ri < e ? ri : pblk)) + 16 * (l & 3);
+    __builtin_amdgcn_global_load_lds(src, &sRec[nb][16 * wv], 16, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 8 * RT; ++t) {
+      const int s = wv + 4 * t;
+      if (s < nsets) {
+        const float* row = wsf + s * ldws + pblk;  // uniform
+        __builtin_amdgcn_global_load_lds(row + l, &sWr[nb][s][0], 4, 0, 0);
+      }
+    }
+  };
+  // z(m, p) = e^{-i m phi_p} for the workgroup's orders by an FP64 chain from
+  // e1^(64 my + 16 g) (thread: point gp, orders 16 g .. 16 g + 15), h_g, seg
+  const int gp = tid & (kBlk - 1);
+  const int gg = __builtin_amdgcn_readfirstlane(tid / kBlk);
+  auto stage = [&](int nb) {
+    const bool ok = gp < bn;
+    const PointRec& r = sRec[nb][gp];
+    const cplx e1 = ok ? cplx{r.ere, r.eim} : cplx{1.0, 0.0};
+    cplx z{1.0, 0.0}, pw = e1;
+    int nn = kOrders * my + 16 * gg;
+    while (nn) {
+      if (nn & 1) z = cmul(z, pw);
+      pw = csq(pw);
+      nn >>= 1;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      sZ[2 * (16 * gg + q)][gp] = (float)z.x;
+      sZ[2 * (16 * gg + q) + 1][gp] = (float)z.y;
+      if (q < 15) z = cmul(z, e1);
+    }
+    const double hk = gg == 0 ? r.h0 : gg == 1 ? r.h1 : gg == 2 ? r.h2 : r.h3;
+    sH[gg][gp] = ok ? (float)hk : 0.f;
+    if (gg == 0) sSeg[gp] = ok ? r.seg : -1;
+  };
+
+  // this lane's column: m = 64 my + 16 wv + (col >> 1), component col & 1
+  const int col = l & 31, hq = l >> 5;
+  const int m = kOrders * my + 16 * wv + (col >> 1);
+  const int cc = col & 1;
+  const int zrow = 32 * wv + col;
+
+  floatx16 acc[RT][4];
+  const floatx16 zero = {};
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[rt][k] = zero;
+
+  int run = __builtin_amdgcn_readfirstlane(run_base[c]);
+  int cur = __builtin_amdgcn_readfirstlane(rec[b].seg);
+  int run_end = min(e, seg_start[cur + 1]);
+
+  // the current run ends at point p: write its slot (unscaled), clear the
+  // accumulators, move to the next run.  Lane l, register r of tile rt holds
+  // C[set 32 rt + 8 (r/4) + 4 (l/32) + r%4][column l%32].
+  auto finish_run = [&](int p) {
+    if (m < Nl) {
+      // the accumulators are read with volatile inline asm, which stays in
+      // this branch (the compiler would otherwise hoist the AGPR reads into
+      // every pass).  The compiler does not see these reads as dependent on
+      // the MFMAs that wrote the registers, so first wait out the longest
+      // MFMA result latency (3 x 8 wait states).
+      asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+      float* q = part + ((int64_t)run * 4 * Nl + m) * 2 + cc;
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int s = 32 * rt + 8 * (r >> 2) + 4 * hq + (r & 3);
+          if (s < nsets) {
+            const float inv = sInv[s];
+            float* qs = q + s * set_stride;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              float v;
+              asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(v) : "a"(acc[rt][k][r]));
+              qs[(int64_t)k * Nl * 2] = v * inv;
+            }
+          }
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[rt][k] = zero;
+    if (tid == 0 && my == 0) run_seg[run] = cur;
+    if (p < e) {
+      ++run;
+      const int off = p - blk;  // point p may lie past this block
+      if (off < bn)
+        cur = __builtin_amdgcn_readfirstlane(sSeg[off]);
+      else
+        cur = __builtin_amdgcn_readfirstlane(rec[p].seg);
+      run_end = min(e, seg_start[cur + 1]);
+    }
+  };
+
+  dma(b, 0);
+  for (int buf = 0; blk < e; blk += kBlk, buf ^= 1) {
+    bn = min(kBlk, e - blk);
+    // this wavefront's copies of block blk have landed; after the barrier,
+    // every wavefront's have, and the previous block's products (readers of
+    // sZ, sH, sSeg and buffer buf ^ 1) are done
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    stage(buf);
+    if (blk + kBlk < e) dma(blk + kBlk, buf ^ 1);  // lands during this block's products
+    __syncthreads();
+    for (int wo = 0; wo < bn; wo += kWin) {
+      const int wlo = blk + wo, whi = blk + min(wo + kWin, bn);
+      const int po = wo + 8 * hq;  // this lane's K slice: points po .. po+7
+      float zv[8];
+      *reinterpret_cast<float4*>(&zv[0]) = *reinterpret_cast<const float4*>(&sZ[zrow][po]);
+      *reinterpret_cast<float4*>(&zv[4]) = *reinterpret_cast<const float4*>(&sZ[zrow][po + 4]);
+      half8 whi_[RT], wlo_[RT];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        float wa[8];
+        const float* wr = &sWr[buf][32 * rt + (l & 31)][po];
+        *reinterpret_cast<float4*>(&wa[0]) = *reinterpret_cast<const float4*>(wr);
+        *reinterpret_cast<float4*>(&wa[4]) = *reinterpret_cast<const float4*>(wr + 4);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) wa[j] *= asc[rt];
+        split8_f16(wa, whi_[rt], wlo_[rt]);
+      }
+      // one pass per run the window meets (one pass for almost all windows):
+      // this window's 4 * 3 * RT products over the points [p, vhi) of run
+      // `cur`, the other points' z zeroed.  A single code path for all
+      // windows keeps each accumulator in one register set.
+      int p = wlo;
+      while (p < whi) {
+        const int vhi = min(whi, run_end);
+        float zm[8];
+        const int rel0 = blk + po - p;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) zm[j] = (unsigned)(rel0 + j) < (unsigned)(vhi - p) ? zv[j] : 0.f;
+        half8 bh[4], bl[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          float x[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) x[j] = sH[k][po + j] * zm[j];
+          split8_f16(x, bh[k], bl[k]);
+        }
+        // 3 rounds over the 4 RT independent accumulators, so that an MFMA
+        // never waits for the one just issued
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+            acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(whi_[rt], bh[k], acc[rt][k], 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+            acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(whi_[rt], bl[k], acc[rt][k], 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+            acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wlo_[rt], bh[k], acc[rt][k], 0, 0, 0);
+        p = vhi;
+        if (p == run_end) finish_run(p);
+        // keep the accumulators in AGPRs across the loop (otherwise the
+        // compiler carries them in VGPRs and copies all of them every pass)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) asm volatile("" : "+a"(acc[rt][k]));
+      }
+    }
+  }
+}
+
+int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                        hipStream_t s) {
+  const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+  if (nchunks == 0) return DSHT_OK;
+  Workspace& ws = pl.ws;
+  if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_part_sets || !ws.wsf ||
+      Np > ws.cap_points) {
+    set_error("dsht: the weight-set scatter takes 1..64 sets within the workspace");
+    return DSHT_EINVAL;
+  }
+  const int* nvalid = ws.seg_start + pl.nseg;
+  k_fill_u32<<<1, 64, 0, s>>>(ws.wmax, nsets, 0u);
+  DSHT_LAUNCH_CHECK();
+  k_sorted_idx<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, nvalid, ws.rec, ws.sidx);
+  DSHT_LAUNCH_CHECK();
+  // rows cover every position a staged block reads: (Np rounded up to 64) + 64 <= ldws
+  const int64_t n = (Np + 63) / 64 * 64 + 64;
+  k_sort_weights<<<dim3((unsigned)((n + 256 * kSortPer - 1) / (256 * kSortPer)), nsets), 256, 0, s>>>(
+      w, ldw, nvalid, ws.sidx, n, ws.wsf, ws.ldws, ws.wmax);
+  DSHT_LAUNCH_CHECK();
+  const int nmy = (pl.Nl + kOrders - 1) / kOrders;
+  const int64_t nwg = (int64_t)8 * ((nchunks + 7) / 8) * nmy;
+  const int64_t stride = part_set_stride(pl);
+  auto go = [&](auto kern) {
+    kern<<<(unsigned)nwg, kThreads, 0, s>>>(ws.rec, ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
+                                            ws.run_base, ws.wsf, ws.ldws, nsets, ws.wmax, nchunks,
+                                            nmy, (float*)ws.part, stride, ws.run_seg);
+  };
+  if (nsets > 32)
+    go(k_scatter_sets<2>);
+  else
+    go(k_scatter_sets<1>);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+}  // namespace dsht
