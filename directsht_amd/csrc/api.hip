@@ -176,7 +176,8 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   if (psets > 1) {
     ws.ldws = (want.points + 63) / 64 * 64 + 64;
     DSHT_ALLOC(ws.sidx, want.points * sizeof(int));
-    DSHT_ALLOC(ws.wsf, psets * ws.ldws * sizeof(float));
+    // sorted rows [psets][ldws], then the point-major copy [cap_points][psets]
+    DSHT_ALLOC(ws.wsf, 2 * psets * ws.ldws * sizeof(float));
   }
   if (with_w) DSHT_ALLOC(ws.wsort, want.points * sizeof(double));
   DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sets * sizeof(double));
@@ -488,7 +489,10 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
     const bool gathers = scatter_gathers(pl);
     if (sets_scatter && (rc = launch_scatter_sets(pl, Np, a.wsets, a.ldw, a.nsets, s))) return rc;
-    for (int set = 0; set < a.nsets; ++set) {
+    // one slot per set: fold every set in one launch
+    const bool fold_all = sets_scatter && mfma;
+    if (fold_all && (rc = launch_fold_sets(pl, a.nsets, s, ws.part, part_set_stride(pl)))) return rc;
+    for (int set = 0; set < a.nsets && !fold_all; ++set) {
       const double* w = a.wsets + set * a.ldw;
       if (sets_scatter) {
         const float* part = (const float*)ws.part + set * part_set_stride(pl);

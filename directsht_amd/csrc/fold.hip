@@ -40,12 +40,13 @@
 namespace dsht {
 
 // Every segment's [first run, last run + 1); segments without runs keep (0, 0).
+// Also clears nrmax per-set magnitude bounds.
 __global__ __launch_bounds__(256) void k_seg_clear(int nseg, int* __restrict__ lo,
                                                    int* __restrict__ hi,
-                                                   unsigned* __restrict__ rmax) {
+                                                   unsigned* __restrict__ rmax, int nrmax) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nseg) lo[s] = hi[s] = 0;
-  if (s == 0 && rmax) *rmax = 0u;
+  if (s < nrmax) rmax[s] = 0u;
 }
 
 __global__ __launch_bounds__(256) void k_seg_runs(const int* __restrict__ run_seg,
@@ -78,9 +79,14 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
                                                            const int* __restrict__ seg_lo,
                                                            const int* __restrict__ seg_hi,
                                                            int Nl, int Nx, int Npair, double dx,
-                                                           int nsets, int set, T* __restrict__ rhs,
+                                                           int nsets, int set0,
+                                                           int64_t part_stride, T* __restrict__ rhs,
                                                            unsigned* __restrict__ rmax) {
   using V = typename Vec2<TP>::type;
+  // slot set0 + z of the batch, its partials part_stride elements further on
+  const int set = set0 + (int)blockIdx.z;
+  part += blockIdx.z * part_stride;
+  if (rmax) rmax += blockIdx.z;
   const int m = blockIdx.y * kFoldM + (threadIdx.x & (kFoldM - 1));
   const int np = blockIdx.x * kFoldNp + threadIdx.x / kFoldM;
   const bool live = m < Nl && np < Npair;
@@ -195,24 +201,143 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   }
 }
 
-int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part) {
+// All nsets slots of a weight-set batch (FP32, the 16-value layout).  A block
+// takes 64 orders (lanes: every partial it reads is one 512-byte stretch) of
+// one node pair, and the sets in chunks of kFoldSetChunk: each wavefront folds
+// two sets of the chunk into an LDS tile, and the block then writes the
+// chunk's rows rhs[m][P][np][chunk sets][16] as 512-byte stretches (rows of
+// different sets are adjacent, and written all at once rather than a 64-byte
+// piece at a time).  Per-set max(|c|, |d/dx|): wavefront, then one atomic per
+// (block, set).
+constexpr int kFoldSetChunk = 8;
+constexpr int kFoldRow = kFoldSetChunk * 16 + 4;  // LDS row: 4 (mod 32) dwords of padding
+__global__ __launch_bounds__(256) void k_fold_sets(
+    const float* __restrict__ part, int64_t part_stride, const int* __restrict__ seg_lo,
+    const int* __restrict__ seg_hi, int Nl, int Nx, int Npair, double dx, int nsets,
+    float* __restrict__ rhs, unsigned* __restrict__ rmax) {
+  __shared__ __attribute__((aligned(16))) float tile[2][kFoldM][kFoldRow];  // [P][m][set, 16]
+  __shared__ float smx[kMaxSets];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int m0 = blockIdx.y * kFoldM, m = m0 + lane;
+  const int np = blockIdx.x;
+  const bool live = m < Nl;
+  // run ranges of the 8 (node, hemisphere, half) slots of each branch
+  // (uniform: they depend on the node pair only)
+  int r0[2][8], r1[2][8];
+#pragma unroll
+  for (int br = 0; br < 2; ++br)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int odd = q >> 2, h = (q >> 1) & 1, half = q & 1;
+      const int j = 2 * np + odd - half;  // bin n (v0, v1) or bin n-1 (v2, v3)
+      r0[br][q] = r1[br][q] = 0;
+      if (j >= 0 && j <= Nx - 2) {
+        const int sg = (br * 2 + h) * (Nx - 1) + j;
+        r0[br][q] = seg_lo[sg];
+        r1[br][q] = seg_hi[sg];
+      }
+    }
+  for (int c0 = 0; c0 < nsets; c0 += kFoldSetChunk) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int sl = 2 * wv + u, set = c0 + sl;
+      if (set >= nsets) break;  // uniform
+      const float2* pv = reinterpret_cast<const float2*>(part + set * part_stride);
+      float vmax = 0.f;
+      float out[2][16];
+#pragma unroll
+      for (int br = 0; br < 2; ++br) {
+        double c[2][2][2] = {}, d[2][2][2] = {};
+        float2 fc[8], fd[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int half = q & 1;
+          const bool any = live && r0[br][q] < r1[br][q];
+          const int64_t r = any ? r0[br][q] : 0;
+          fc[q] = pv[(r * 4 + 2 * half) * Nl + (live ? m : 0)];
+          fd[q] = pv[(r * 4 + 2 * half + 1) * Nl + (live ? m : 0)];
+          if (!any) fc[q] = fd[q] = make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int odd = q >> 2, h = (q >> 1) & 1, half = q & 1;
+          c[odd][h][0] += (double)fc[q].x;
+          c[odd][h][1] += (double)fc[q].y;
+          d[odd][h][0] += (double)fd[q].x;
+          d[odd][h][1] += (double)fd[q].y;
+          for (int r = r0[br][q] + 1; live && r < r1[br][q]; ++r) {
+            const float2 vc = pv[((int64_t)r * 4 + 2 * half) * Nl + m];
+            const float2 vd = pv[((int64_t)r * 4 + 2 * half + 1) * Nl + m];
+            c[odd][h][0] += (double)vc.x;
+            c[odd][h][1] += (double)vc.y;
+            d[odd][h][0] += (double)vd.x;
+            d[odd][h][1] += (double)vd.y;
+          }
+        }
+#pragma unroll
+        for (int P = 0; P < 2; ++P) {
+          const double sg = P ? -1.0 : 1.0;
+#pragma unroll
+          for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+            for (int odd = 0; odd < 2; ++odd) {
+              const double cv = c[odd][0][ri] + sg * c[odd][1][ri];
+              const double dv = d[odd][0][ri] + sg * d[odd][1][ri];
+              out[P][8 * br + 4 * ri + odd] = (float)cv;
+              out[P][8 * br + 4 * ri + 2 + odd] = (float)(dx * dv);
+              vmax = fmaxf(vmax, (float)fmax(fabs(cv), fabs(dv)));
+            }
+        }
+      }
+#pragma unroll
+      for (int P = 0; P < 2; ++P)
+#pragma unroll
+        for (int q = 0; q < 16; q += 4)
+          *reinterpret_cast<float4*>(&tile[P][lane][16 * sl + q]) =
+              make_float4(out[P][q], out[P][q + 1], out[P][q + 2], out[P][q + 3]);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+      if (lane == 0) smx[set] = vmax;
+    }
+    __syncthreads();
+    // the chunk's rows: (m, P) -> nc sets x 16 floats, contiguous in rhs
+    const int nc = min(kFoldSetChunk, nsets - c0);
+    const int per_row = nc * 4;  // float4s per row
+    for (int f = tid; f < 2 * kFoldM * per_row; f += 256) {
+      const int row = f / per_row, within = f - row * per_row;
+      const int ml = row >> 1, P = row & 1;
+      if (m0 + ml < Nl)
+        *reinterpret_cast<float4*>(rhs + ((((int64_t)(m0 + ml) * 2 + P) * Npair + np) * nsets + c0) *
+                                             16 + 4 * within) =
+            *reinterpret_cast<const float4*>(&tile[P][ml][4 * within]);
+    }
+    __syncthreads();
+  }
+  if (tid < nsets && smx[tid] > 0.f) atomicMax(rmax + tid, __float_as_uint(smx[tid]));
+}
+
+static int fold_impl(Plan& pl, int nsets, int set0, int count, hipStream_t s, const void* part,
+                     int64_t part_stride) {
   Workspace& ws = pl.ws;
   const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
   const bool scaled = use_mfma_contract(pl, nsets);
   const bool yform = !scaled && !pl.caller_tables;  // plain contraction of Y-only tables
-  unsigned* rmax = scaled ? ws.rmax + set : nullptr;
-  k_seg_clear<<<(pl.nseg + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi, rmax);
+  unsigned* rmax = scaled ? ws.rmax + set0 : nullptr;
+  const int nclear = std::max(pl.nseg, scaled ? count : 0);
+  k_seg_clear<<<(nclear + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo, ws.seg_run_hi, rmax,
+                                                   scaled ? count : 0);
   DSHT_LAUNCH_CHECK();
   const int64_t max_runs = (int64_t)ws.cur_chunks + pl.nseg + 1;
   k_seg_runs<<<(int)((max_runs + 255) / 256), 256, 0, s>>>(ws.run_seg, run_total, ws.seg_run_lo,
                                                           ws.seg_run_hi);
   DSHT_LAUNCH_CHECK();
-  dim3 grid((pl.Npair + kFoldNp - 1) / kFoldNp, (pl.Nl + kFoldM - 1) / kFoldM);
+  dim3 grid((pl.Npair + kFoldNp - 1) / kFoldNp, (pl.Nl + kFoldM - 1) / kFoldM, count);
   auto go = [&](auto tp, auto yf) {
     using T = decltype(tp);
     k_fold<T, T, decltype(yf)::value><<<grid, kFoldM * kFoldNp, 0, s>>>(
-        (const T*)(part ? part : ws.part), ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets,
-        set, (T*)ws.rhs, rmax);
+        (const T*)(part ? part : ws.part), ws.seg_run_lo, ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair,
+        pl.dx, nsets, set0, part_stride, (T*)ws.rhs, rmax);
   };
   if (pl.precision == DSHT_FP32) {
     if (yform) go(0.f, std::true_type{});
@@ -221,6 +346,32 @@ int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part) {
     if (yform) go(0.0, std::true_type{});
     else go(0.0, std::false_type{});
   }
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part) {
+  return fold_impl(pl, nsets, set, 1, s, part, 0);
+}
+
+int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride) {
+  Workspace& ws = pl.ws;
+  if (!use_mfma_contract(pl, nsets) || nsets > kMaxSets) {
+    set_error("dsht: the batched fold serves the matrix-core contraction (4..64 sets)");
+    return DSHT_EINVAL;
+  }
+  const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
+  k_seg_clear<<<(std::max(pl.nseg, nsets) + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo,
+                                                                      ws.seg_run_hi, ws.rmax, nsets);
+  DSHT_LAUNCH_CHECK();
+  const int64_t max_runs = (int64_t)ws.cur_chunks + pl.nseg + 1;
+  k_seg_runs<<<(int)((max_runs + 255) / 256), 256, 0, s>>>(ws.run_seg, run_total, ws.seg_run_lo,
+                                                          ws.seg_run_hi);
+  DSHT_LAUNCH_CHECK();
+  dim3 grid(pl.Npair, (pl.Nl + kFoldM - 1) / kFoldM);
+  k_fold_sets<<<grid, 256, 0, s>>>((const float*)part, part_stride, ws.seg_run_lo,
+                                                ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx,
+                                                nsets, (float*)ws.rhs, ws.rmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

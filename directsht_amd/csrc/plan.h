@@ -47,7 +47,8 @@ struct Workspace {
   int* sidx = nullptr;        // [cap_points]: input index of each sorted point
                               // (weight-set scatter)
   float* wsf = nullptr;       // [cap_part_sets][ldws]: weight sets in sorted order,
-                              // FP32, zero past the valid points (weight-set scatter)
+                              // FP32, zero past the valid points (weight-set scatter);
+                              // followed by the same size of scratch for the sort
   int64_t ldws = 0;           // row stride of wsf: cap_points rounded up to 64, + 64
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
@@ -188,6 +189,9 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
 // slot): slot `set` of nsets for the matrix-core contraction and caller
 // tables, else the single plain slot
 int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part = nullptr);
+// the fold of all nsets slots at once for the matrix-core contraction (FP32),
+// slot s's partials at part + s * part_stride
+int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride);
 // node ranges of the plain contraction (work items per row group)
 #ifndef DSHT_CONTRACT_SPLITS
 #define DSHT_CONTRACT_SPLITS 2

@@ -21,9 +21,9 @@
 // into [0.5, 1) (exact; undone at the run's write), so the fp16 split keeps
 // ~2^-22 relative precision whatever the weights' magnitude.
 //
-// The weights are first put in sorted order once (k_sort_weights: FP32 rows
-// wsf[s][pos], one random-gather pass per set, which also finds max |w_s|),
-// so the scatter reads them coalesced.
+// The weights are first put in sorted order once (FP32 rows wsf[s][pos], by a
+// transpose to point-major rows and a gather of whole rows, which also finds
+// max |w_s|), so the scatter reads them coalesced.
 //
 // Workgroups: (chunk of sorted points, block of 64 orders), the order blocks
 // of a chunk on one XCD.  One workgroup of 4 wavefronts, two per CU (each
@@ -73,38 +73,81 @@ __global__ __launch_bounds__(256) void k_sorted_idx(int64_t Np, const int* __res
   if (pos < Np && pos < *nvalid) sidx[pos] = rec[pos].idx;
 }
 
-// Weight set s (blockIdx.y) in sorted order, FP32, zero past the valid points
-// up to n: wsf[s][pos] = w_s[sidx[pos]], and smax[s] = max |w_s| as float bits
-// (non-negative floats order like their bit patterns; smax cleared first).
-// The sets are the slow grid index, so the random gathers of one set's row
-// run together and hit the caches.  One block covers kSortPer * 256
-// positions and issues one atomic (same-address atomics serialize).
-constexpr int kSortPer = 8;
-__global__ __launch_bounds__(256) void k_sort_weights(const double* __restrict__ w, int64_t ldw,
-                                                      const int* __restrict__ nvalid,
-                                                      const int* __restrict__ sidx, int64_t n,
-                                                      float* __restrict__ wsf, int64_t ldws,
-                                                      unsigned* __restrict__ smax) {
-  __shared__ float red[4];
-  const int s = blockIdx.y;
-  const int nv = *nvalid;
-  const double* ws = w + (int64_t)s * ldw;
-  float* out = wsf + (int64_t)s * ldws;
-  float mx = 0.f;
+// The sort of the weight sets into the scatter's order is two coalesced
+// passes instead of nsets random 8-byte gathers:
+//  k_transpose_w: wt[idx][s] = (float) w_s[idx]  (point-major, nsets floats a
+//                 point), 64 x 64 tiles through LDS;
+//  k_gather_sets: wsf[s][pos] = wt[sidx[pos]][s] (one 4 * nsets-byte row per
+//                 sorted point), zero past the valid points up to n, again
+//                 through a 64 x 64 LDS tile, and smax[s] = max |w_s| as float
+//                 bits (non-negative floats order like their bit patterns;
+//                 smax cleared first).  A block covers kGatherTiles tiles and
+//                 issues one atomic per set (same-address atomics serialize).
+constexpr int kTile = 64;
+constexpr int kGatherTiles = 16;
+__global__ __launch_bounds__(256) void k_transpose_w(const double* __restrict__ w, int64_t ldw,
+                                                     int64_t Np, int nsets,
+                                                     float* __restrict__ wt) {
+  __shared__ float tile[kTile][kTile + 1];  // [idx][set]
+  const int64_t i0 = (int64_t)blockIdx.x * kTile;
+  const int s0 = blockIdx.y * kTile;
+  const int t = threadIdx.x, lo = t & 63, hi = t >> 6;
 #pragma unroll
-  for (int i = 0; i < kSortPer; ++i) {
-    const int64_t pos = ((int64_t)blockIdx.x * kSortPer + i) * 256 + threadIdx.x;
-    float v = 0.f;
-    if (pos < nv) v = (float)ws[sidx[pos]];
-    if (pos < n) out[pos] = v;
-    mx = fmaxf(mx, fabsf(v));
+  for (int r = 0; r < kTile / 4; ++r) {
+    const int sl = hi + 4 * r;
+    const int64_t i = i0 + lo;
+    tile[lo][sl] = (s0 + sl < nsets && i < Np) ? (float)w[(int64_t)(s0 + sl) * ldw + i] : 0.f;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kTile / 4; ++r) {
+    const int il = hi + 4 * r;
+    const int64_t i = i0 + il;
+    if (s0 + lo < nsets && i < Np) wt[i * nsets + s0 + lo] = tile[il][lo];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_sets(const float* __restrict__ wt, int nsets,
+                                                     const int* __restrict__ nvalid,
+                                                     const int* __restrict__ sidx, int64_t n,
+                                                     float* __restrict__ wsf, int64_t ldws,
+                                                     unsigned* __restrict__ smax) {
+  __shared__ float tile[kTile][kTile + 1];  // [pos][set]
+  const int nv = *nvalid;
+  const int t = threadIdx.x, lo = t & 63, hi = t >> 6;
+  float mx[kTile / 4];
+#pragma unroll
+  for (int r = 0; r < kTile / 4; ++r) mx[r] = 0.f;
+  for (int tt = 0; tt < kGatherTiles; ++tt) {
+    const int64_t p0 = ((int64_t)blockIdx.x * kGatherTiles + tt) * kTile;
+    if (p0 >= n) break;  // uniform
+#pragma unroll
+    for (int r = 0; r < kTile / 4; ++r) {
+      const int pl = hi + 4 * r;
+      const int64_t pos = p0 + pl;
+      float v = 0.f;
+      if (pos < nv && lo < nsets) v = wt[(int64_t)sidx[pos] * nsets + lo];
+      tile[pl][lo] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kTile / 4; ++r) {
+      const int sl = hi + 4 * r;
+      const int64_t pos = p0 + lo;
+      const float v = tile[lo][sl];
+      if (sl < nsets && pos < n) wsf[(int64_t)sl * ldws + pos] = v;
+      mx[r] = fmaxf(mx[r], fabsf(v));
+    }
+    __syncthreads();
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-  __syncthreads();
-  if (threadIdx.x == 0)
-    atomicMax(smax + s, __float_as_uint(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))));
+  for (int r = 0; r < kTile / 4; ++r) {
+    float m = mx[r];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    const int sl = hi + 4 * r;
+    if (lo == 0 && sl < nsets) atomicMax(smax + sl, __float_as_uint(m));
+  }
 }
 
 // Weight-set scatter, RT row tiles of 32 sets (RT = 2: up to 64 sets).
@@ -126,14 +169,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
   const int b = c * chunk, e = min(nvalid, b + chunk);
   if (b >= e) return;
 
-  // double-buffered copies of a block's weights (FP32, unscaled) and records,
-  // filled by LDS DMA (global_load_lds) while the previous block is
-  // multiplied out; z, h and segments of the current block
-  __shared__ __attribute__((aligned(16))) float sWr[2][32 * RT][kRowF];  // [buf][set][point]
-  __shared__ PointRec sRec[2][kBlk];
-  __shared__ __attribute__((aligned(16))) float sZ[2 * kOrders][kRowF];  // [(m, re|im)][point]
-  __shared__ __attribute__((aligned(16))) float sH[4][kBlk];
-  __shared__ int sSeg[kBlk];
+  // double-buffered: a block's weights (FP32, unscaled; filled by LDS DMA,
+  // global_load_lds) and its z, h and segments (computed by the threads);
+  // block i+1's are written while block i is multiplied out
+  __shared__ __attribute__((aligned(16))) float sWr[2][32 * RT][kRowF];   // [buf][set][point]
+  __shared__ __attribute__((aligned(16))) float sZ[2][2 * kOrders][kRowF];  // [buf][(m, re|im)][point]
+  __shared__ __attribute__((aligned(16))) float sH[2][4][kBlk];
+  __shared__ int sSeg[2][kBlk];
   __shared__ float sInv[32 * RT];  // 2^e_s
 
   // weight set s is scaled by 2^-e_s, max |w_s| 2^-e_s in [0.5, 1); this
@@ -151,18 +193,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
     if (wv == 0 && l < 32) sInv[s] = ldexpf(1.0f, ex);
   }
 
-  int blk = b, bn = 0;  // current block and its number of points
+  int blk = b, bn = 0, buf = 0;  // current block, its number of points, its buffer
 
-  // LDS DMA of the block at pblk into buffer nb.  Records: wavefront wv
-  // copies records 16 wv .. 16 wv + 15 (lane l: 16 bytes, quarter l % 4 of
-  // record l / 4; padding records past e repeat record pblk).  Weights: one
-  // 256-byte row per instruction, sets wv + 4 t below nsets (rows of other
-  // sets are never used: their results are not written).  wsf rows are long
-  // enough for a whole block.
-  auto dma = [&](int pblk, int nb) {
-    const int ri = pblk + 16 * wv + (l >> 2);
-    const char* src = reinterpret_cast<const char*>(rec + (ri < e ? ri : pblk)) + 16 * (l & 3);
-    __builtin_amdgcn_global_load_lds(src, &sRec[nb][16 * wv], 16, 0, 0);
+  // LDS DMA of the weights of the block at pblk into buffer nb: one 256-byte
+  // row per instruction, sets wv + 4 t below nsets (rows of other sets are
+  // never used: their results are not written).  wsf rows are long enough
+  // for a whole block.
+  auto dma_w = [&](int pblk, int nb) {
 #pragma unroll
     for (int t = 0; t < 8 * RT; ++t) {
       const int s = wv + 4 * t;
@@ -172,14 +209,29 @@ __global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
       }
     }
   };
-  // z(m, p) = e^{-i m phi_p} for the workgroup's orders by an FP64 chain from
-  // e1^(64 my + 16 g) (thread: point gp, orders 16 g .. 16 g + 15), h_g, seg
+  // staging: thread (point gp, group gg) computes z(m, p) = e^{-i m phi_p}
+  // for orders 16 gg .. 16 gg + 15 of the workgroup by an FP64 chain from
+  // e1^(64 my + 16 gg), and h_gg; the chain runs in four slices of four
+  // orders
   const int gp = tid & (kBlk - 1);
   const int gg = __builtin_amdgcn_readfirstlane(tid / kBlk);
-  auto stage = [&](int nb) {
-    const bool ok = gp < bn;
-    const PointRec& r = sRec[nb][gp];
-    const cplx e1 = ok ? cplx{r.ere, r.eim} : cplx{1.0, 0.0};
+  struct RecF {
+    double re, im, h;
+    int seg;
+  };
+  // this thread's record fields of the block at pblk (padding points repeat
+  // the block's first record)
+  auto load_rec = [&](int pblk) {
+    const int pbn = min(kBlk, e - pblk);
+    const PointRec* pr = rec + pblk + (gp < pbn ? gp : 0);
+    RecF f;
+    f.re = pr->ere;
+    f.im = pr->eim;
+    f.h = gg == 0 ? pr->h0 : gg == 1 ? pr->h1 : gg == 2 ? pr->h2 : pr->h3;
+    f.seg = pr->seg;
+    return f;
+  };
+  auto zbase = [&](cplx e1) {
     cplx z{1.0, 0.0}, pw = e1;
     int nn = kOrders * my + 16 * gg;
     while (nn) {
@@ -187,15 +239,25 @@ __global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
       pw = csq(pw);
       nn >>= 1;
     }
+    return z;
+  };
+  // orders 16 gg + q0 .. + 3 from the chain state zs (= z at q0) into buffer
+  // nb; returns the state at q0 + 4
+  auto zslice = [&](int nb, cplx e1, cplx zs, int q0) {
+    cplx z = zs;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      sZ[2 * (16 * gg + q)][gp] = (float)z.x;
-      sZ[2 * (16 * gg + q) + 1][gp] = (float)z.y;
-      if (q < 15) z = cmul(z, e1);
+    for (int q = 0; q < 4; ++q) {
+      sZ[nb][2 * (16 * gg + q0 + q)][gp] = (float)z.x;
+      sZ[nb][2 * (16 * gg + q0 + q) + 1][gp] = (float)z.y;
+      z = cmul(z, e1);
     }
-    const double hk = gg == 0 ? r.h0 : gg == 1 ? r.h1 : gg == 2 ? r.h2 : r.h3;
-    sH[gg][gp] = ok ? (float)hk : 0.f;
-    if (gg == 0) sSeg[gp] = ok ? r.seg : -1;
+    return z;
+  };
+  // h and segment of the block in buffer nb (the segment written by all four
+  // groups, with the same value)
+  auto stage_hs = [&](int nb, const RecF& f, bool ok) {
+    sH[nb][gg][gp] = ok ? (float)f.h : 0.f;
+    sSeg[nb][gp] = ok ? f.seg : -1;
   };
 
   // this lane's column: m = 64 my + 16 wv + (col >> 1), component col & 1
@@ -253,87 +315,127 @@ __global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
       ++run;
       const int off = p - blk;  // point p may lie past this block
       if (off < bn)
-        cur = __builtin_amdgcn_readfirstlane(sSeg[off]);
+        cur = __builtin_amdgcn_readfirstlane(sSeg[buf][off]);
       else
         cur = __builtin_amdgcn_readfirstlane(rec[p].seg);
       run_end = min(e, seg_start[cur + 1]);
     }
   };
 
-  dma(b, 0);
-  for (int buf = 0; blk < e; blk += kBlk, buf ^= 1) {
+  // block b is staged up front
+  dma_w(b, 0);
+  {
+    const RecF f = load_rec(b);
+    const bool ok = gp < min(kBlk, e - b);
+    const cplx e1 = ok ? cplx{f.re, f.im} : cplx{1.0, 0.0};
+    cplx z = zbase(e1);
+#pragma unroll
+    for (int q0 = 0; q0 < 16; q0 += 4) z = zslice(0, e1, z, q0);
+    stage_hs(0, f, ok);
+  }
+  RecF fn{1.0, 0.0, 0.0, -1};  // record fields of the next block
+  if (b + kBlk < e) fn = load_rec(b + kBlk);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+
+  for (; blk < e; blk += kBlk, buf ^= 1) {
     bn = min(kBlk, e - blk);
-    // this wavefront's copies of block blk have landed; after the barrier,
-    // every wavefront's have, and the previous block's products (readers of
-    // sZ, sH, sSeg and buffer buf ^ 1) are done
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-    stage(buf);
-    if (blk + kBlk < e) dma(blk + kBlk, buf ^ 1);  // lands during this block's products
-    __syncthreads();
-    for (int wo = 0; wo < bn; wo += kWin) {
-      const int wlo = blk + wo, whi = blk + min(wo + kWin, bn);
-      const int po = wo + 8 * hq;  // this lane's K slice: points po .. po+7
-      float zv[8];
-      *reinterpret_cast<float4*>(&zv[0]) = *reinterpret_cast<const float4*>(&sZ[zrow][po]);
-      *reinterpret_cast<float4*>(&zv[4]) = *reinterpret_cast<const float4*>(&sZ[zrow][po + 4]);
-      half8 whi_[RT], wlo_[RT];
+    // the next block: its weights by DMA now, its z, h and segments in
+    // slices interleaved with this block's products (computed for a
+    // nonexistent next block too, into the idle buffer, which keeps the
+    // product code free of branches); the block after it: record fields
+    const int nblk = blk + kBlk;
+    const int nbn = min(kBlk, e - nblk);  // <= 0 if there is no next block
+    if (nblk < e) dma_w(nblk, buf ^ 1);
+    RecF fnn{1.0, 0.0, 0.0, -1};
+    if (nblk + kBlk < e) fnn = load_rec(nblk + kBlk);
+    const bool nok = gp < nbn;
+    const cplx ne1 = nok ? cplx{fn.re, fn.im} : cplx{1.0, 0.0};
+    cplx zst = ne1;  // chain state between slices
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) {
-        float wa[8];
-        const float* wr = &sWr[buf][32 * rt + (l & 31)][po];
-        *reinterpret_cast<float4*>(&wa[0]) = *reinterpret_cast<const float4*>(wr);
-        *reinterpret_cast<float4*>(&wa[4]) = *reinterpret_cast<const float4*>(wr + 4);
+    for (int w = 0; w < kBlk / kWin; ++w) {
+      const int wo = kWin * w;
+      if (wo < bn) {  // uniform
+        const int wlo = blk + wo, whi = blk + min(wo + kWin, bn);
+        const int po = wo + 8 * hq;  // this lane's K slice: points po .. po+7
+        float zv[8];
+        *reinterpret_cast<float4*>(&zv[0]) = *reinterpret_cast<const float4*>(&sZ[buf][zrow][po]);
+        *reinterpret_cast<float4*>(&zv[4]) =
+            *reinterpret_cast<const float4*>(&sZ[buf][zrow][po + 4]);
+        half8 whi_[RT], wlo_[RT];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) wa[j] *= asc[rt];
-        split8_f16(wa, whi_[rt], wlo_[rt]);
-      }
-      // one pass per run the window meets (one pass for almost all windows):
-      // this window's 4 * 3 * RT products over the points [p, vhi) of run
-      // `cur`, the other points' z zeroed.  A single code path for all
-      // windows keeps each accumulator in one register set.
-      int p = wlo;
-      while (p < whi) {
-        const int vhi = min(whi, run_end);
-        float zm[8];
-        const int rel0 = blk + po - p;
+        for (int rt = 0; rt < RT; ++rt) {
+          float wa[8];
+          const float* wr = &sWr[buf][32 * rt + (l & 31)][po];
+          *reinterpret_cast<float4*>(&wa[0]) = *reinterpret_cast<const float4*>(wr);
+          *reinterpret_cast<float4*>(&wa[4]) = *reinterpret_cast<const float4*>(wr + 4);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) zm[j] = (unsigned)(rel0 + j) < (unsigned)(vhi - p) ? zv[j] : 0.f;
-        half8 bh[4], bl[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          float x[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) x[j] = sH[k][po + j] * zm[j];
-          split8_f16(x, bh[k], bl[k]);
+          for (int j = 0; j < 8; ++j) wa[j] *= asc[rt];
+          split8_f16(wa, whi_[rt], wlo_[rt]);
         }
-        // 3 rounds over the 4 RT independent accumulators, so that an MFMA
-        // never waits for the one just issued
+        // one pass per run the window meets (one pass for almost all
+        // windows): this window's 4 * 3 * RT products over the points
+        // [p, vhi) of run `cur`, the other points' z zeroed, and slice w of
+        // the next block's staging (repeated identically by a second pass).
+        // A single code path for all windows keeps each accumulator in one
+        // register set.
+        cplx zout = zst;
+        int p = wlo;
+        while (p < whi) {
+          const int vhi = min(whi, run_end);
+          float zm[8];
+          const int rel0 = blk + po - p;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+          for (int j = 0; j < 8; ++j)
+            zm[j] = (unsigned)(rel0 + j) < (unsigned)(vhi - p) ? zv[j] : 0.f;
+          half8 bh[4], bl[4];
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) x[j] = sH[buf][k][po + j] * zm[j];
+            split8_f16(x, bh[k], bl[k]);
+          }
+          if (w == 0) {  // compile-time
+            zout = zslice(buf ^ 1, ne1, zbase(ne1), 0);
+            stage_hs(buf ^ 1, fn, nok);
+          } else {
+            zout = zslice(buf ^ 1, ne1, zst, 4 * w);
+          }
+          // 3 rounds over the 4 RT independent accumulators, so that an
+          // MFMA never waits for the one just issued
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+              acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(whi_[rt], bh[k], acc[rt][k], 0, 0, 0);
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+              acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(whi_[rt], bl[k], acc[rt][k], 0, 0, 0);
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+              acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wlo_[rt], bh[k], acc[rt][k], 0, 0, 0);
+          p = vhi;
+          if (p == run_end) finish_run(p);
+          // keep the accumulators in AGPRs across the loop (otherwise the
+          // compiler carries them in VGPRs and copies all of them every pass)
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
-            acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(whi_[rt], bh[k], acc[rt][k], 0, 0, 0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-            acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(whi_[rt], bl[k], acc[rt][k], 0, 0, 0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
-            acc[rt][k] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wlo_[rt], bh[k], acc[rt][k], 0, 0, 0);
-        p = vhi;
-        if (p == run_end) finish_run(p);
-        // keep the accumulators in AGPRs across the loop (otherwise the
-        // compiler carries them in VGPRs and copies all of them every pass)
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-          for (int k = 0; k < 4; ++k) asm volatile("" : "+a"(acc[rt][k]));
+            for (int k = 0; k < 4; ++k) asm volatile("" : "+a"(acc[rt][k]));
+        }
+        zst = zout;
       }
     }
+    fn = fnn;
+    // the next block's weights have landed (this wavefront's DMA) and, after
+    // the barrier, its staging is complete and this block's readers are done
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
   }
 }
 
@@ -352,10 +454,16 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
   DSHT_LAUNCH_CHECK();
   k_sorted_idx<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, nvalid, ws.rec, ws.sidx);
   DSHT_LAUNCH_CHECK();
+  // wt (point-major copy) lives in the second half of the wsf allocation
+  float* wt = ws.wsf + (int64_t)ws.cap_part_sets * ws.ldws;
+  k_transpose_w<<<dim3((unsigned)((Np + kTile - 1) / kTile), (nsets + kTile - 1) / kTile), 256, 0,
+                  s>>>(w, ldw, Np, nsets, wt);
+  DSHT_LAUNCH_CHECK();
   // rows cover every position a staged block reads: (Np rounded up to 64) + 64 <= ldws
   const int64_t n = (Np + 63) / 64 * 64 + 64;
-  k_sort_weights<<<dim3((unsigned)((n + 256 * kSortPer - 1) / (256 * kSortPer)), nsets), 256, 0, s>>>(
-      w, ldw, nvalid, ws.sidx, n, ws.wsf, ws.ldws, ws.wmax);
+  const int64_t per = (int64_t)kTile * kGatherTiles;
+  k_gather_sets<<<(unsigned)((n + per - 1) / per), 256, 0, s>>>(wt, nsets, nvalid, ws.sidx, n,
+                                                                ws.wsf, ws.ldws, ws.wmax);
   DSHT_LAUNCH_CHECK();
   const int nmy = (pl.Nl + kOrders - 1) / kOrders;
   const int64_t nwg = (int64_t)8 * ((nchunks + 7) / 8) * nmy;
