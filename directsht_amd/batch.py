@@ -2,18 +2,18 @@
 
 A batch is a list of point sets (device-resident theta, phi, w; each with its
 own Nl, Nx).  BatchTransform prepares everything once -- resident tables per
-(Nl, Nx), a tuned configuration per (Nl, size bucket), workspaces reserved to
-size, output buffers -- and captures the whole batch as one HIP graph.  Sets
-that share tables and configuration go through one dsht_transform_many call
-(up to 64 sets): each set is binned, scattered and folded into its own slot,
-and the table contraction (a matrix-core GEMM over all slots) and the polar
-rotation run once for the call, so the tables are streamed from HBM once per
-call instead of once per set.  Calls are spread over several streams
+(Nl, Nx), a tuned configuration per tables (the one tuned for the size bucket
+of their largest set), workspaces reserved to size, output buffers -- and
+captures the whole batch as one HIP graph.  Sets that share tables go through
+one dsht_transform_many call (up to 64 sets): each set is binned, scattered and folded
+into its own slot, and the table contraction (a matrix-core GEMM over all
+slots) and the polar rotation run once for the call, so the tables are
+streamed from HBM once per call instead of once per set.  Calls are spread over several streams
 (longest first, onto the least loaded stream) so they overlap, and the graph
 is replayed with a single launch.  Nothing in a replay allocates or
 synchronizes.
 
-Autotuner: for each (Nl, Nx, size bucket) present in a batch it times every
+Autotuner: for each (Nl, Nx, size bucket) it is asked about it times every
 candidate (scatter engine, points per scatter work item) on a synthetic
 uniform-sphere set of the bucket's size and keeps the fastest; results are
 cached per process (and optionally in a JSON file).
@@ -100,7 +100,7 @@ class BatchTransform:
     """
 
     def __init__(self, sets, dflt_type="float32", xmax=0.75, nstreams=4, tuner=None,
-                 nx_factor=2, device=None, graph=True):
+                 nx_factor=2, device=None, graph=True, split_calls=False):
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = torch.device(device)
@@ -116,18 +116,39 @@ class BatchTransform:
         for (_, _, _, Nl, Nx) in self.sets:
             if (Nl, Nx) not in self.bases:
                 self.bases[(Nl, Nx)] = DirectSHT(Nl, Nx, xmax, dflt_type, device=self.device)
-        # tuned configuration per set
-        self.config = [self.tuner.choose(self.bases[(s[3], s[4])], s[0].numel()) for s in self.sets]
-        # calls: sets with the same tables and configuration, up to MAX_SETS per
-        # dsht_transform_many call (one table stream for the whole call)
-        groups = {}
+        # tuned configuration per (tables): the one tuned for the largest set
+        # of those tables (the scatter of the large sets dominates), so that
+        # all sets sharing tables can share a call and its table stream
+        by_tables = {}
         for i, s in enumerate(self.sets):
-            groups.setdefault((s[3], s[4]) + tuple(self.config[i]), []).append(i)
+            by_tables.setdefault((s[3], s[4]), []).append(i)
+        self.config = [None] * len(self.sets)
+        for tk, idx in by_tables.items():
+            big = max(idx, key=lambda i: self.sets[i][0].numel())
+            cfg = tuple(self.tuner.choose(self.bases[tk], self.sets[big][0].numel()))
+            for i in idx:
+                self.config[i] = cfg
+        # calls: sets with the same tables, up to MAX_SETS per
+        # dsht_transform_many call (one table stream for the whole call).
+        # With split_calls, a call much longer than a stream's fair share is
+        # split in two (alternating sets by size), trading another table
+        # stream for balance across the streams (off by default: on the
+        # uneven 200-set batch it was slower)
         self.calls = []
-        for key, idx in groups.items():
+        for tk, idx in by_tables.items():
             idx.sort(key=lambda i: -self.sets[i][0].numel())
+            key = tk + self.config[idx[0]]
             for c0 in range(0, len(idx), _lib.MAX_SETS):
                 self.calls.append((key, idx[c0:c0 + _lib.MAX_SETS]))
+        nstr = max(1, int(nstreams))
+        while split_calls and nstr > 1 and len(self.calls) < 4 * nstr:
+            costs = [self._call_cost(c) for c in self.calls]
+            j = max(range(len(self.calls)), key=lambda q: costs[q])
+            key, idx = self.calls[j]
+            if len(idx) < 2 or costs[j] <= 1.25 * sum(costs) / nstr:
+                break
+            self.calls[j] = (key, idx[0::2])
+            self.calls.append((key, idx[1::2]))
         # assignment of calls to streams: longest first onto the least loaded
         cost = [self._call_cost(c) for c in self.calls]
         order = sorted(range(len(self.calls)), key=lambda i: -cost[i])
@@ -167,9 +188,11 @@ class BatchTransform:
             self._capture()
 
     def _call_cost(self, call):
+        """Relative time of a call: binning and scatter ~ points x Nl; the table
+        contraction streams 8 bytes per (l, m, node), ~10 point-orders each."""
         key, idx = call
         Nl, Nx = key[0], key[1]
-        return sum(self.sets[i][0].numel() * Nl for i in idx) + 0.5 * Nl * (Nl + 1) // 2 * Nx
+        return sum(self.sets[i][0].numel() * Nl for i in idx) + 10 * (Nl * (Nl + 1) // 2) * Nx
 
     def _enqueue(self, origin):
         for j, calls in enumerate(self.assign):

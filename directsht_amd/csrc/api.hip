@@ -78,7 +78,9 @@ static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
   const int tile = sort_tile(Np);
   WsSizes z;
   z.points = std::max<int64_t>(Np, 1);
-  z.chunks = (Np + chunk - 1) / chunk;
+  // the most work items of any transform up to Np points (work_chunk)
+  z.chunks = std::max<int64_t>((Np + chunk - 1) / chunk,
+                               std::min<int64_t>((Np + kChunkMin - 1) / kChunkMin, kChunkTarget));
   z.runs = z.chunks + num_segments(Nx) + 1;
   // tiles of the largest transform up to Np points: sort_tile() keeps every
   // size at or below ~512 tiles, and sizes below 2^21 use ceil(Np / 4096)
@@ -142,7 +144,7 @@ static int64_t rhs_per_pair(const Plan& pl, int nsets) {
 
 static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false,
                             int part_sets = 1) {
-  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk);
+  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk_max);
   Workspace& ws = pl.ws;
   const int64_t rhs_need = (int64_t)pl.Nl * 2 * pl.Npair * rhs_per_pair(pl, nsets);
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
@@ -485,6 +487,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   if ((rc = mark(0))) return rc;
   if (a.shared) {
     const int64_t Np = a.Np[0];
+    pl.chunk = work_chunk(pl.chunk_max, Np);
     ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
     if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
     const bool gathers = scatter_gathers(pl);
@@ -514,6 +517,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   } else {
     for (int set = 0; set < a.nsets; ++set) {
       const int64_t Np = a.Np[set];
+      pl.chunk = work_chunk(pl.chunk_max, Np);
       ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
       if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
         return rc;
@@ -590,10 +594,10 @@ int dsht_plan_configure(dsht_plan* p, int option, int value) {
       set_error("dsht_plan_configure: chunk must be in [64, 65536]");
       return DSHT_EINVAL;
     }
-    if (value != pl.chunk) {
+    if (value != pl.chunk_max) {
       DeviceGuard g(pl.device);
       DSHT_HIP(hipDeviceSynchronize());  // queued transforms use the old decomposition
-      pl.chunk = value;
+      pl.chunk_max = pl.chunk = value;
       pl.ws.cap_chunks = pl.ws.cap_runs = 0;  // force a workspace resize on next use
     }
     return DSHT_OK;
@@ -703,6 +707,7 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.acos_mxmax = s.acos_mxmax;
   d.precision = s.precision;
   d.chunk = s.chunk;
+  d.chunk_max = s.chunk_max;
   d.scatter = s.scatter;
   d.contract = s.contract;
   d.ngroups = s.ngroups;

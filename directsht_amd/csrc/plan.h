@@ -89,7 +89,9 @@ struct Plan {
   double xmax = 0, dx = 0, acos_xmax = 0, acos_mxmax = 0;
   int precision = DSHT_FP64;
   // scatter decomposition
-  int chunk = 2048;  // sorted points per scatter work item
+  int chunk_max = 2048;  // configured points per scatter work item (the largest used)
+  int chunk = 2048;      // points per scatter work item of the transform being
+                         // enqueued: work_chunk(chunk_max, Np)
   int scatter = DSHT_SCATTER_MATRIX;  // engine of FP32 plans
   int contract = DSHT_CONTRACT_MATRIX;  // engine of batched FP32 contractions
   // contraction layout
@@ -166,6 +168,17 @@ int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipS
 inline bool scatter_gathers(const Plan& pl) {
   return pl.precision == DSHT_FP32 &&
          (pl.scatter == DSHT_SCATTER_MATRIX || pl.scatter == DSHT_SCATTER_SPREAD);
+}
+// Points per scatter work item of a transform of Np points: the configured
+// chunk, or less for a small transform, so that it still spreads over about
+// kChunkTarget work items (at least kChunkMin points each, a multiple of 64).
+constexpr int kChunkMin = 256;
+constexpr int kChunkTarget = 1024;
+inline int work_chunk(int chunk_max, int64_t Np) {
+  int64_t c = (Np + kChunkTarget - 1) / kChunkTarget;
+  c = (c + 63) / 64 * 64;
+  return (int)std::max<int64_t>(std::min<int64_t>(kChunkMin, chunk_max),
+                                std::min<int64_t>(c, chunk_max));
 }
 constexpr int kMfmaMinSets = 4;  // FP32 plans: matrix-core contraction from this many sets
 constexpr int kMaxSets = 64;     // sets per batched transform

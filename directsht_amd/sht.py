@@ -162,10 +162,10 @@ class DirectSHT:
         """Select the scatter engine of FP32 plans ('matrix': fp16x3 MFMA
         direct sums, 'vector': FP32 VALU direct sums, 'spread': per-run
         non-uniform FFT, ~2e-8 relative), the sorted points per scatter work
-        item, and the table contraction of batched FP32 transforms ('matrix':
-        fp16x3 MFMA GEMM over all sets, 'vector': FP32 GEMV per set).
-        Results do not depend on these beyond rounding (and the spreading
-        kernel's ~2e-8)."""
+        item (the largest; small transforms use smaller items), and the
+        table contraction of batched FP32 transforms ('matrix': fp16x3 MFMA
+        GEMM over all sets, 'vector': FP32 GEMV per set).  Results do not
+        depend on these beyond rounding (and the spreading kernel's ~2e-8)."""
         lib = _lib.load()
         if contract is not None:
             v = {"matrix": _lib.DSHT_CONTRACT_MATRIX, "vector": _lib.DSHT_CONTRACT_VECTOR}[contract]
