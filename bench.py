@@ -298,6 +298,13 @@ def main():
               "serving": lambda: bench_serving(nreq=args.serve_requests,
                                                warm=min(100, args.serve_requests // 10))}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
+        if args.copy_calibration:
+            src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+            dst = torch.empty_like(src)
+            for _ in range(3):
+                dst.copy_(src)
+            torch.cuda.synchronize()
+            log("copy calibration: 3 x 1 GiB device copies done")
         return
     Nl, Nx, N = 1000, 2048, int(args.npoints)
     Nlm = Nl * (Nl + 1) // 2

@@ -172,7 +172,7 @@ inline bool scatter_gathers(const Plan& pl) {
 // Points per scatter work item of a transform of Np points: the configured
 // chunk, or less for a small transform, so that it still spreads over about
 // kChunkTarget work items (at least kChunkMin points each, a multiple of 64).
-constexpr int kChunkMin = 256;
+constexpr int kChunkMin = 64;
 constexpr int kChunkTarget = 1024;
 inline int work_chunk(int chunk_max, int64_t Np) {
   int64_t c = (Np + kChunkTarget - 1) / kChunkTarget;

@@ -165,7 +165,7 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
  *                     DSHT_FP64 plans always use the FP64 vector scatter.
  *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536; default
  *                     2048).  A transform of fewer than 1024 x chunk points
- *                     uses smaller items (about Np / 1024, at least 256) so
+ *                     uses smaller items (about Np / 1024, at least 64) so
  *                     that it still spreads over the whole device.
  *   DSHT_OPT_CONTRACT table contraction of batched DSHT_FP32 transforms with
  *                     4 or more sets: DSHT_CONTRACT_MATRIX (default) fp16x3
