@@ -43,7 +43,7 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
                                                         const int* __restrict__ rot_exp,
                                                         const double* __restrict__ rot_AB,
                                                         const double* __restrict__ rot_iB) {
-  const int ell = blockIdx.x;
+  const int ell = Nl - 1 - (int)blockIdx.x;  // the longest recursions first
   const int s0 = blockIdx.y * G;
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   // per set: 0 = not in this block, 1 = zeros (polar range failure),
@@ -82,73 +82,109 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
     }
     __syncthreads();
   }
-  for (int m = threadIdx.x; m <= ell; m += blockDim.x) {
-    const int64_t idx = hp_index(ell, m, Nl);
-    double accE[G], accO[G];
-    double d0 = 0.0;
+  // kChains values of m per thread (m0, m0 + blockDim, ...): independent
+  // recursions interleaved step by step, so the dependent FMA of one hides
+  // behind the others (a lone recursion is latency-bound)
+  constexpr int kChains = 4;
+  for (int m0 = threadIdx.x; m0 <= ell; m0 += kChains * blockDim.x) {
+    int mm[kChains];
+    bool act[kChains];
+    double accE[kChains][G], accO[kChains][G], d0[kChains], cur[kChains], nxt[kChains], twom[kChains];
+    int E[kChains];
 #pragma unroll
-    for (int g = 0; g < G; ++g) accE[g] = accO[g] = 0.0;
+    for (int c = 0; c < kChains; ++c) {
+      mm[c] = m0 + c * (int)blockDim.x;
+      act[c] = mm[c] <= ell;
+      d0[c] = 0.0;
+#pragma unroll
+      for (int g = 0; g < G; ++g) accE[c][g] = accO[c][g] = 0.0;
+    }
     if (any_rot) {
       // Step s = l - m' contributes d_{m',m} * Re a'_{m'} when l + m + m' is
       // even and d_{m',m} * Im a'_{m'} when it is odd, i.e. by the parity of
-      // m + s: even steps use xe, odd steps xo (no divergent branch).
-      const int pe = (m & 1), po = 1 - (m & 1);  // re|im row of even / odd steps
-      const double twom = 2.0 * m;
-      double cur = rot_top[tri + m], nxt = 0.0;
-      int E = rot_exp[tri + m];
+      // m + s: even steps use xe, odd steps xo (no divergent branch).  The
+      // chains of a thread have the same parity (blockDim is even).
+      const int pe = (m0 & 1), po = 1 - (m0 & 1);  // re|im row of even / odd steps
+#pragma unroll
+      for (int c = 0; c < kChains; ++c) {
+        twom[c] = 2.0 * mm[c];
+        cur[c] = act[c] ? rot_top[tri + mm[c]] : 0.0;  // inactive chains stay 0
+        E[c] = act[c] ? rot_exp[tri + mm[c]] : 0;
+        nxt[c] = 0.0;
+      }
       int mp = ell;
       // d_{mp-1} = d_mp * (2m/B) - d_{mp+1} * (A/B): one dependent FMA per step
       while (mp >= 2) {
         const int pairs = min(4, mp >> 1);
         for (int q = 0; q < pairs; ++q) {
-          double dv = ldexp(cur, E);
+          const double be = sB[mp], ae = sAB[mp], bo = sB[mp - 1], ao = sAB[mp - 1];
+          double xe[G], xo[G];
 #pragma unroll
-          for (int g = 0; g < G; ++g) accE[g] = fma(dv, sx[(2 * g + pe) * (ell + 1) + mp], accE[g]);
-          double nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
-          nxt = cur;
-          cur = nw;
-          --mp;
-          dv = ldexp(cur, E);
+          for (int g = 0; g < G; ++g) {
+            xe[g] = sx[(2 * g + pe) * (ell + 1) + mp];
+            xo[g] = sx[(2 * g + po) * (ell + 1) + mp - 1];
+          }
 #pragma unroll
-          for (int g = 0; g < G; ++g) accO[g] = fma(dv, sx[(2 * g + po) * (ell + 1) + mp], accO[g]);
-          nw = fma(cur, twom * sB[mp], -nxt * sAB[mp]);
-          nxt = cur;
-          cur = nw;
-          --mp;
+          for (int c = 0; c < kChains; ++c) {
+            double dv = ldexp(cur[c], E[c]);
+#pragma unroll
+            for (int g = 0; g < G; ++g) accE[c][g] = fma(dv, xe[g], accE[c][g]);
+            double nw = fma(cur[c], twom[c] * be, -nxt[c] * ae);
+            nxt[c] = cur[c];
+            cur[c] = nw;
+            dv = ldexp(cur[c], E[c]);
+#pragma unroll
+            for (int g = 0; g < G; ++g) accO[c][g] = fma(dv, xo[g], accO[c][g]);
+            nw = fma(cur[c], twom[c] * bo, -nxt[c] * ao);
+            nxt[c] = cur[c];
+            cur[c] = nw;
+          }
+          mp -= 2;
         }
         // at most 8 steps since the last check: growth per step is below 3l,
         // so values stay below 2^400 * (3l)^8 < 2^1023
-        if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
-          cur *= kScaleLo;
-          nxt *= kScaleLo;
-          E += 400;
-        }
-      }
-      if (mp == 1) {
-        const double dv = ldexp(cur, E);
 #pragma unroll
-        for (int g = 0; g < G; ++g) accE[g] = fma(dv, sx[(2 * g + pe) * (ell + 1) + 1], accE[g]);
-        cur = fma(cur, twom * sB[1], -nxt * sAB[1]);
+        for (int c = 0; c < kChains; ++c)
+          if (fmax(fabs(cur[c]), fabs(nxt[c])) > kScaleHi) {
+            cur[c] *= kScaleLo;
+            nxt[c] *= kScaleLo;
+            E[c] += 400;
+          }
       }
-      d0 = ldexp(cur, E);  // d_{0,m}
+#pragma unroll
+      for (int c = 0; c < kChains; ++c) {
+        if (mp == 1) {
+          const double dv = ldexp(cur[c], E[c]);
+#pragma unroll
+          for (int g = 0; g < G; ++g) accE[c][g] = fma(dv, sx[(2 * g + pe) * (ell + 1) + 1], accE[c][g]);
+          cur[c] = fma(cur[c], twom[c] * sB[1], -nxt[c] * sAB[1]);
+        }
+        d0[c] = ldexp(cur[c], E[c]);  // d_{0,m}
+      }
     }
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      if (mode[g] == 0) continue;
-      const int64_t o = ((int64_t)(s0 + g) * Nlm + idx) * 2;
-      double re = 0.0, im = 0.0;
-      if (mode[g] >= 2) {
-        re = a_equ[o];
-        im = a_equ[o + 1];
+    for (int c = 0; c < kChains; ++c) {
+      if (!act[c]) continue;
+      const int m = mm[c];
+      const int64_t idx = hp_index(ell, m, Nl);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        if (mode[g] == 0) continue;
+        const int64_t o = ((int64_t)(s0 + g) * Nlm + idx) * 2;
+        double re = 0.0, im = 0.0;
+        if (mode[g] >= 2) {
+          re = a_equ[o];
+          im = a_equ[o + 1];
+        }
+        if (mode[g] == 3) {
+          const double sr = (m & 1) ? accO[c][g] : accE[c][g];
+          const double si = (m & 1) ? accE[c][g] : accO[c][g];
+          re += fma(d0[c], sx[(2 * g) * (ell + 1)], 2.0 * sr);
+          im += fma(d0[c], sx[(2 * g + 1) * (ell + 1)], 2.0 * si);
+        }
+        alm[o] = re;
+        alm[o + 1] = im;
       }
-      if (mode[g] == 3) {
-        const double sr = (m & 1) ? accO[g] : accE[g];
-        const double si = (m & 1) ? accE[g] : accO[g];
-        re += fma(d0, sx[(2 * g) * (ell + 1)], 2.0 * sr);
-        im += fma(d0, sx[(2 * g + 1) * (ell + 1)], 2.0 * si);
-      }
-      alm[o] = re;
-      alm[o + 1] = im;
     }
   }
 }
