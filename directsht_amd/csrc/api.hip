@@ -114,7 +114,7 @@ static void free_workspace(Workspace& ws) {
                   (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
                   (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.rmax, (void*)ws.wmax, (void*)ws.wsort,
                   (void*)ws.sidx, (void*)ws.wsf,
-                  (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.a_der, (void*)ws.flags})
+                  (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.a_tmp, (void*)ws.a_der, (void*)ws.flags})
     if (p) (void)hipFree(p);
   ws = Workspace{};
 }
@@ -184,6 +184,7 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   if (with_w) DSHT_ALLOC(ws.wsort, want.points * sizeof(double));
   DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sets * sizeof(double));
   DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sets * sizeof(double));
+  if (sets > 1) DSHT_ALLOC(ws.a_tmp, pl.Nlm * 2 * sets * sizeof(double));
   DSHT_ALLOC(ws.a_der, (int64_t)kContractSplits * pl.Nlm * 8 * sizeof(double));
   DSHT_ALLOC(ws.flags, 4 * sets * sizeof(int));
   ws.cap_points = want.points;

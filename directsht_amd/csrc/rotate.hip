@@ -189,6 +189,46 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
   }
 }
 
+// Between the healpy order (m-major: idx = m(2Nl-1-m)/2 + l) and the l-major
+// order (idx = l(l+1)/2 + m) of nsets arrays [set][Nlm] of complex doubles,
+// through 32 x 32 (l, m) tiles in LDS so that both sides are read and written
+// as contiguous stretches (in healpy order, consecutive l at fixed m; in
+// l-major order, consecutive m at fixed l).  The batched rotation works on
+// whole rows l, which are scattered in the healpy order.
+template <bool kToLMajor>
+__global__ __launch_bounds__(256) void k_tri_transpose(const double2* __restrict__ in,
+                                                       double2* __restrict__ out, int Nl) {
+  __shared__ double2 tile[32][33];  // [m - m0][l - l0]
+  const int l0 = 32 * blockIdx.x, m0 = 32 * blockIdx.y;
+  if (m0 > l0 + 31) return;  // tile above the diagonal: no m <= l
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  const double2* src = in + blockIdx.z * Nlm;
+  double2* dst = out + blockIdx.z * Nlm;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // read: healpy order walks l at fixed m, l-major order walks m at fixed l
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = ty + 8 * r;
+    const int l = kToLMajor ? l0 + tx : l0 + i;
+    const int m = kToLMajor ? m0 + i : m0 + tx;
+    if (l < Nl && m <= l) {
+      const int64_t idx = kToLMajor ? hp_index(l, m, Nl) : (int64_t)l * (l + 1) / 2 + m;
+      tile[m - m0][l - l0] = src[idx];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = ty + 8 * r;
+    const int l = kToLMajor ? l0 + i : l0 + tx;
+    const int m = kToLMajor ? m0 + tx : m0 + i;
+    if (l < Nl && m <= l) {
+      const int64_t idx = kToLMajor ? (int64_t)l * (l + 1) / 2 + m : hp_index(l, m, Nl);
+      dst[idx] = tile[m - m0][l - l0];
+    }
+  }
+}
+
 // Many sets: G sets per workgroup (up to 32), one thread per m (rounds of
 // 256), the recursion for each (l, m) generated once and applied to all G
 // sets.  The a'_{l m'} of the G sets are streamed through LDS in chunks of
@@ -196,23 +236,26 @@ __global__ __launch_bounds__(256) void k_rotate_combine(const double* __restrict
 // the LDS footprint does not grow with l.
 constexpr int kRotChunk = 32;
 
+// Inputs and output in l-major order (k_tri_transpose): a_pol of the sets,
+// and a_equ, which is overwritten with the result (each thread reads its own
+// (l, m) entries before writing them).
 template <int G>
-__global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ a_equ,
+__global__ __launch_bounds__(256) void k_rotate_sets(const double* a_equ,
                                                      const double* __restrict__ a_pol,
-                                                     double* __restrict__ alm, int Nl, int nsets,
+                                                     double* alm, int Nl, int nsets,
                                                      const int* __restrict__ flags, int flag_stride,
                                                      int* __restrict__ status,
                                                      const double* __restrict__ rot_top,
                                                      const int* __restrict__ rot_exp,
                                                      const double* __restrict__ rot_AB,
                                                      const double* __restrict__ rot_iB) {
-  const int ell = blockIdx.x;
+  const int ell = Nl - 1 - (int)blockIdx.x;  // the longest recursions first
   const int s0 = blockIdx.y * G;
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   // a'_{l m'}, m' = hi - row, as [row][re|im][set]: a lane reads its
   // component for two sets per 16-byte LDS read
-  __shared__ __attribute__((aligned(16))) double sv[kRotChunk][2][G];
-  __shared__ double sAB[kRotChunk], sBi[kRotChunk];
+  __shared__ __attribute__((aligned(16))) double sv[1][kRotChunk][2][G];
+  __shared__ double sAB[1][kRotChunk], sBi[1][kRotChunk];
   __shared__ int smode[G];
   if (threadIdx.x < G) {
     const int set = s0 + threadIdx.x;
@@ -244,18 +287,21 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
       for (int hi = ell; hi >= 1; hi -= kRotChunk) {
         const int lo = max(1, hi - kRotChunk + 1);
         const int n = hi - lo + 1;
+        const int b = 0;
         __syncthreads();  // the previous chunk has been consumed
+        // consecutive threads take consecutive m' of one set: a contiguous
+        // stretch in l-major order
         for (int e = threadIdx.x; e < n * G; e += blockDim.x) {
-          const int row = e / G, g = e % G;
-          const int64_t idx = hp_index(ell, hi - row, Nl);
-          const bool r = smode[g] == 3;
-          const double* ap = a_pol + ((int64_t)(s0 + g) * Nlm + idx) * 2;
-          sv[row][0][g] = r ? ap[0] : 0.0;
-          sv[row][1][g] = r ? ap[1] : 0.0;
+          const int row = e % n, g = e / n;
+          double2 v = make_double2(0.0, 0.0);
+          if (smode[g] == 3)
+            v = *reinterpret_cast<const double2*>(a_pol + ((int64_t)(s0 + g) * Nlm + tri + hi - row) * 2);
+          sv[b][row][0][g] = v.x;
+          sv[b][row][1][g] = v.y;
         }
         for (int e = threadIdx.x; e < n; e += blockDim.x) {
-          sAB[e] = rot_AB[tri + hi - e];
-          sBi[e] = rot_iB[tri + hi - e];
+          sAB[b][e] = rot_AB[tri + hi - e];
+          sBi[b][e] = rot_iB[tri + hi - e];
         }
         __syncthreads();
         if (live) {
@@ -265,7 +311,7 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
             const double dv = ldexp(cur, E);
             typedef double d2 __attribute__((ext_vector_type(2)));
             if (((ell - hi + row) & 1) == 0) {  // uniform branch
-              const d2* v = reinterpret_cast<const d2*>(&sv[row][cm][0]);
+              const d2* v = reinterpret_cast<const d2*>(&sv[b][row][cm][0]);
 #pragma unroll
               for (int g = 0; g < G; g += 2) {
                 const d2 a = v[g / 2];
@@ -273,7 +319,7 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
                 accE[g + 1] = fma(dv, a.y, accE[g + 1]);
               }
             } else {
-              const d2* v = reinterpret_cast<const d2*>(&sv[row][cm ^ 1][0]);
+              const d2* v = reinterpret_cast<const d2*>(&sv[b][row][cm ^ 1][0]);
 #pragma unroll
               for (int g = 0; g < G; g += 2) {
                 const d2 a = v[g / 2];
@@ -283,7 +329,7 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
             }
             // d_{m'-1} = d_m' * (2m/B) - d_{m'+1} * (A/B); values stay below 2^1023
             // since they are rescaled as soon as they pass 2^400
-            const double nw = fma(cur, twom * sBi[row], -nxt * sAB[row]);
+            const double nw = fma(cur, twom * sBi[b][row], -nxt * sAB[b][row]);
             nxt = cur;
             cur = nw;
             if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
@@ -297,8 +343,7 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
     }
     if (live) {
       const double d0 = ldexp(cur, E);  // d_{0,m}
-      const int64_t idx = hp_index(ell, m, Nl);
-      const int64_t idx0 = hp_index(ell, 0, Nl);
+      const int64_t idx = tri + m, idx0 = tri;  // l-major
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int md = smode[g];
@@ -323,13 +368,28 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* __restrict__ 
   }
 }
 
+// a_pol -> a_tmp and a_equ -> a_pol in l-major order; the rotation and
+// combine in place in a_pol; back to the healpy order into alm.
 template <int G>
 static int launch_rotate_sets(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
                               int* status, hipStream_t s) {
+  Workspace& ws = pl.ws;
+  if (!ws.a_tmp || nsets > ws.cap_sets) {
+    set_error("dsht: batched rotation without its workspace");
+    return DSHT_EINVAL;
+  }
+  const int nt = (pl.Nl + 31) / 32;
+  const dim3 tg(nt, nt, nsets);
+  k_tri_transpose<true><<<tg, 256, 0, s>>>((const double2*)ws.a_pol, (double2*)ws.a_tmp, pl.Nl);
+  DSHT_LAUNCH_CHECK();
+  k_tri_transpose<true><<<tg, 256, 0, s>>>((const double2*)ws.a_equ, (double2*)ws.a_pol, pl.Nl);
+  DSHT_LAUNCH_CHECK();
   dim3 grid(pl.Nl, (nsets + G - 1) / G);
-  k_rotate_sets<G><<<grid, 256, 0, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, nsets, flags,
+  k_rotate_sets<G><<<grid, 256, 0, s>>>(ws.a_pol, ws.a_tmp, ws.a_pol, pl.Nl, nsets, flags,
                                         flag_stride, status, pl.rot_top, pl.rot_exp, pl.rot_AB,
                                         pl.rot_iB);
+  DSHT_LAUNCH_CHECK();
+  k_tri_transpose<false><<<tg, 256, 0, s>>>((const double2*)ws.a_pol, (double2*)alm, pl.Nl);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
