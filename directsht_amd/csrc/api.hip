@@ -607,6 +607,16 @@ int dsht_plan_configure(dsht_plan* p, int option, int value) {
   return DSHT_EINVAL;
 }
 
+int dsht_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
+                void* stream) {
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  if (Nl < 1 || nsets < 1 || nsets > kMaxSets || ld < Nlm || !alm || !cl) {
+    set_error("dsht_alm2cl: need Nl >= 1, 1 <= nsets <= 64, ld >= Nl(Nl+1)/2, non-null arrays");
+    return DSHT_EINVAL;
+  }
+  return launch_alm2cl(Nl, nsets, alm, ld, cl, cross, (hipStream_t)stream);
+}
+
 int dsht_plan_debug_rhs(dsht_plan* p, void* out, int64_t bytes) {
   if (!p || !out) {
     set_error("dsht_plan_debug_rhs: null argument");

@@ -207,6 +207,9 @@ int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part = 
 // the fold of all nsets slots at once for the matrix-core contraction (FP32),
 // slot s's partials at part + s * part_stride
 int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride);
+// angular power spectra of nsets alm arrays (dsht_alm2cl)
+int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
+                  hipStream_t s);
 // node ranges of the plain contraction (work items per row group)
 #ifndef DSHT_CONTRACT_SPLITS
 #define DSHT_CONTRACT_SPLITS 2

@@ -1,0 +1,84 @@
+// Angular power spectra of alm (healpy order) on the device: the reference's
+// DirectSHT.alm2cl (sht/sht.py:95-109),
+//   C_l = [ (Re a_l0)^2 + 2 sum_{m>0} |a_lm|^2 ] / (2l + 1),
+// and, for pairs of sets, the cross-spectra its callers take with healpy's
+// alm2cl(alm1, alm2) (notebooks/analyzing_mocks.ipynb), with the same m = 0
+// convention (real parts only):
+//   C^{st}_l = [ Re a^s_l0 Re a^t_l0 + 2 sum_{m>0} Re(a^s_lm conj(a^t_lm)) ] / (2l + 1).
+// C^{ss} is the auto-spectrum of set s.
+//
+// One workgroup per l (the longest rows first).  The row's a_lm of all sets
+// are staged through LDS kSpecM orders at a time ([m][set]); each thread
+// accumulates its pairs (s, t) in FP64 in order of m.
+#include "common.h"
+#include "plan.h"
+
+namespace dsht {
+
+constexpr int kSpecM = 32;                                      // orders per LDS chunk
+constexpr int kSpecPairs = (kMaxSets * (kMaxSets + 1) / 2 + 255) / 256;  // pairs per thread
+
+__global__ __launch_bounds__(256) void k_alm2cl(const double2* __restrict__ alm, int64_t ld,
+                                                int Nl, int nsets, int cross,
+                                                double* __restrict__ cl) {
+  __shared__ double2 sa[kSpecM][kMaxSets];
+  const int ell = Nl - 1 - (int)blockIdx.x;
+  const int npairs = cross ? nsets * (nsets + 1) / 2 : nsets;
+  // this thread's pairs p = tid + 256 q: (s, t), s <= t, in row order
+  int ps[kSpecPairs], pt[kSpecPairs];
+#pragma unroll
+  for (int q = 0; q < kSpecPairs; ++q) {
+    int p = threadIdx.x + 256 * q;
+    ps[q] = pt[q] = -1;
+    if (p >= npairs) continue;
+    if (!cross) {
+      ps[q] = pt[q] = p;
+      continue;
+    }
+    int s = 0;
+    while (p >= nsets - s) {  // row s holds pairs (s, s .. nsets-1)
+      p -= nsets - s;
+      ++s;
+    }
+    ps[q] = s;
+    pt[q] = s + p;
+  }
+  double acc[kSpecPairs];
+#pragma unroll
+  for (int q = 0; q < kSpecPairs; ++q) acc[q] = 0.0;
+  for (int m0 = 0; m0 <= ell; m0 += kSpecM) {
+    const int nm = min(kSpecM, ell + 1 - m0);
+    __syncthreads();  // the previous chunk has been used
+    for (int e = threadIdx.x; e < nm * nsets; e += blockDim.x) {
+      const int mi = e % nm, s = e / nm;
+      sa[mi][s] = alm[(int64_t)s * ld + hp_index(ell, m0 + mi, Nl)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kSpecPairs; ++q) {
+      if (ps[q] < 0) continue;
+      double a = acc[q];
+      for (int mi = 0; mi < nm; ++mi) {
+        const double2 x = sa[mi][ps[q]], y = sa[mi][pt[q]];
+        if (m0 + mi == 0)
+          a = fma(x.x, y.x, a);
+        else
+          a = fma(2.0 * x.x, y.x, fma(2.0 * x.y, y.y, a));
+      }
+      acc[q] = a;
+    }
+  }
+  const double inv = 1.0 / (2.0 * ell + 1.0);
+#pragma unroll
+  for (int q = 0; q < kSpecPairs; ++q)
+    if (ps[q] >= 0) cl[(int64_t)(threadIdx.x + 256 * q) * Nl + ell] = acc[q] * inv;
+}
+
+int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
+                  hipStream_t s) {
+  k_alm2cl<<<Nl, 256, 0, s>>>(reinterpret_cast<const double2*>(alm), ld, Nl, nsets, cross, cl);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+}  // namespace dsht

@@ -148,14 +148,46 @@ class DirectSHT:
         return self.Yd[self.indx(ell, m), :]
 
     def alm2cl(self, alm):
-        """1/(2l+1) sum_m |a_lm|^2 -- sht/sht.py:95-109."""
-        alm = alm.cpu().numpy() if isinstance(alm, torch.Tensor) else np.asarray(alm)
-        cl = np.zeros(self.Nell)
-        for ell in range(self.Nell):
-            ii = self.indx(ell, np.arange(1, ell + 1))
-            cl[ell] = alm[self.indx(ell, 0)].real ** 2 + 2 * np.sum(alm[ii].real ** 2 + alm[ii].imag ** 2)
-            cl[ell] /= 2 * ell + 1.0
-        return cl
+        """[(Re a_l0)^2 + 2 sum_{m>0} |a_lm|^2] / (2l+1) -- sht/sht.py:95-109.
+
+        Computed on the device (dsht_alm2cl).  alm: Nlm complex values
+        (numpy -> numpy result, CUDA tensor -> CUDA tensor), or (S, Nlm) for
+        the S auto-spectra (S, Nl)."""
+        return self._spectra(alm, cross=False)
+
+    def cross_spectra(self, alms):
+        """Cross-spectra of S alm arrays (S, Nlm): (S(S+1)/2, Nl) for the pairs
+        (s, t), s <= t, in row order; C^{st}_l = [Re a^s_l0 Re a^t_l0 +
+        2 sum_{m>0} Re(a^s_lm conj(a^t_lm))] / (2l+1), healpy's
+        alm2cl(alm1, alm2) with the m = 0 convention of alm2cl.  On the device."""
+        return self._spectra(alms, cross=True)
+
+    def _spectra(self, alm, cross):
+        on_device = isinstance(alm, torch.Tensor) and alm.is_cuda
+        a = alm if isinstance(alm, torch.Tensor) else torch.from_numpy(np.asarray(alm))
+        one = a.dim() == 1
+        a = a.reshape(-1, a.shape[-1]).to(device=self.device, dtype=torch.complex128).contiguous()
+        S, n = a.shape
+        if n != self.Nlm:
+            raise ValueError(f"alm has {n} values per set, expected Nlm={self.Nlm}")
+        if cross and S > _lib.MAX_SETS:
+            raise ValueError(f"cross_spectra takes at most {_lib.MAX_SETS} sets")
+        lib = _lib.load()
+        out = []
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device)
+            for s0 in range(0, S, _lib.MAX_SETS):
+                grp = a[s0:s0 + _lib.MAX_SETS]
+                k = grp.shape[0]
+                npairs = k * (k + 1) // 2 if cross else k
+                cl = torch.empty((npairs, self.Nell), dtype=torch.float64, device=self.device)
+                _lib.check(lib.dsht_alm2cl(self.Nell, k, grp.data_ptr(), self.Nlm, cl.data_ptr(),
+                                           int(cross), stream.cuda_stream), "dsht_alm2cl")
+                out.append(cl)
+        cl = torch.cat(out) if len(out) > 1 else out[0]
+        if one and not cross:
+            cl = cl[0]
+        return cl if on_device else cl.cpu().numpy()
 
     # ---- tuning -----------------------------------------------------------
     def configure(self, scatter=None, chunk=None, contract=None):

@@ -201,6 +201,21 @@ int dsht_plan_get_timing(dsht_plan* plan, double stage_ms[DSHT_NSTAGES], int64_t
  * polar re, im, node = even, odd node of the pair.  Waits for the device. */
 int dsht_plan_debug_rhs(dsht_plan* plan, void* out, int64_t bytes);
 
+/* Angular power spectra of nsets alm arrays on the device (healpy order,
+ * complex doubles; set s at alm + 2*s*ld doubles, ld >= Nl(Nl+1)/2), written
+ * to the device array cl, enqueued on `stream`, nothing synchronized.
+ *   cross == 0: cl[s*Nl + l], the auto-spectrum of set s,
+ *     C_l = [(Re a_l0)^2 + 2 sum_{m>0} |a_lm|^2] / (2l+1)
+ *     -- DirectSHT.alm2cl, sht/sht.py:95-109;
+ *   cross != 0: cl[p*Nl + l] for the nsets(nsets+1)/2 pairs p = (s, t),
+ *     s <= t, in row order ((0,0), (0,1), .., (0,S-1), (1,1), ..),
+ *     C^{st}_l = [Re a^s_l0 Re a^t_l0 + 2 sum_{m>0} Re(a^s_lm conj(a^t_lm))] / (2l+1)
+ *     -- healpy alm2cl(alm1, alm2) as its callers use it
+ *     (notebooks/analyzing_mocks.ipynb), m = 0 real parts as above.
+ * 1 <= nsets <= 64.  No counterpart in csht. */
+int dsht_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
+                void* stream);
+
 /* Message describing the calling thread's last error ("" if none). */
 const char* dsht_last_error(void);
 

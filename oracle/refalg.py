@@ -18,7 +18,11 @@ Parity pins (tests/test_oracle_tables.py, tests/test_oracle_sht.py):
     exactly, linearity, phi-shift and theta -> pi - theta invariants, and
     csht golden fixtures for the equatorial path;
   * the polar branch: rotation of the exact-oracle alm (oracle.wigner), and
-    independence of the result from the rotation convention.
+    independence of the result from the rotation convention;
+  * alm2cl: the reference's own per-l loop form (sht/sht.py:95-109), closed
+    forms for single modes, and invariance under rotation (oracle.wigner);
+    alm2cl_cross: cross(a, a) = alm2cl(a), symmetry, and the polarization
+    identity cross(a, b) = [alm2cl(a + b) - alm2cl(a - b)] / 4.
 """
 import numpy as np
 
@@ -27,7 +31,7 @@ from . import wigner
 __all__ = [
     "indx", "getlm", "x_grid", "compute_Plm_table", "compute_der_table",
     "norm_ext", "make_tables", "basic_sht", "direct_sht", "rotate_points",
-    "ROT_Y_PLUS_90", "alm2cl",
+    "ROT_Y_PLUS_90", "alm2cl", "alm2cl_cross",
 ]
 
 # R_y(+90 deg), active: n' = R n maps (x, y, z) -> (z, y, -x).  Every rotation
@@ -269,5 +273,16 @@ def alm2cl(alm, Nl):
     """Pseudo-spectrum 1/(2l+1) sum_m |a_lm|^2 -- sht/sht.py:95-109."""
     ell, m = getlm(Nl - 1)
     p = np.where(m == 0, alm.real ** 2, 2 * (alm.real ** 2 + alm.imag ** 2))
+    cl = np.bincount(ell, weights=p, minlength=Nl)
+    return cl / (2 * np.arange(Nl) + 1.0)
+
+
+def alm2cl_cross(a, b, Nl):
+    """Cross-spectrum 1/(2l+1) [Re a_l0 Re b_l0 + 2 sum_{m>0} Re(a_lm conj(b_lm))],
+    healpy's alm2cl(alm1, alm2) as the reference's callers use it
+    (notebooks/analyzing_mocks.ipynb) with alm2cl's m = 0 convention
+    (sht/sht.py:95-109); alm2cl_cross(a, a) = alm2cl(a)."""
+    ell, m = getlm(Nl - 1)
+    p = np.where(m == 0, a.real * b.real, 2 * (a.real * b.real + a.imag * b.imag))
     cl = np.bincount(ell, weights=p, minlength=Nl)
     return cl / (2 * np.arange(Nl) + 1.0)
