@@ -617,6 +617,24 @@ int dsht_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, in
   return launch_alm2cl(Nl, nsets, alm, ld, cl, cross, (hipStream_t)stream);
 }
 
+int dsht_threej000_table(int N, double* store, void* stream) {
+  if (N < 1 || N > tj_max_n() || !store) {
+    set_error("dsht_threej000_table: need 1 <= N <= " + std::to_string(tj_max_n()) +
+              " and a non-null store");
+    return DSHT_EINVAL;
+  }
+  return launch_threej000_table(N, store, (hipStream_t)stream);
+}
+
+int dsht_mode_coupling(int Nl, const double* W, int64_t nW, double* M, void* stream) {
+  if (Nl < 1 || Nl > tj_max_n() || nW < 0 || (nW > 0 && !W) || !M) {
+    set_error("dsht_mode_coupling: need 1 <= Nl <= " + std::to_string(tj_max_n()) +
+              ", nW >= 0, W non-null if nW > 0, and a non-null M");
+    return DSHT_EINVAL;
+  }
+  return launch_mode_coupling(Nl, W, nW, M, (hipStream_t)stream);
+}
+
 int dsht_plan_debug_rhs(dsht_plan* p, void* out, int64_t bytes) {
   if (!p || !out) {
     set_error("dsht_plan_debug_rhs: null argument");

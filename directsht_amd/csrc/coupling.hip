@@ -1,0 +1,190 @@
+// Wigner 3j symbols (l1 l2 l3; 0 0 0) and the pseudo-C_l mode-coupling matrix
+// of a window function, on the device.
+//
+// Both use the closed form of the m = 0 symbol (Racah): with J = l1 + l2 + l3
+// even, g = J/2 and
+//     G(p) = (2p)! / (4^p (p!)^2) = prod_{q=1..p} (2q - 1)/(2q),
+//     H(g) = 1 / (G(g) (2g + 1)),
+//     (l1 l2 l3; 0 0 0)^2 = G(g - l1) G(g - l2) G(g - l3) H(g),
+// sign (-1)^g, zero for odd J or off the triangle |l1 - l2| <= l3 <= l1 + l2.
+// G and H are computed once per device on the host in extended precision
+// (80-bit x87 on the x86 host) and kept in HBM (96 KB), so a symbol costs
+// three products and a square root instead of the reference's recursion of up
+// to j3 steps (sht/threej000.py:9-42, csht/threej000_helper.c:28-56).
+//
+// Mode coupling (MaskDeconvolution.get_M, sht/mask_deconvolution.py:171-192):
+//     M[l1][l2] = (2 l2 + 1)/(4 pi) sum_{l3} (2 l3 + 1) W(l3) (l1 l2 l3; 0 0 0)^2.
+// The sum S(l1, l2) is symmetric, so only l2 <= l1 is summed; with
+// d = l1 - l2 and l3 = d + 2k (k = 0 .. l2), g = l1 + k and
+//     (l1 l2 l3; 0 0 0)^2 = G(k) G(d + k) G(l2 - k) H(l1 + k).
+// A wavefront takes 64 consecutive l2 and four rows l1; G(k) G(l2 - k) is
+// shared by its rows, H(l1 + k) is uniform per row.  The terms are summed in
+// ascending l3, as the reference does.
+#include <mutex>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace dsht {
+
+constexpr int kTjMaxN = 4096;      // largest table size N and mode-coupling Nl
+constexpr int kTjG = 4096;         // G(p), p < kTjG
+constexpr int kTjH = 8192;         // H(g), g < kTjH  (g <= 3(N-1)/2 and <= 2Nl-2)
+constexpr int kTjChunk = 4096;     // table entries per work chunk (16 per thread)
+constexpr int kMcRows = 16;        // rows l1 per mode-coupling tile (4 per wavefront)
+constexpr int kMcCols = 64;        // columns l2 per tile (one per lane)
+
+namespace {
+std::mutex g_tj_mu;
+std::vector<double*> g_tj_coef;    // per device: G[kTjG] then H[kTjH]
+}  // namespace
+
+// G and H for this device (built on its first use; never freed).
+static int tj_coefficients(const double** out) {
+  int dev = 0;
+  DSHT_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_tj_mu);
+  if ((int)g_tj_coef.size() <= dev) g_tj_coef.resize(dev + 1, nullptr);
+  if (!g_tj_coef[dev]) {
+    std::vector<double> h(kTjG + kTjH);
+    long double G = 1.0L;
+    for (int p = 0; p < kTjH; ++p) {
+      if (p > 0) G *= (long double)(2 * p - 1) / (long double)(2 * p);
+      if (p < kTjG) h[p] = (double)G;
+      h[kTjG + p] = (double)(1.0L / (G * (long double)(2 * p + 1)));
+    }
+    double* d = nullptr;
+    DSHT_HIP(hipMalloc(&d, h.size() * sizeof(double)));
+    hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      set_error(std::string("hipMemcpy of the 3j coefficients: ") + hipGetErrorString(e));
+      return DSHT_EHIP;
+    }
+    g_tj_coef[dev] = d;
+  }
+  *out = g_tj_coef[dev];
+  return DSHT_OK;
+}
+
+__host__ __device__ inline int64_t tetra(int64_t j) { return j * (j + 1) * (j + 2) / 6; }
+
+// store[tetra(j1) + j2(j2+1)/2 + j3] = (j1 j2 j3; 0 0 0), N > j1 >= j2 >= j3.
+// The flat index space is cut into chunks of kTjChunk entries; a block walks
+// chunks with a grid stride, each thread 16 entries 256 apart (coalesced
+// stores), carrying j1 forward from the chunk's first entry.
+__global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
+                                                         const double* __restrict__ G,
+                                                         const double* __restrict__ H,
+                                                         double* __restrict__ store) {
+  for (int64_t c0 = (int64_t)blockIdx.x * kTjChunk; c0 < total;
+       c0 += (int64_t)gridDim.x * kTjChunk) {
+    // j1 of the chunk's first entry: the largest j with tetra(j) <= c0
+    int64_t j1 = (int64_t)cbrt(6.0 * (double)c0);
+    while (j1 > 0 && tetra(j1) > c0) --j1;
+    while (tetra(j1 + 1) <= c0) ++j1;
+#pragma unroll 4
+    for (int r = 0; r < kTjChunk / 256; ++r) {
+      const int64_t i = c0 + threadIdx.x + 256 * r;
+      if (i >= total) break;
+      while (tetra(j1 + 1) <= i) ++j1;
+      const int64_t t = i - tetra(j1);
+      int64_t j2 = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+      while (j2 > 0 && j2 * (j2 + 1) / 2 > t) --j2;
+      while ((j2 + 1) * (j2 + 2) / 2 <= t) ++j2;
+      const int64_t j3 = t - j2 * (j2 + 1) / 2;
+      const int64_t J = j1 + j2 + j3;
+      double v = 0.0;
+      if (!(J & 1) && j3 >= j1 - j2) {
+        const int64_t g = J >> 1;
+        v = sqrt(G[g - j1] * G[g - j2] * G[g - j3] * H[g]);
+        if (g & 1) v = -v;
+      }
+      store[i] = v;
+    }
+  }
+}
+
+// One tile: rows l1 in [R0, R0 + 16), columns l2 in [C0, C0 + 64); entries
+// with l2 <= l1 are summed and written to M[l1][l2] and M[l2][l1].  Tiles are
+// taken with the longest sums (largest C0) first.
+__global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __restrict__ W,
+                                                       int64_t nW,
+                                                       const double* __restrict__ G,
+                                                       const double* __restrict__ H,
+                                                       double* __restrict__ M) {
+  const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
+  // block -> (tc, tr): tc from TC-1 down, tr from the first row tile that
+  // reaches the diagonal (R0 + 15 >= C0) to TR-1
+  int b = blockIdx.x, tc = TC - 1, tr0 = 0;
+  for (;; --tc) {
+    tr0 = (tc * kMcCols) / kMcRows;
+    const int n = TR - tr0;
+    if (b < n) break;
+    b -= n;
+  }
+  const int C0 = tc * kMcCols, R0 = (tr0 + b) * kMcRows;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l2 = C0 + lane;
+  const int nl3 = (int)min<int64_t>(nW, 2 * (int64_t)Nl - 1);  // W(l3) = 0 beyond
+  int l1[4], d[4];
+  bool on[4];
+  double acc[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    l1[r] = R0 + 4 * wv + r;
+    d[r] = l1[r] - l2;
+    on[r] = l1[r] < Nl && l2 < Nl && d[r] >= 0;
+    acc[r] = 0.0;
+  }
+  const int kend = min(C0 + kMcCols - 1, R0 + kMcRows - 1);  // largest l2 summed here
+  for (int k = 0; k <= kend; ++k) {
+    const bool kin = k <= l2 && l2 < Nl;
+    const double q = G[k] * (kin ? G[l2 - k] : 0.0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int l3 = d[r] + 2 * k;
+      if (on[r] && kin && l3 < nl3) {
+        const double w = (double)(2 * l3 + 1) * W[l3];
+        const double tj2 = q * G[d[r] + k] * H[l1[r] + k];
+        acc[r] += w * tj2;
+      }
+    }
+  }
+  const double four_pi = 4.0 * M_PI;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (!on[r]) continue;
+    M[(int64_t)l1[r] * Nl + l2] = (acc[r] * (double)(2 * l2 + 1)) / four_pi;
+    if (d[r] > 0) M[(int64_t)l2 * Nl + l1[r]] = (acc[r] * (double)(2 * l1[r] + 1)) / four_pi;
+  }
+}
+
+int launch_threej000_table(int N, double* store, hipStream_t s) {
+  const double* coef = nullptr;
+  int rc = tj_coefficients(&coef);
+  if (rc != DSHT_OK) return rc;
+  const int64_t total = tetra(N);
+  const int64_t chunks = (total + kTjChunk - 1) / kTjChunk;
+  const int nb = (int)std::min<int64_t>(chunks, 8 * 1024);
+  k_threej000_table<<<nb, 256, 0, s>>>(N, total, coef, coef + kTjG, store);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_mode_coupling(int Nl, const double* W, int64_t nW, double* M, hipStream_t s) {
+  const double* coef = nullptr;
+  int rc = tj_coefficients(&coef);
+  if (rc != DSHT_OK) return rc;
+  const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
+  int nb = 0;
+  for (int tc = 0; tc < TC; ++tc) nb += TR - (tc * kMcCols) / kMcRows;
+  k_mode_coupling<<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int tj_max_n() { return kTjMaxN; }
+
+}  // namespace dsht

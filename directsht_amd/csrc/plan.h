@@ -210,6 +210,9 @@ int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64
 // angular power spectra of nsets alm arrays (dsht_alm2cl)
 int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
                   hipStream_t s);
+int launch_threej000_table(int N, double* store, hipStream_t s);
+int launch_mode_coupling(int Nl, const double* W, int64_t nW, double* M, hipStream_t s);
+int tj_max_n();
 // node ranges of the plain contraction (work items per row group)
 #ifndef DSHT_CONTRACT_SPLITS
 #define DSHT_CONTRACT_SPLITS 2

@@ -216,6 +216,24 @@ int dsht_plan_debug_rhs(dsht_plan* plan, void* out, int64_t bytes);
 int dsht_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
                 void* stream);
 
+/* Wigner 3j symbols (j1 j2 j3; 0 0 0) for N > j1 >= j2 >= j3 >= 0, written
+ * to the device array store of N(N+1)(N+2)/6 doubles at
+ * j1(j1+1)(j1+2)/6 + j2(j2+1)/2 + j3 (zero for odd j1+j2+j3 or j3 < j1-j2),
+ * enqueued on `stream`.  1 <= N <= 4096.  The first call on a device uploads a
+ * 96 KB coefficient table (synchronously).
+ * Replaces make_table of csht/threej000_helper.c:62 (a separate library in
+ * the reference, whose symbol name clashes with csht's make_table) and
+ * Wigner3j.get_3js, sht/threej000.py:107-116. */
+int dsht_threej000_table(int N, double* store, void* stream);
+
+/* Mode-coupling matrix of a window function, on the device:
+ *   M[l1*Nl + l2] = (2 l2 + 1)/(4 pi) sum_{l3} (2 l3 + 1) W[l3] (l1 l2 l3; 0 0 0)^2
+ * for 0 <= l1, l2 < Nl, with W (device, nW doubles) taken as zero from nW on
+ * (the reference zero-pads it to 2Nl-1).  Enqueued on `stream`.
+ * 1 <= Nl <= 4096, nW >= 0.  First call on a device: as dsht_threej000_table.
+ * Replaces MaskDeconvolution.get_M, sht/mask_deconvolution.py:171-192. */
+int dsht_mode_coupling(int Nl, const double* W, int64_t nW, double* M, void* stream);
+
 /* Message describing the calling thread's last error ("" if none). */
 const char* dsht_last_error(void);
 

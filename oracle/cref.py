@@ -1,4 +1,5 @@
-"""ctypes driver for the reference's own C code, oracle/_ref/sht_helper.so.
+"""ctypes drivers for the reference's own C code, oracle/_ref/sht_helper.so
+(and, at the end, oracle/_ref/threej000_helper.so).
 
 TEST INFRASTRUCTURE ONLY.  The shared object is compiled by oracle/Makefile
 from csht/sht_helper.c in the reference checkout (it is never copied into this
@@ -84,3 +85,24 @@ class CRef:
             ii = pos[np.argsort(cost[pos])]
             alm += self._one_sign(cost[ii], phi[ii], wt[ii])
         return alm
+
+
+TJ_PATH = os.path.join(_HERE, "_ref", "threej000_helper.so")
+
+
+def threej_available():
+    return os.path.exists(TJ_PATH)
+
+
+def threej000_table(N):
+    """The reference's C Wigner-3j table, make_table(Nl, store)
+    (csht/threej000_helper.c:62-77), compiled by oracle/Makefile:
+    N(N+1)(N+2)/6 values (j1 j2 j3; 0 0 0), N > j1 >= j2 >= j3, at
+    j1(j1+1)(j1+2)/6 + j2(j2+1)/2 + j3."""
+    lib = ct.CDLL(TJ_PATH)
+    lib.make_table.argtypes = [ct.c_int, ct.POINTER(ct.c_double)]
+    lib.make_table.restype = ct.c_int
+    store = np.zeros(N * (N + 1) * (N + 2) // 6)
+    if lib.make_table(N, _p(store)) != 0:
+        raise RuntimeError("make_table failed")
+    return store
