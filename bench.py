@@ -17,8 +17,10 @@ on its stream during the timed steps; cpu_baseline = the reference's own C
 (csht, compiled into oracle/_ref) on a bounded sample of the same workload.
 Also (same line): "batched" -- many weight sets on shared points and an
 uneven batch of point sets, batched vs plain single calls, with errors --
-and "serving" -- p50/p99 latency and aggregate throughput of the transform
-server under Poisson arrivals.  Progress goes to stderr.
+"serving" -- p50/p99 latency and aggregate throughput of the transform
+server under Poisson arrivals -- and "downstream" -- power spectra, the
+mode-coupling matrix and the Wigner-3j table on the device.  Progress goes
+to stderr.
 """
 import argparse
 import json
@@ -215,6 +217,95 @@ def bench_uneven(nsets=200, reps=3):
     return res
 
 
+def _event_ms(fn, reps):
+    """Mean milliseconds of fn() on the current stream, from HIP events
+    around reps calls (after one untimed call)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_downstream(reps=20):
+    """The consumers of alm on the device: power spectra of 64 alm sets at
+    Nl=1000 (dsht_alm2cl, auto and all 2080 cross pairs), the mode-coupling
+    matrix of MaskDeconvolution(564) (the reference notebook's Nl=500 plus 64
+    buffer multipoles) and at Nl=2048 (dsht_mode_coupling), and the Wigner-3j
+    table that MaskDeconvolution(564) builds, N=1127 (dsht_threej000_table).
+    CPU: the reference's C 3j table (csht/threej000_helper.c, OpenMP) on the
+    host's cores at N=1127."""
+    import numpy as np
+    import torch
+
+    from directsht_amd import _lib
+    from directsht_amd.mask_deconvolution import mode_coupling_device
+    from directsht_amd.threej000 import threej000_table_device
+    lib = _lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    out = {}
+    Nl, S = 1000, 64
+    Nlm = Nl * (Nl + 1) // 2
+    g = torch.Generator(device="cuda").manual_seed(3)
+    alm = torch.randn((S, Nlm), dtype=torch.complex128, device="cuda", generator=g)
+    cl = torch.empty((S * (S + 1) // 2, Nl), dtype=torch.float64, device="cuda")
+    for name, cross in (("alm2cl_64sets_Nl1000", 0), ("cross_spectra_64sets_Nl1000", 1)):
+        ms = _event_ms(lambda: _lib.check(lib.dsht_alm2cl(Nl, S, alm.data_ptr(), Nlm, cl.data_ptr(),
+                                                          cross, st), "dsht_alm2cl"), reps)
+        nbytes = S * Nlm * 16
+        out[name] = {"ms": ms, "alm_bytes": nbytes, "hbm_GBs": nbytes / ms / 1e6,
+                     "hbm_frac": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+        log(f"downstream {name}: {out[name]}")
+    del alm, cl
+    rng = np.random.default_rng(1)
+    for nl in (564, 2048):
+        W = torch.from_numpy(np.exp(-np.arange(2 * nl - 1) / 50.0) * rng.uniform(0.5, 1.5, 2 * nl - 1))
+        W = W.cuda()
+        M = torch.empty((nl, nl), dtype=torch.float64, device="cuda")
+        ms = _event_ms(lambda: _lib.check(lib.dsht_mode_coupling(nl, W.data_ptr(), W.numel(),
+                                                                 M.data_ptr(), st),
+                                          "dsht_mode_coupling"), reps)
+        terms = sum((l2 + 1) * (nl - l2) for l2 in range(nl))  # (l1 >= l2, l3) terms
+        flop = 5.0 * terms
+        out[f"mode_coupling_Nl{nl}"] = {"ms": ms, "terms": terms,
+                                        "fp64_TFLOPs": flop / ms / 1e9,
+                                        "fp64_frac": flop / ms / 1e9 / 78.6}
+        log(f"downstream mode_coupling Nl={nl}: {out[f'mode_coupling_Nl{nl}']}")
+        del M
+    N = 1127
+    tab = threej000_table_device(N)
+    ms = _event_ms(lambda: _lib.check(lib.dsht_threej000_table(N, tab.data_ptr(), st),
+                                      "dsht_threej000_table"), 5)
+    nbytes = tab.numel() * 8
+    out["threej000_table_N1127"] = {"ms": ms, "bytes": nbytes, "hbm_GBs": nbytes / ms / 1e6,
+                                    "hbm_frac": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+    del tab
+    torch.cuda.empty_cache()
+    try:
+        from oracle import cref
+        if cref.threej_available():
+            t0 = time.perf_counter()
+            cref.threej000_table(N)
+            dt = time.perf_counter() - t0
+            out["threej000_table_N1127"]["cpu_baseline"] = {
+                "ms": dt * 1e3, "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                "kind": "reference",
+                "sample": "csht/threej000_helper.c make_table(1127), whole table"}
+    except Exception as e:  # reported, never fatal
+        log(f"3j cpu baseline failed: {e!r}")
+    out["reference_notebook_s"] = {
+        "MaskDeconvolution_564": 150.0,
+        "source": "notebooks/analyzing_mocks.ipynb cell 17 output (Wigner3j(1127) + get_M, "
+                  "numba + Python loops, the authors' machine)"}
+    log(f"downstream 3j table: {out['threej000_table_N1127']}")
+    return out
+
+
 def bench_serving(nreq=2100, warm=100, rate=50.0):
     """Serving loop: Poisson arrivals (rate/s) of requests with N log-uniform
     in [1e4, 5e7] and Nl in {256, 512, 1024, 1536} (Nx = 2 Nl), FP32 resident
@@ -274,7 +365,9 @@ def main():
                     help="sorted points per scatter work item of the plain workload")
     ap.add_argument("--skip-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--serve-requests", type=int, default=2100)
-    ap.add_argument("--only", choices=("weight_sets", "uneven", "serving"),
+    ap.add_argument("--skip-downstream", action="store_true",
+                    help="skip the spectra / mode-coupling / 3j timings")
+    ap.add_argument("--only", choices=("weight_sets", "uneven", "serving", "downstream"),
                     help="run only this workload and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run a 1 GiB device copy 3 times (the known-size "
@@ -295,6 +388,7 @@ def main():
     torch.cuda.set_device(0)
     if args.only:
         fn = {"weight_sets": bench_weight_sets, "uneven": bench_uneven,
+              "downstream": bench_downstream,
               "serving": lambda: bench_serving(nreq=args.serve_requests,
                                                warm=min(100, args.serve_requests // 10))}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
@@ -438,6 +532,12 @@ def main():
         except Exception as e:
             log(f"serving failed: {e!r}")
             result["serving"] = {"error": repr(e)}
+    if not args.skip_downstream:
+        try:
+            result["downstream"] = bench_downstream()
+        except Exception as e:
+            log(f"downstream failed: {e!r}")
+            result["downstream"] = {"error": repr(e)}
     print(json.dumps(result), flush=True)
 
 

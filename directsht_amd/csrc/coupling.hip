@@ -17,8 +17,9 @@
 // The sum S(l1, l2) is symmetric, so only l2 <= l1 is summed; with
 // d = l1 - l2 and l3 = d + 2k (k = 0 .. l2), g = l1 + k and
 //     (l1 l2 l3; 0 0 0)^2 = G(k) G(d + k) G(l2 - k) H(l1 + k).
-// A wavefront takes 64 consecutive l2 and four rows l1; G(k) G(l2 - k) is
-// shared by its rows, H(l1 + k) is uniform per row.  The terms are summed in
+// A block takes 64 consecutive l2 (one per lane) and four rows l1, its four
+// wavefronts a quarter of the k's each; G(k) G(l2 - k) is shared by the rows,
+// H(l1 + k) is uniform per row.  Each wavefront sums its terms in
 // ascending l3, as the reference does.
 #include <mutex>
 #include <vector>
@@ -32,7 +33,7 @@ constexpr int kTjMaxN = 4096;      // largest table size N and mode-coupling Nl
 constexpr int kTjG = 4096;         // G(p), p < kTjG
 constexpr int kTjH = 8192;         // H(g), g < kTjH  (g <= 3(N-1)/2 and <= 2Nl-2)
 constexpr int kTjChunk = 4096;     // table entries per work chunk (16 per thread)
-constexpr int kMcRows = 16;        // rows l1 per mode-coupling tile (4 per wavefront)
+constexpr int kMcRows = 4;         // rows l1 per mode-coupling tile
 constexpr int kMcCols = 64;        // columns l2 per tile (one per lane)
 
 namespace {
@@ -80,25 +81,31 @@ __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
                                                          double* __restrict__ store) {
   for (int64_t c0 = (int64_t)blockIdx.x * kTjChunk; c0 < total;
        c0 += (int64_t)gridDim.x * kTjChunk) {
-    // j1 of the chunk's first entry: the largest j with tetra(j) <= c0
+    // j1 of the chunk's first entry: the largest j with tetra(j) <= c0; the
+    // row j1 spans [rs, re) of the flat index
     int64_t j1 = (int64_t)cbrt(6.0 * (double)c0);
     while (j1 > 0 && tetra(j1) > c0) --j1;
     while (tetra(j1 + 1) <= c0) ++j1;
+    int64_t rs = tetra(j1), re = tetra(j1 + 1);
 #pragma unroll 4
     for (int r = 0; r < kTjChunk / 256; ++r) {
       const int64_t i = c0 + threadIdx.x + 256 * r;
       if (i >= total) break;
-      while (tetra(j1 + 1) <= i) ++j1;
-      const int64_t t = i - tetra(j1);
-      int64_t j2 = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+      while (i >= re) {
+        ++j1;
+        rs = re;
+        re += (j1 + 1) * (j1 + 2) / 2;
+      }
+      const int t = (int)(i - rs);  // < N(N+1)/2 < 2^24
+      int j2 = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
       while (j2 > 0 && j2 * (j2 + 1) / 2 > t) --j2;
       while ((j2 + 1) * (j2 + 2) / 2 <= t) ++j2;
-      const int64_t j3 = t - j2 * (j2 + 1) / 2;
-      const int64_t J = j1 + j2 + j3;
+      const int j3 = t - j2 * (j2 + 1) / 2;
+      const int J = (int)j1 + j2 + j3;
       double v = 0.0;
-      if (!(J & 1) && j3 >= j1 - j2) {
-        const int64_t g = J >> 1;
-        v = sqrt(G[g - j1] * G[g - j2] * G[g - j3] * H[g]);
+      if (!(J & 1) && j3 >= (int)j1 - j2) {
+        const int g = J >> 1;
+        v = sqrt(G[g - (int)j1] * G[g - j2] * G[g - j3] * H[g]);
         if (g & 1) v = -v;
       }
       store[i] = v;
@@ -106,17 +113,20 @@ __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
   }
 }
 
-// One tile: rows l1 in [R0, R0 + 16), columns l2 in [C0, C0 + 64); entries
-// with l2 <= l1 are summed and written to M[l1][l2] and M[l2][l1].  Tiles are
-// taken with the longest sums (largest C0) first.
+// One tile: rows l1 in [R0, R0 + 4), columns l2 in [C0, C0 + 64); entries
+// with l2 <= l1 are summed and written to M[l1][l2] and M[l2][l1].  The four
+// wavefronts of a block split the sum (k = w, w + 4, ...) and their partial
+// sums are added in wavefront order.  Tiles are taken with the longest sums
+// (largest C0) first.
 __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __restrict__ W,
                                                        int64_t nW,
                                                        const double* __restrict__ G,
                                                        const double* __restrict__ H,
                                                        double* __restrict__ M) {
+  __shared__ double part[4][kMcRows][kMcCols];
   const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
   // block -> (tc, tr): tc from TC-1 down, tr from the first row tile that
-  // reaches the diagonal (R0 + 15 >= C0) to TR-1
+  // reaches the diagonal (R0 + kMcRows - 1 >= C0) to TR-1
   int b = blockIdx.x, tc = TC - 1, tr0 = 0;
   for (;; --tc) {
     tr0 = (tc * kMcCols) / kMcRows;
@@ -125,40 +135,47 @@ __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __r
     b -= n;
   }
   const int C0 = tc * kMcCols, R0 = (tr0 + b) * kMcRows;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar k loads
   const int l2 = C0 + lane;
   const int nl3 = (int)min<int64_t>(nW, 2 * (int64_t)Nl - 1);  // W(l3) = 0 beyond
-  int l1[4], d[4];
-  bool on[4];
-  double acc[4];
+  int l1[kMcRows], d[kMcRows];
+  bool on[kMcRows];
+  double acc[kMcRows];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    l1[r] = R0 + 4 * wv + r;
+  for (int r = 0; r < kMcRows; ++r) {
+    l1[r] = R0 + r;
     d[r] = l1[r] - l2;
     on[r] = l1[r] < Nl && l2 < Nl && d[r] >= 0;
     acc[r] = 0.0;
   }
-  const int kend = min(C0 + kMcCols - 1, R0 + kMcRows - 1);  // largest l2 summed here
-  for (int k = 0; k <= kend; ++k) {
+  // largest l2 summed here; nothing to sum for an empty window
+  const int kend = nl3 > 0 ? min(C0 + kMcCols - 1, R0 + kMcRows - 1) : -1;
+  // Branch-free body: masked terms load from clamped in-bounds indices and add
+  // zero, so the loads of all rows are in flight together.
+  for (int k = wv; k <= kend; k += 4) {
     const bool kin = k <= l2 && l2 < Nl;
-    const double q = G[k] * (kin ? G[l2 - k] : 0.0);
+    const double q = G[k] * G[kin ? l2 - k : 0];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < kMcRows; ++r) {
       const int l3 = d[r] + 2 * k;
-      if (on[r] && kin && l3 < nl3) {
-        const double w = (double)(2 * l3 + 1) * W[l3];
-        const double tj2 = q * G[d[r] + k] * H[l1[r] + k];
-        acc[r] += w * tj2;
-      }
+      const bool v = on[r] && kin && l3 < nl3;
+      const double w = (double)(2 * l3 + 1) * W[v ? l3 : 0];
+      const double h = H[l1[r] < Nl ? l1[r] + k : 0];  // uniform: a scalar load
+      const double tj2 = q * G[v ? d[r] + k : 0] * h;
+      acc[r] += v ? w * tj2 : 0.0;
     }
   }
-  const double four_pi = 4.0 * M_PI;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (!on[r]) continue;
-    M[(int64_t)l1[r] * Nl + l2] = (acc[r] * (double)(2 * l2 + 1)) / four_pi;
-    if (d[r] > 0) M[(int64_t)l2 * Nl + l1[r]] = (acc[r] * (double)(2 * l1[r] + 1)) / four_pi;
-  }
+  for (int r = 0; r < kMcRows; ++r) part[wv][r][lane] = acc[r];
+  __syncthreads();
+  if (wv >= kMcRows) return;
+  const int r = wv;  // wavefront r writes row R0 + r
+  if (!on[r]) return;
+  const double S = ((part[0][r][lane] + part[1][r][lane]) + part[2][r][lane]) + part[3][r][lane];
+  const double four_pi = 4.0 * M_PI;
+  M[(int64_t)l1[r] * Nl + l2] = (S * (double)(2 * l2 + 1)) / four_pi;
+  if (d[r] > 0) M[(int64_t)l2 * Nl + l1[r]] = (S * (double)(2 * l1[r] + 1)) / four_pi;
 }
 
 int launch_threej000_table(int N, double* store, hipStream_t s) {

@@ -7,9 +7,10 @@
 //   C^{st}_l = [ Re a^s_l0 Re a^t_l0 + 2 sum_{m>0} Re(a^s_lm conj(a^t_lm)) ] / (2l + 1).
 // C^{ss} is the auto-spectrum of set s.
 //
-// One workgroup per l (the longest rows first).  The row's a_lm of all sets
-// are staged through LDS kSpecM orders at a time ([m][set]); each thread
-// accumulates its pairs (s, t) in FP64 in order of m.
+// Cross-spectra: one workgroup per l (the longest rows first).  The row's a_lm
+// of all sets are staged through LDS kSpecM orders at a time ([m][set]); each
+// thread accumulates its pairs (s, t) in FP64 in order of m.  Auto-spectra
+// alone take a coalesced kernel (k_alm2cl_auto).
 #include "common.h"
 #include "plan.h"
 
@@ -74,8 +75,52 @@ __global__ __launch_bounds__(256) void k_alm2cl(const double2* __restrict__ alm,
     if (ps[q] >= 0) cl[(int64_t)(threadIdx.x + 256 * q) * Nl + ell] = acc[q] * inv;
 }
 
+// Auto-spectra: block (64 consecutive l, one set); lane = l, the four
+// wavefronts take m = w, w + 4, ...  For fixed m the l's of a healpy row are
+// contiguous, so each wavefront load is one coalesced 1 KB read and every
+// alm is read once.  The four partial sums are added in wavefront order.
+__global__ __launch_bounds__(256) void k_alm2cl_auto(const double2* __restrict__ alm, int64_t ld,
+                                                     int Nl, double* __restrict__ cl) {
+  __shared__ double part[4][64];
+  const int nlc = (Nl + 63) / 64;
+  const int lc = nlc - 1 - (int)blockIdx.x;  // longest rows first
+  const int s = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ell = lc * 64 + lane;
+  const int mend = min(lc * 64 + 63, Nl - 1);
+  const double2* a = alm + (int64_t)s * ld;
+  double acc = 0.0;
+  const int mtop = ell < Nl ? min(ell, mend) : -1;  // this lane's last m
+  for (int m0 = w; m0 <= mend; m0 += 16) {           // four loads in flight
+    double2 x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int m = m0 + 4 * u;
+      x[u] = m <= mtop ? a[hp_index(ell, m, Nl)] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int m = m0 + 4 * u;
+      acc = m == 0 ? fma(x[u].x, x[u].x, acc)
+                   : fma(2.0 * x[u].x, x[u].x, fma(2.0 * x[u].y, x[u].y, acc));
+    }
+  }
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && ell < Nl) {
+    const double t = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    cl[(int64_t)s * Nl + ell] = t / (2.0 * ell + 1.0);
+  }
+}
+
 int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
                   hipStream_t s) {
+  if (!cross) {
+    k_alm2cl_auto<<<dim3((Nl + 63) / 64, nsets), 256, 0, s>>>(
+        reinterpret_cast<const double2*>(alm), ld, Nl, cl);
+    DSHT_LAUNCH_CHECK();
+    return DSHT_OK;
+  }
   k_alm2cl<<<Nl, 256, 0, s>>>(reinterpret_cast<const double2*>(alm), ld, Nl, nsets, cross, cl);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
