@@ -51,6 +51,7 @@ SIGNATURES = {
     "dsht_plan_debug_rhs": (ct.c_int, [_vp, _vp, ct.c_int64]),
     "dsht_threej000_table": (ct.c_int, [ct.c_int, _vp, _vp]),
     "dsht_mode_coupling": (ct.c_int, [ct.c_int, _vp, ct.c_int64, _vp, _vp]),
+    "dsht_synthesis_rings": (ct.c_int, [ct.c_int, _vp, ct.c_int, _vp, _vp, _vp]),
     "dsht_alm2cl": (ct.c_int, [ct.c_int, ct.c_int, _vp, ct.c_int64, _vp, ct.c_int, _vp]),
     "dsht_last_error": (ct.c_char_p, []),
 }

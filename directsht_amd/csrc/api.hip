@@ -635,6 +635,17 @@ int dsht_mode_coupling(int Nl, const double* W, int64_t nW, double* M, void* str
   return launch_mode_coupling(Nl, W, nW, M, (hipStream_t)stream);
 }
 
+int dsht_synthesis_rings(int Nl, const double* alm, int nrings, const double* x, double* F,
+                         void* stream) {
+  if (Nl < 1 || Nl > synth_max_nl() || nrings < 0 || (nrings > 0 && (!alm || !x || !F))) {
+    set_error("dsht_synthesis_rings: need 1 <= Nl <= " + std::to_string(synth_max_nl()) +
+              ", nrings >= 0 and non-null arrays");
+    return DSHT_EINVAL;
+  }
+  if (nrings == 0) return DSHT_OK;
+  return launch_synth_rings(Nl, alm, nrings, x, F, (hipStream_t)stream);
+}
+
 int dsht_plan_debug_rhs(dsht_plan* p, void* out, int64_t bytes) {
   if (!p || !out) {
     set_error("dsht_plan_debug_rhs: null argument");

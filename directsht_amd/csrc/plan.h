@@ -213,6 +213,9 @@ int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, 
 int launch_threej000_table(int N, double* store, hipStream_t s);
 int launch_mode_coupling(int Nl, const double* W, int64_t nW, double* M, hipStream_t s);
 int tj_max_n();
+int launch_synth_rings(int Nl, const double* alm, int nrings, const double* x, double* F,
+                       hipStream_t s);
+int synth_max_nl();
 // node ranges of the plain contraction (work items per row group)
 #ifndef DSHT_CONTRACT_SPLITS
 #define DSHT_CONTRACT_SPLITS 2

@@ -234,6 +234,18 @@ int dsht_threej000_table(int N, double* store, void* stream);
  * Replaces MaskDeconvolution.get_M, sht/mask_deconvolution.py:171-192. */
 int dsht_mode_coupling(int Nl, const double* W, int64_t nW, double* M, void* stream);
 
+/* Harmonic synthesis on iso-latitude rings, on the device:
+ *   F[r*Nl + m] = sum_{l=m}^{Nl-1} a_lm Ybar_lm(x[r])     (complex doubles)
+ * for alm in healpy order (Nl(Nl+1)/2 complex doubles), ring cosines x
+ * (nrings doubles in [-1, 1]) and Ybar the fully normalized associated
+ * Legendre function with the Condon-Shortley phase; a real field on ring r
+ * is then Re F[r][0] + 2 Re sum_{m>0} F[r][m] e^{i m phi}.  Enqueued on
+ * `stream`.  1 <= Nl <= 2048, nrings >= 0.
+ * Stands in for healpy synfast in the reference's lognormal mocks
+ * (sht/mocks.py:60); no counterpart in csht. */
+int dsht_synthesis_rings(int Nl, const double* alm, int nrings, const double* x, double* F,
+                         void* stream);
+
 /* Message describing the calling thread's last error ("" if none). */
 const char* dsht_last_error(void);
 

@@ -68,3 +68,14 @@ def exact_alm(Nl, theta, phi, w, chunk=4096):
 def exact_alm_multi(Nl, theta, phi, W):
     """exact_alm for several weight vectors W (S, N) sharing the points."""
     return np.stack([exact_alm(Nl, theta, phi, w) for w in np.atleast_2d(W)])
+
+
+def synth_rings(alm, Nl, x):
+    """F[r, m] = sum_{l>=m} a_lm Y_lm(theta_r, 0), cos(theta_r) = x[r]: the
+    harmonic synthesis on rings, from ylm_values (shape (len(x), Nl))."""
+    Y = ylm_values(Nl, np.arccos(np.clip(np.asarray(x, np.float64), -1, 1)))
+    F = np.zeros((len(x), Nl), dtype=np.complex128)
+    for m in range(Nl):
+        i0 = _indx(m, m, Nl)
+        F[:, m] = np.asarray(alm)[i0:i0 + Nl - m] @ Y[i0:i0 + Nl - m]
+    return F
