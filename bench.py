@@ -237,7 +237,9 @@ def bench_downstream(reps=20):
     Nl=1000 (dsht_alm2cl, auto and all 2080 cross pairs), the mode-coupling
     matrix of MaskDeconvolution(564) (the reference notebook's Nl=500 plus 64
     buffer multipoles) and at Nl=2048 (dsht_mode_coupling), and the Wigner-3j
-    table that MaskDeconvolution(564) builds, N=1127 (dsht_threej000_table).
+    table that MaskDeconvolution(564) builds, N=1127 (dsht_threej000_table);
+    the lognormal mock generator at the reference's defaults (nside=2048,
+    lmax=1000, 1e7 points) and its ring synthesis kernel.
     CPU: the reference's C 3j table (csht/threej000_helper.c, OpenMP) on the
     host's cores at N=1127."""
     import numpy as np
@@ -298,6 +300,30 @@ def bench_downstream(reps=20):
                 "sample": "csht/threej000_helper.c make_table(1127), whole table"}
     except Exception as e:  # reported, never fatal
         log(f"3j cpu baseline failed: {e!r}")
+    # lognormal mocks at the reference's defaults (LogNormalMocks(Npnt, nside=2048,
+    # lmax=1000)): the ring synthesis kernel alone, then a whole catalogue
+    from directsht_amd import mocks
+    Nl = 1000
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    alm = mocks.gaussian_alm(0.01 * (10.0 / (np.arange(Nl) + 10.0)) ** 2, gen)
+    x = torch.cos((torch.arange(6144, dtype=torch.float64, device="cuda") + 0.5) * (math.pi / 6144))
+    F = torch.empty((6144, Nl), dtype=torch.complex128, device="cuda")
+    ms = _event_ms(lambda: _lib.check(lib.dsht_synthesis_rings(Nl, alm.data_ptr(), 6144, x.data_ptr(),
+                                                               F.data_ptr(), st),
+                                      "dsht_synthesis_rings"), 5)
+    iters = 6144 * Nl * (Nl + 1) // 2
+    out["synthesis_rings_Nl1000_6144rings"] = {"ms": ms, "ring_modes": iters,
+                                               "fp64_TFLOPs": 6.0 * iters / ms / 1e9}
+    del F
+    lm = mocks.LogNormalMocks(10_000_000, nside=2048, lmax=1000)
+    lm(seed=1, verbose=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, _, _ = lm(seed=2, verbose=False)
+    out["lognormal_mocks_1e7_nside2048"] = {"s": time.perf_counter() - t0, "points": int(t.size),
+                                            "grid": "6144 x 8192 pixels"}
+    log(f"downstream mocks: {out['lognormal_mocks_1e7_nside2048']}")
     out["reference_notebook_s"] = {
         "MaskDeconvolution_564": 150.0,
         "source": "notebooks/analyzing_mocks.ipynb cell 17 output (Wigner3j(1127) + get_M, "
