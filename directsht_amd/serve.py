@@ -114,6 +114,31 @@ class TransformServer:
         self._wake.set()
         return req.future
 
+    def submit_wire(self, frame):
+        """Serve one request frame (directsht_amd.wire): returns a Future of
+        the response frame (bytes).  Malformed frames, requests over the HBM
+        budget and failed transforms give error frames, not exceptions."""
+        from . import wire
+        out = cf.Future()
+        try:
+            rid, Nl, Nx, t, p, w = wire.decode_request(frame)
+        except ValueError as e:
+            out.set_result(wire.encode_error(0, wire.MALFORMED, e))
+            return out
+        # the frame's arrays are read-only views; the request owns a copy
+        fut = self.submit(t.copy(), p.copy(), w.copy(), Nl, Nx if Nx else None)
+
+        def done(f):
+            try:
+                out.set_result(wire.encode_response(rid, Nl, f.result()))
+            except MemoryError as e:
+                out.set_result(wire.encode_error(rid, wire.REJECTED, e, Nl))
+            except Exception as e:  # the transform's own errors (AssertionError, ...)
+                out.set_result(wire.encode_error(rid, wire.FAILED, repr(e), Nl))
+
+        fut.add_done_callback(done)
+        return out
+
     def close(self):
         self._running = False
         self._wake.set()

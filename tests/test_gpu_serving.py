@@ -146,3 +146,32 @@ def test_server_admission_by_hbm(torch_cuda):
     for d, a in zip(data, res):
         assert rel(a, ref(*d)) < TOL["float32"]
     assert need > 0
+
+
+def test_server_wire_frames(torch_cuda):
+    """Request frames in, response frames out (directsht_amd.wire): the alm
+    of a frame equal the same transform alone; malformed, over-budget and
+    failing requests come back as error frames."""
+    torch = torch_cuda
+    from directsht_amd import wire
+    from directsht_amd.serve import TransformServer
+    from directsht_amd.sht import DirectSHT
+    data = [tuple(a.cpu().numpy() for a in d) for d in uniform_sets(torch, 9, [5000, 20000, 1])]
+    nls = [32, 64, 48]
+    with TransformServer(max_inflight=2, dflt_type="float64") as srv:
+        futs = [srv.submit_wire(wire.encode_request(*d, Nl=nl, Nx=0 if i else 3 * nl, rid=100 + i))
+                for i, (d, nl) in enumerate(zip(data, nls))]
+        bad = srv.submit_wire(b"DSHQ" + b"\0" * 10)
+        nan = srv.submit_wire(wire.encode_request([np.nan, 0.1], [0.0, 0.0], [1.0, 1.0], Nl=16))
+        outs = [wire.decode_response(f.result(timeout=120)) for f in futs]
+        assert wire.decode_response(bad.result(timeout=60))[1] == wire.MALFORMED
+        rid, st, alm = wire.decode_response(nan.result(timeout=60))
+        assert st == wire.OK     # the reference drops NaN points
+    for i, ((rid, st, alm), d, nl) in enumerate(zip(outs, data, nls)):
+        assert (rid, st) == (100 + i, wire.OK)
+        ref = DirectSHT(nl, 3 * nl if i == 0 else 2 * nl)(*d)
+        assert rel(alm, ref) < TOL["float64"], i
+    with TransformServer(hbm_budget=1 << 20, max_inflight=1) as small:
+        rid, st, msg = wire.decode_response(
+            small.submit_wire(wire.encode_request(*data[0], Nl=1000, rid=9)).result(timeout=60))
+        assert (rid, st) == (9, wire.REJECTED) and "HBM" in msg
