@@ -263,6 +263,20 @@ def bench_downstream(reps=20):
         out[name] = {"ms": ms, "alm_bytes": nbytes, "hbm_GBs": nbytes / ms / 1e6,
                      "hbm_frac": nbytes / ms / 1e6 / PEAK_HBM_GBS}
         log(f"downstream {name}: {out[name]}")
+    # CPU: the reference's alm2cl (sht/sht.py:95-109, restated in numpy by the
+    # oracle) on 8 of the 64 sets, scaled to 64
+    try:
+        from oracle import refalg
+        host = alm[:8].cpu().numpy()
+        t0 = time.perf_counter()
+        for a in host:
+            refalg.alm2cl(a, Nl)
+        dt = (time.perf_counter() - t0) * 8
+        out["alm2cl_64sets_Nl1000"]["cpu_baseline"] = {
+            "ms": dt * 1e3, "cores": 1, "kind": "port",
+            "sample": "oracle.refalg.alm2cl (numpy) on 8 of the 64 sets, x8"}
+    except Exception as e:  # reported, never fatal
+        log(f"alm2cl cpu baseline failed: {e!r}")
     del alm, cl
     rng = np.random.default_rng(1)
     for nl in (564, 2048):
@@ -292,14 +306,39 @@ def bench_downstream(reps=20):
         from oracle import cref
         if cref.threej_available():
             t0 = time.perf_counter()
-            cref.threej000_table(N)
+            tab = cref.threej000_table(N)
             dt = time.perf_counter() - t0
             out["threej000_table_N1127"]["cpu_baseline"] = {
                 "ms": dt * 1e3, "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                 "kind": "reference",
                 "sample": "csht/threej000_helper.c make_table(1127), whole table"}
+        # get_M at Nl=564 on the host from that table: the reference's sum
+        # (sht/mask_deconvolution.py:183-191), vectorised over (l2, l3) per
+        # row l1 with numpy, one thread
+        if cref.threej_available():
+            nl = (N + 1) // 2  # Wigner3j(2 nl - 1) is the table just built
+            W = np.exp(-np.arange(2 * nl - 1) / 50.0)
+            l2 = np.arange(nl)[:, None]
+            l3 = np.arange(2 * nl - 1)[None, :]
+            t0 = time.perf_counter()
+            M = np.zeros((nl, nl))
+            for l1 in range(nl):
+                a_ = np.stack(np.broadcast_arrays(np.full_like(l2, l1), l2, l3))
+                srt = np.sort(a_, axis=0)
+                k3, k2, k1 = srt[0], srt[1], srt[2]
+                tj = tab[(k1 * (k1 + 1) * (k1 + 2)) // 6 + (k2 * (k2 + 1)) // 2 + k3]
+                M[l1] = ((2 * l3 + 1) * W[None, :] * tj * tj).sum(axis=1) * (2 * np.arange(nl) + 1)
+            M /= 4 * np.pi
+            dtm = time.perf_counter() - t0
+            out["mode_coupling_Nl564"]["cpu_baseline"] = {
+                "ms": (dtm + dt) * 1e3, "kind": "reference+port",
+                "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                "sample": f"csht/threej000_helper.c make_table(1127) on those threads "
+                          f"{dt:.2f} s + numpy get_M from it on one thread {dtm:.2f} s, "
+                          f"whole matrix"}
+            del tab
     except Exception as e:  # reported, never fatal
-        log(f"3j cpu baseline failed: {e!r}")
+        log(f"3j / mode-coupling cpu baseline failed: {e!r}")
     # lognormal mocks at the reference's defaults (LogNormalMocks(Npnt, nside=2048,
     # lmax=1000)): the ring synthesis kernel alone, then a whole catalogue
     from directsht_amd import mocks
