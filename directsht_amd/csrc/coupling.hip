@@ -118,12 +118,20 @@ __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
 // wavefronts of a block split the sum (k = w, w + 4, ...) and their partial
 // sums are added in wavefront order.  Tiles are taken with the longest sums
 // (largest C0) first.
+// kLds (Nl <= kMcLdsNl): the block's range of G and its window of
+// (2 l3 + 1) W(l3) are staged in LDS first, and the per-row gathers read LDS
+// instead of the L1 cache.
+constexpr int kMcLdsNl = 2048;
+
+template <bool kLds>
 __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __restrict__ W,
                                                        int64_t nW,
                                                        const double* __restrict__ G,
                                                        const double* __restrict__ H,
                                                        double* __restrict__ M) {
   __shared__ double part[4][kMcRows][kMcCols];
+  __shared__ double sG[kLds ? kMcLdsNl : 1];
+  __shared__ double sW[kLds ? 2 * kMcLdsNl + 2 * kMcCols + 2 * kMcRows : 1];
   const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
   // block -> (tc, tr): tc from TC-1 down, tr from the first row tile that
   // reaches the diagonal (R0 + kMcRows - 1 >= C0) to TR-1
@@ -151,18 +159,29 @@ __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __r
   }
   // largest l2 summed here; nothing to sum for an empty window
   const int kend = nl3 > 0 ? min(C0 + kMcCols - 1, R0 + kMcRows - 1) : -1;
+  // window of l3 = l1 - l2 + 2k that this tile can reach: [wlo, whi)
+  const int wlo = max(0, R0 - C0 - (kMcCols - 1));
+  const int whi = min(nl3, R0 + C0 + kMcRows + kMcCols);
+  if constexpr (kLds) {
+    const int gcount = min(Nl, max(C0 + kMcCols, R0 + kMcRows));  // G indices used
+    for (int i = threadIdx.x; i < gcount; i += blockDim.x) sG[i] = G[i];
+    for (int i = threadIdx.x; i < whi - wlo; i += blockDim.x)
+      sW[i] = (double)(2 * (wlo + i) + 1) * W[wlo + i];
+    __syncthreads();
+  }
   // Branch-free body: masked terms load from clamped in-bounds indices and add
   // zero, so the loads of all rows are in flight together.
   for (int k = wv; k <= kend; k += 4) {
     const bool kin = k <= l2 && l2 < Nl;
-    const double q = G[k] * G[kin ? l2 - k : 0];
+    const double q = kLds ? sG[k] * sG[kin ? l2 - k : 0] : G[k] * G[kin ? l2 - k : 0];
 #pragma unroll
     for (int r = 0; r < kMcRows; ++r) {
       const int l3 = d[r] + 2 * k;
       const bool v = on[r] && kin && l3 < nl3;
-      const double w = (double)(2 * l3 + 1) * W[v ? l3 : 0];
+      const double w = kLds ? sW[v ? l3 - wlo : 0] : (double)(2 * l3 + 1) * W[v ? l3 : 0];
       const double h = H[l1[r] < Nl ? l1[r] + k : 0];  // uniform: a scalar load
-      const double tj2 = q * G[v ? d[r] + k : 0] * h;
+      const double g = kLds ? sG[v ? d[r] + k : 0] : G[v ? d[r] + k : 0];
+      const double tj2 = q * g * h;
       acc[r] += v ? w * tj2 : 0.0;
     }
   }
@@ -197,7 +216,10 @@ int launch_mode_coupling(int Nl, const double* W, int64_t nW, double* M, hipStre
   const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
   int nb = 0;
   for (int tc = 0; tc < TC; ++tc) nb += TR - (tc * kMcCols) / kMcRows;
-  k_mode_coupling<<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
+  if (Nl <= kMcLdsNl)
+    k_mode_coupling<true><<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
+  else
+    k_mode_coupling<false><<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

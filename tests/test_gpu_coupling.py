@@ -147,3 +147,25 @@ def test_cabi_argument_errors(torch_cuda):
     assert lib.dsht_mode_coupling(4, None, 0, M.data_ptr(), st) == 0
     torch.cuda.synchronize()
     assert (M == 0).all()
+
+
+def test_mode_coupling_beyond_lds_range(torch_cuda):
+    """Nl above 2048 takes the kernel variant that gathers from global memory
+    instead of LDS: the same identities, and entries against the recursion."""
+    from directsht_amd.mask_deconvolution import mode_coupling_device
+    Nl = 2500
+    I = mode_coupling_device(Nl, np.array([4 * np.pi])).cpu().numpy()
+    np.testing.assert_allclose(I, np.eye(Nl), atol=1e-12)
+    c = 0.3
+    M = mode_coupling_device(Nl, np.full(2 * Nl - 1, c)).cpu().numpy()
+    np.testing.assert_allclose(M, np.broadcast_to((2 * np.arange(Nl) + 1) * c / (4 * np.pi), M.shape),
+                               rtol=1e-12)
+    rng = np.random.default_rng(3)
+    W = rng.uniform(0, 1, 700)
+    M = mode_coupling_device(Nl, W).cpu().numpy()
+    for l1, l2 in [(2499, 2), (300, 250), (1800, 1600)]:
+        s = 0.0
+        for l3 in range(abs(l1 - l2), min(l1 + l2, 699) + 1, 2):
+            s += (2 * l3 + 1) * W[l3] * coupling.threej000(l1, l2, l3, prefill=True) ** 2
+        np.testing.assert_allclose(M[l1, l2], s * (2 * l2 + 1) / (4 * np.pi), rtol=1e-11,
+                                   atol=1e-14 * np.abs(M).max())
