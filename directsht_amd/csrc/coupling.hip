@@ -74,7 +74,8 @@ __host__ __device__ inline int64_t tetra(int64_t j) { return j * (j + 1) * (j + 
 // store[tetra(j1) + j2(j2+1)/2 + j3] = (j1 j2 j3; 0 0 0), N > j1 >= j2 >= j3.
 // The flat index space is cut into chunks of kTjChunk entries; a block walks
 // chunks with a grid stride, each thread 16 entries 256 apart (coalesced
-// stores), carrying j1 forward from the chunk's first entry.
+// stores): the first entry's (j1, j2, j3) from the chunk's j1 and a square
+// root, the next ones by stepping j3 by 256 with carries into j2 and j1.
 __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
                                                          const double* __restrict__ G,
                                                          const double* __restrict__ H,
@@ -87,20 +88,32 @@ __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
     while (j1 > 0 && tetra(j1) > c0) --j1;
     while (tetra(j1 + 1) <= c0) ++j1;
     int64_t rs = tetra(j1), re = tetra(j1 + 1);
-#pragma unroll 4
-    for (int r = 0; r < kTjChunk / 256; ++r) {
-      const int64_t i = c0 + threadIdx.x + 256 * r;
-      if (i >= total) break;
-      while (i >= re) {
-        ++j1;
-        rs = re;
-        re += (j1 + 1) * (j1 + 2) / 2;
+    // this thread's first entry: (j1, j2, j3) from its offset in row j1; the
+    // next entries, 256 further on, by stepping j3 and carrying into j2, j1
+    int64_t i = c0 + threadIdx.x;
+    if (i >= total) continue;
+    while (i >= re) {
+      ++j1;
+      rs = re;
+      re += (j1 + 1) * (j1 + 2) / 2;
+    }
+    int t0 = (int)(i - rs);  // < N(N+1)/2 < 2^24
+    int j2 = (int)((sqrtf(8.0f * (float)t0 + 1.0f) - 1.0f) * 0.5f);
+    while (j2 > 0 && j2 * (j2 + 1) / 2 > t0) --j2;
+    while ((j2 + 1) * (j2 + 2) / 2 <= t0) ++j2;
+    int j3 = t0 - j2 * (j2 + 1) / 2;
+    for (int r = 0; r < kTjChunk / 256; ++r, i += 256) {
+      if (r > 0) {
+        if (i >= total) break;
+        j3 += 256;
+        while (j3 > j2) {  // carry: row j2 holds j3 = 0 .. j2
+          j3 -= j2 + 1;
+          if (++j2 > j1) {  // triangle j1 holds j2 = 0 .. j1
+            j2 = 0;
+            ++j1;
+          }
+        }
       }
-      const int t = (int)(i - rs);  // < N(N+1)/2 < 2^24
-      int j2 = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-      while (j2 > 0 && j2 * (j2 + 1) / 2 > t) --j2;
-      while ((j2 + 1) * (j2 + 2) / 2 <= t) ++j2;
-      const int j3 = t - j2 * (j2 + 1) / 2;
       const int J = (int)j1 + j2 + j3;
       double v = 0.0;
       if (!(J & 1) && j3 >= (int)j1 - j2) {

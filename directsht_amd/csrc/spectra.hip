@@ -91,15 +91,15 @@ __global__ __launch_bounds__(256) void k_alm2cl_auto(const double2* __restrict__
   const double2* a = alm + (int64_t)s * ld;
   double acc = 0.0;
   const int mtop = ell < Nl ? min(ell, mend) : -1;  // this lane's last m
-  for (int m0 = w; m0 <= mend; m0 += 16) {           // four loads in flight
-    double2 x[4];
+  for (int m0 = w; m0 <= mend; m0 += 32) {           // eight loads in flight
+    double2 x[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int m = m0 + 4 * u;
       x[u] = m <= mtop ? a[hp_index(ell, m, Nl)] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int m = m0 + 4 * u;
       acc = m == 0 ? fma(x[u].x, x[u].x, acc)
                    : fma(2.0 * x[u].x, x[u].x, fma(2.0 * x[u].y, x[u].y, acc));
