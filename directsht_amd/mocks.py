@@ -230,7 +230,13 @@ class LogNormalMocks:
     def __call__(self, seed=None, verbose=True):
         """(theta, phi, wt) of the catalogue inside theta_range x phi_range
         (sht/mocks.py:43-53); numpy arrays."""
-        self.seed = seed
+        dev = self.device if self.device is not None else \
+            torch.device("cuda", torch.cuda.current_device())
+        self.gen = torch.Generator(device=dev)   # the reference's self.rng
+        if seed is None:
+            self.gen.seed()
+        else:
+            self.gen.manual_seed(int(seed))
         t, p, w = self.lognormal_catalog(verbose)
         keep = self.make_mask(t, p)
         return t[keep], p[keep], w[keep]
@@ -242,17 +248,16 @@ class LogNormalMocks:
 
     def lognormal_catalog(self, verbose=False):
         """(theta, phi, wt = 1) of Poisson(Npnt) points with intensity
-        proportional to exp(g) (sht/mocks.py:55-88).  The intensity is
+        proportional to exp(g) (sht/mocks.py:55-88), drawn from the generator
+        that __call__ seeds (successive calls continue its stream).  The intensity is
         constant over each pixel (value at its centre) and the points are
         uniform in its area, so any number of points may share a pixel."""
         dev = self.device if self.device is not None else \
             torch.device("cuda", torch.cuda.current_device())
-        gen = torch.Generator(device=dev)
-        seed = getattr(self, "seed", None)
-        if seed is None:
+        gen = getattr(self, "gen", None)
+        if gen is None:  # called before __call__: a fresh, randomly seeded stream
+            gen = self.gen = torch.Generator(device=dev)
             gen.seed()
-        else:
-            gen.manual_seed(int(seed))
         Nl = self.clg.size
         nth, nph = self._grid()
         edges = torch.linspace(0.0, math.pi, nth + 1, dtype=torch.float64, device=dev)

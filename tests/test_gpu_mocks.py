@@ -86,6 +86,9 @@ def test_lognormal_catalogue(torch_cuda):
     np.testing.assert_array_equal(p, p2)
     t3, _, _ = m(seed=8, verbose=False)
     assert t3.size != t.size or not np.array_equal(t3, t)
+    m(seed=7, verbose=False)
+    t4, _, _ = m.lognormal_catalog(False)   # continues the seeded stream
+    assert t4.size != t.size or not np.array_equal(t4, t)
     assert abs(t.size - Npnt) < 6 * np.sqrt(Npnt)
     assert (w == 1).all() and (t > 0).all() and (t < np.pi).all()
     assert (p > 0).all() and (p < 2 * np.pi).all()
