@@ -637,9 +637,10 @@ int dsht_mode_coupling(int Nl, const double* W, int64_t nW, double* M, void* str
 
 int dsht_synthesis_rings(int Nl, const double* alm, int nrings, const double* x, double* F,
                          void* stream) {
-  if (Nl < 1 || Nl > synth_max_nl() || nrings < 0 || (nrings > 0 && (!alm || !x || !F))) {
+  if (Nl < 1 || Nl > synth_max_nl() || nrings < 0 || nrings > synth_max_rings() ||
+      (nrings > 0 && (!alm || !x || !F))) {
     set_error("dsht_synthesis_rings: need 1 <= Nl <= " + std::to_string(synth_max_nl()) +
-              ", nrings >= 0 and non-null arrays");
+              ", 0 <= nrings <= " + std::to_string(synth_max_rings()) + " and non-null arrays");
     return DSHT_EINVAL;
   }
   if (nrings == 0) return DSHT_OK;

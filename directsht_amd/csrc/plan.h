@@ -216,6 +216,7 @@ int tj_max_n();
 int launch_synth_rings(int Nl, const double* alm, int nrings, const double* x, double* F,
                        hipStream_t s);
 int synth_max_nl();
+int synth_max_rings();
 // node ranges of the plain contraction (work items per row group)
 #ifndef DSHT_CONTRACT_SPLITS
 #define DSHT_CONTRACT_SPLITS 2

@@ -97,4 +97,7 @@ int launch_synth_rings(int Nl, const double* alm, int nrings, const double* x, d
 
 int synth_max_nl() { return kSynthMaxNl; }
 
+// grid.y = ceil(nrings / (256 kSynthR)) must stay within 65535
+int synth_max_rings() { return 65535 * 256 * kSynthR; }
+
 }  // namespace dsht

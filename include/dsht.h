@@ -240,7 +240,7 @@ int dsht_mode_coupling(int Nl, const double* W, int64_t nW, double* M, void* str
  * (nrings doubles in [-1, 1]) and Ybar the fully normalized associated
  * Legendre function with the Condon-Shortley phase; a real field on ring r
  * is then Re F[r][0] + 2 Re sum_{m>0} F[r][m] e^{i m phi}.  Enqueued on
- * `stream`.  1 <= Nl <= 2048, nrings >= 0.
+ * `stream`.  1 <= Nl <= 2048, 0 <= nrings <= 67107840.
  * Stands in for healpy synfast in the reference's lognormal mocks
  * (sht/mocks.py:60); no counterpart in csht. */
 int dsht_synthesis_rings(int Nl, const double* alm, int nrings, const double* x, double* F,

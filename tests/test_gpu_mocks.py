@@ -121,7 +121,8 @@ def test_synthesis_errors(torch_cuda):
     for args in [(0, buf.data_ptr(), 1, buf.data_ptr(), buf.data_ptr()),
                  (2049, buf.data_ptr(), 1, buf.data_ptr(), buf.data_ptr()),
                  (4, buf.data_ptr(), -1, buf.data_ptr(), buf.data_ptr()),
-                 (4, None, 1, buf.data_ptr(), buf.data_ptr())]:
+                 (4, None, 1, buf.data_ptr(), buf.data_ptr()),
+                 (4, buf.data_ptr(), 65535 * 1024 + 1, buf.data_ptr(), buf.data_ptr())]:
         assert lib.dsht_synthesis_rings(*args, st) != 0
         assert b"dsht_synthesis_rings" in lib.dsht_last_error()
     assert lib.dsht_synthesis_rings(4, None, 0, None, None, st) == 0
