@@ -1,7 +1,7 @@
 """GPU parity of the HIP path (libdsht.so via DirectSHT) against the FP64
 oracle (oracle.refalg), the exact direct sum and the csht golden fixtures.
 
-Tolerances (north star): FP64 mode within 1e-11 and FP32 mode within 1e-6 of
+Tolerances: FP64 mode within 1e-11 and FP32 mode within 1e-6 of
 the FP64 restatement of the reference algorithm, measured as
 max|a - a_ref| / max|a_ref| (and the rms analogue).
 """

@@ -1,6 +1,6 @@
 """Shared-table clones, graph-captured batches and the transform server.
 
-The bar (north star): every result of a batched or served transform equals
+The bar: every result of a batched or served transform equals
 the same transform run alone, within the precision's tolerance (FP32 1e-6,
 FP64 1e-11 relative to max|a|), and matches the FP64 oracle.  Graph replays
 are checked one by one against the eager result, with the program's own
