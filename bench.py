@@ -84,10 +84,18 @@ def cpu_baseline(theta, phi, w, Nl, Nx, xmax=0.75):
     dt = time.perf_counter() - t0
     Nlm = Nl * (Nl + 1) // 2
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": n * Nlm / dt,
         "unit": "points*modes/s",
         "cores": cores,
+        "nproc": os.cpu_count(),
+        "cpu_model": model,
         "kind": "reference",
         "sample": (f"{n} points of the same workload (Nl={Nl}, Nx={Nx}); csht/sht_helper.c "
                    f"do_transform via its csht/sht.py driver, both branches, tables prebuilt, "
