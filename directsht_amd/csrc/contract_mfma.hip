@@ -71,8 +71,9 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
   auto load = [&](int c0) {
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
-      const int e = tid + kCThreads * v;
-      const int np = c0 + (e >> 6), r = e & 63;
+      // node pairs 2 (tid / 64) + v of the chunk: the two halves of one
+      // A-operand element, so the staging writes whole 16-byte elements
+      const int np = c0 + 2 * (tid >> 6) + v, r = tid & 63;
       if (np < Npair && r < gd.nrows) {  // read once: non-temporal
         typedef float v4 __attribute__((ext_vector_type(4)));
         const v4 t4 = __builtin_nontemporal_load(
@@ -94,21 +95,28 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
   int buf = 0;
   for (int c0 = 0; c0 < Npair; c0 += kCK, buf ^= 1) {
     // stage this chunk's table block (split into hi/lo) into LDS buffer `buf`
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      const int e = tid + kCThreads * v;
-      const int u = e >> 6, r = e & 63;
-      float4 y = ya[v];
-      y.z *= dx;
-      y.w *= dx;
-      uint32_t h01, l01, h23, l23;
-      split2_f16(y.x, y.y, h01, l01);
-      split2_f16(y.z, y.w, h23, l23);
-      // K index within the step: 4 (u % 4) + q -> lane half (u % 4) / 2, slot 4 ((u % 4) % 2) + q
-      const int t = u >> 2, hh = (u >> 1) & 1, half = u & 1;
+    {
+      // node pairs u = 2 (tid / 64) + half, half = 0, 1; K index within the
+      // step: 4 (u % 4) + q -> lane half (u % 4) / 2, slot 4 ((u % 4) % 2) + q
+      const int w = tid >> 6, r = tid & 63;
+      const int t = w >> 1, hh = w & 1;
       const int rt = r >> 5, lane = hh * 32 + (r & 31);
-      reinterpret_cast<u32x2*>(&sA[buf][t][rt][0][lane])[half] = (u32x2){h01, h23};
-      reinterpret_cast<u32x2*>(&sA[buf][t][rt][1][lane])[half] = (u32x2){l01, l23};
+      u32x4 hv, lv;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        float4 y = ya[half];
+        y.z *= dx;
+        y.w *= dx;
+        uint32_t h01, l01, h23, l23;
+        split2_f16(y.x, y.y, h01, l01);
+        split2_f16(y.z, y.w, h23, l23);
+        hv[2 * half] = h01;
+        hv[2 * half + 1] = h23;
+        lv[2 * half] = l01;
+        lv[2 * half + 1] = l23;
+      }
+      sA[buf][t][rt][0][lane] = hv;
+      sA[buf][t][rt][1][lane] = lv;
     }
     // B of this chunk, scaled; then the next chunk's loads go in flight
     float x[4][8];
