@@ -65,9 +65,11 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
     sincos(phi, &sp, &cp);
     const double X = ct, Y = st * sp, Z = -(st * cp);
     const double rho = sqrt(X * X + Y * Y);
-    const double tr = atan2(rho, Z);
-    if (!(tr > p.acos_xmax && tr < p.acos_mxmax)) c.flag |= DSHT_STATUS_POLAR_RANGE;
-    xu = cos(tr);
+    // cos of the rotated theta, Z / |n| (|n| = 1 up to rounding); the range
+    // check theta' in (acos xmax, acos(-xmax)) of basic_sht is the same as
+    // -xmax < cos theta' < xmax
+    xu = Z / sqrt(rho * rho + Z * Z);
+    if (!(xu < p.xmax && xu > -p.xmax)) c.flag |= DSHT_STATUS_POLAR_RANGE;
     if (rho > 0.0) {
       c.ere = X / rho;
       c.eim = -(Y / rho);
