@@ -15,7 +15,9 @@ This is a different recurrence from the reference's table build
 The result is the "method-error-free" target: the reference algorithm differs
 from it by its cubic-Hermite spline error.
 
-Pinned against scipy.special.sph_harm_y (tests/test_oracle_tables.py).
+Pinned against scipy.special.sph_harm_y (tests/test_oracle_tables.py);
+synth_rings also by the exact Gauss-Legendre quadrature round trip
+(tests/test_mocks_host.py).
 """
 import numpy as np
 

@@ -56,3 +56,27 @@ def test_mask():
     t = np.array([0.4, 0.6, 0.6, 0.9, 1.0])
     p = np.array([1.5, 1.5, 2.5, 1.1, 1.5])
     np.testing.assert_array_equal(m.make_mask(t, p), [1, 3])
+
+
+def test_synth_rings_oracle_quadrature_round_trip():
+    """oracle.exact.synth_rings: the map it synthesizes on Nl Gauss-Legendre
+    rings and 2Nl - 1 longitudes returns its alm under exact quadrature."""
+    from oracle import exact
+    Nl = 24
+    nphi = 2 * Nl - 1
+    rng = np.random.default_rng(2)
+    Nlm = Nl * (Nl + 1) // 2
+    a = rng.normal(size=Nlm) + 1j * rng.normal(size=Nlm)
+    a[:Nl] = a[:Nl].real
+    xr, wr = leggauss(Nl)
+    F = exact.synth_rings(a, Nl, xr)
+    phi = 2 * np.pi * np.arange(nphi) / nphi
+    m = np.arange(Nl)
+    f = (F[:, :1].real + 2 * (F[:, 1:, None] * np.exp(1j * m[1:, None] * phi)).real.sum(axis=1))
+    fm = np.fft.fft(f, axis=1)[:, :Nl] * (2 * np.pi / nphi)
+    Y = exact.ylm_values(Nl, np.arccos(xr))
+    back = np.zeros_like(a)
+    for mm in range(Nl):
+        i0 = exact._indx(mm, mm, Nl)
+        back[i0:i0 + Nl - mm] = Y[i0:i0 + Nl - mm] @ (wr * fm[:, mm])
+    np.testing.assert_allclose(back, a, rtol=0, atol=1e-12 * np.abs(a).max())
