@@ -7,7 +7,8 @@
 //   C^{st}_l = [ Re a^s_l0 Re a^t_l0 + 2 sum_{m>0} Re(a^s_lm conj(a^t_lm)) ] / (2l + 1).
 // C^{ss} is the auto-spectrum of set s.
 //
-// Cross-spectra: one workgroup per l (the longest rows first).  The row's a_lm
+// Cross-spectra: one workgroup per l (the longest rows first, four
+// consecutive l's per XCD).  The row's a_lm
 // of all sets are staged through LDS kSpecM orders at a time ([m][set]); each
 // thread accumulates its pairs (s, t) in FP64 in order of m.  Auto-spectra
 // alone take a coalesced kernel (k_alm2cl_auto).
@@ -23,7 +24,13 @@ __global__ __launch_bounds__(256) void k_alm2cl(const double2* __restrict__ alm,
                                                 int Nl, int nsets, int cross,
                                                 double* __restrict__ cl) {
   __shared__ double2 sa[kSpecM][kMaxSets];
-  const int ell = Nl - 1 - (int)blockIdx.x;
+  // Four consecutive l's on one XCD (block b runs on XCD b % 8): their a_lm
+  // for a fixed m are adjacent 16-byte values of one healpy row, so each line
+  // is read from HBM once and shared through that XCD's L2.  Groups of four
+  // are interleaved over the XCDs, longest rows first.
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int ell = Nl - 1 - (((slot >> 2) * 8 + xcd) * 4 + (slot & 3));
+  if (ell < 0) return;
   const int npairs = cross ? nsets * (nsets + 1) / 2 : nsets;
   // this thread's pairs p = tid + 256 q: (s, t), s <= t, in row order
   int ps[kSpecPairs], pt[kSpecPairs];
@@ -121,7 +128,8 @@ int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, 
     DSHT_LAUNCH_CHECK();
     return DSHT_OK;
   }
-  k_alm2cl<<<Nl, 256, 0, s>>>(reinterpret_cast<const double2*>(alm), ld, Nl, nsets, cross, cl);
+  k_alm2cl<<<(Nl + 31) / 32 * 32, 256, 0, s>>>(reinterpret_cast<const double2*>(alm), ld, Nl, nsets,
+                                               cross, cl);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
