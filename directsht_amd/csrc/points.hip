@@ -147,34 +147,62 @@ __global__ __launch_bounds__(256) void k_hist(int64_t Np, const double* __restri
   }
 }
 
-// Exclusive scan of n ints by one workgroup of 1024 threads:
-// out[i] = sum_{k<i} v(in[k]), out[n] = total, with v(x) = x, or v(x) = (x > 0)
-// when `indicator` is set; out2 (if given) gets a copy.
-__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n, int indicator,
-                                               int* __restrict__ out, int* __restrict__ out2) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int b = tid * per, e = min(n, b + per);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += indicator ? (in[i] > 0) : in[i];
-  part[tid] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
+// Exclusive scans of n ints by one workgroup of 1024 threads: out[i] =
+// sum_{k<i} in[k], out[n] = total (out2, if given, gets a copy), and, if
+// rank is given, rank[i] = #{k < i : in[k] > 0} for i = 0 .. nrank
+// (nrank <= n: the count over the first nrank entries).
+// Each thread sums a contiguous slice; the slice totals are scanned within
+// each wavefront by shuffles and across the 16 wavefronts in LDS.
+__device__ inline int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off);
+    if (lane >= off) v += t;
   }
-  int run = tid ? part[tid - 1] : 0;
+  return v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n,
+                                               int* __restrict__ out, int* __restrict__ out2,
+                                               int* __restrict__ rank, int nrank) {
+  __shared__ int wsum[16], wcnt[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int per = (n + 1023) / 1024;
+  const int b = min(n, tid * per), e = min(n, b + per);
+  int s = 0, c = 0;
   for (int i = b; i < e; ++i) {
+    const int v = in[i];
+    s += v;
+    c += v > 0;
+  }
+  const int si = wave_incl_scan(s, lane), ci = wave_incl_scan(c, lane);
+  if (lane == 63) {
+    wsum[w] = si;
+    wcnt[w] = ci;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int a = lane < 16 ? wsum[lane] : 0, k = lane < 16 ? wcnt[lane] : 0;
+    const int as = wave_incl_scan(a, lane), ks = wave_incl_scan(k, lane);
+    if (lane < 16) {
+      wsum[lane] = as - a;  // exclusive prefix of the wavefront totals
+      wcnt[lane] = ks - k;
+    }
+  }
+  __syncthreads();
+  int run = wsum[w] + si - s, cnt = wcnt[w] + ci - c;  // exclusive prefix of this slice
+  for (int i = b; i < e; ++i) {
+    const int v = in[i];
     out[i] = run;
     if (out2) out2[i] = run;
-    run += indicator ? (in[i] > 0) : in[i];
+    if (rank && i <= nrank) rank[i] = cnt;
+    run += v;
+    cnt += v > 0;
   }
   if (tid == 1023) {
-    out[n] = part[1023];
-    if (out2) out2[n] = part[1023];
+    out[n] = run;
+    if (out2) out2[n] = run;
+    if (rank && nrank == n) rank[n] = cnt;
   }
 }
 
@@ -419,9 +447,10 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
     }
     DSHT_LAUNCH_CHECK();
   }
-  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg + 1, 0, ws.seg_start, ws.seg_cursor);
-  DSHT_LAUNCH_CHECK();
-  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg, 1, ws.ne_rank, nullptr);
+  // segment starts (and cursors) over nseg + 1 buckets (the last, dropped
+  // points), and the rank of each non-empty segment among the first nseg
+  k_scan<<<1, 1024, 0, s>>>(ws.seg_count, pl.nseg + 1, ws.seg_start, ws.seg_cursor, ws.ne_rank,
+                            pl.nseg);
   DSHT_LAUNCH_CHECK();
   if (Np > 0) {
     if (tiled) {
@@ -439,7 +468,7 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
                                                       nchunks, ws.run_count);
     DSHT_LAUNCH_CHECK();
   }
-  k_scan<<<1, 1024, 0, s>>>(ws.run_count, nchunks, 0, ws.run_base, nullptr);
+  k_scan<<<1, 1024, 0, s>>>(ws.run_count, nchunks, ws.run_base, nullptr, nullptr, 0);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
