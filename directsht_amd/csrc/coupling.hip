@@ -17,7 +17,7 @@
 // The sum S(l1, l2) is symmetric, so only l2 <= l1 is summed; with
 // d = l1 - l2 and l3 = d + 2k (k = 0 .. l2), g = l1 + k and
 //     (l1 l2 l3; 0 0 0)^2 = G(k) G(d + k) G(l2 - k) H(l1 + k).
-// A block takes 64 consecutive l2 (one per lane) and four rows l1, its four
+// A block takes 64 consecutive l2 (one per lane) and 4 or 8 rows l1, its four
 // wavefronts a quarter of the k's each; G(k) G(l2 - k) is shared by the rows,
 // H(l1 + k) is uniform per row.  Each wavefront sums its terms in
 // ascending l3, as the reference does.
@@ -33,7 +33,8 @@ constexpr int kTjMaxN = 4096;      // largest table size N and mode-coupling Nl
 constexpr int kTjG = 4096;         // G(p), p < kTjG
 constexpr int kTjH = 8192;         // H(g), g < kTjH  (g <= 3(N-1)/2 and <= 2Nl-2)
 constexpr int kTjChunk = 4096;     // table entries per work chunk (16 per thread)
-constexpr int kMcRows = 4;         // rows l1 per mode-coupling tile
+constexpr int kMcRowsMax = 8;      // rows l1 per mode-coupling tile: 4, or 8 from kMcRows8Nl
+constexpr int kMcRows8Nl = 1536;   // (more independent sums per thread, fewer blocks)
 constexpr int kMcCols = 64;        // columns l2 per tile (one per lane)
 
 namespace {
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
   }
 }
 
-// One tile: rows l1 in [R0, R0 + 4), columns l2 in [C0, C0 + 64); entries
+// One tile: rows l1 in [R0, R0 + kMcRows), columns l2 in [C0, C0 + 64); entries
 // with l2 <= l1 are summed and written to M[l1][l2] and M[l2][l1].  The four
 // wavefronts of a block split the sum (k = w, w + 4, ...) and their partial
 // sums are added in wavefront order.  Tiles are taken with the longest sums
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(256) void k_threej000_table(int N, int64_t total,
 // instead of the L1 cache.
 constexpr int kMcLdsNl = 2048;
 
-template <bool kLds>
+template <bool kLds, int kMcRows>
 __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __restrict__ W,
                                                        int64_t nW,
                                                        const double* __restrict__ G,
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __r
                                                        double* __restrict__ M) {
   __shared__ double part[4][kMcRows][kMcCols];
   __shared__ double sG[kLds ? kMcLdsNl : 1];
-  __shared__ double sW[kLds ? 2 * kMcLdsNl + 2 * kMcCols + 2 * kMcRows : 1];
+  __shared__ double sW[kLds ? 2 * kMcLdsNl + 2 * kMcCols + 2 * kMcRowsMax : 1];
   const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
   // block -> (tc, tr): tc from TC-1 down, tr from the first row tile that
   // reaches the diagonal (R0 + kMcRows - 1 >= C0) to TR-1
@@ -201,13 +202,14 @@ __global__ __launch_bounds__(256) void k_mode_coupling(int Nl, const double* __r
 #pragma unroll
   for (int r = 0; r < kMcRows; ++r) part[wv][r][lane] = acc[r];
   __syncthreads();
-  if (wv >= kMcRows) return;
-  const int r = wv;  // wavefront r writes row R0 + r
-  if (!on[r]) return;
-  const double S = ((part[0][r][lane] + part[1][r][lane]) + part[2][r][lane]) + part[3][r][lane];
   const double four_pi = 4.0 * M_PI;
-  M[(int64_t)l1[r] * Nl + l2] = (S * (double)(2 * l2 + 1)) / four_pi;
-  if (d[r] > 0) M[(int64_t)l2 * Nl + l1[r]] = (S * (double)(2 * l1[r] + 1)) / four_pi;
+#pragma unroll
+  for (int r = 0; r < kMcRows; ++r) {  // wavefront w writes rows R0 + w, R0 + w + 4, ...
+    if ((r & 3) != wv || !on[r]) continue;
+    const double S = ((part[0][r][lane] + part[1][r][lane]) + part[2][r][lane]) + part[3][r][lane];
+    M[(int64_t)l1[r] * Nl + l2] = (S * (double)(2 * l2 + 1)) / four_pi;
+    if (d[r] > 0) M[(int64_t)l2 * Nl + l1[r]] = (S * (double)(2 * l1[r] + 1)) / four_pi;
+  }
 }
 
 int launch_threej000_table(int N, double* store, hipStream_t s) {
@@ -222,19 +224,26 @@ int launch_threej000_table(int N, double* store, hipStream_t s) {
   return DSHT_OK;
 }
 
+template <int kRows>
+static int launch_mode_coupling_rows(int Nl, const double* W, int64_t nW, const double* coef,
+                                     double* M, hipStream_t s) {
+  const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kRows - 1) / kRows;
+  int nb = 0;
+  for (int tc = 0; tc < TC; ++tc) nb += TR - (tc * kMcCols) / kRows;
+  if (Nl <= kMcLdsNl)
+    k_mode_coupling<true, kRows><<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
+  else
+    k_mode_coupling<false, kRows><<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
 int launch_mode_coupling(int Nl, const double* W, int64_t nW, double* M, hipStream_t s) {
   const double* coef = nullptr;
   int rc = tj_coefficients(&coef);
   if (rc != DSHT_OK) return rc;
-  const int TC = (Nl + kMcCols - 1) / kMcCols, TR = (Nl + kMcRows - 1) / kMcRows;
-  int nb = 0;
-  for (int tc = 0; tc < TC; ++tc) nb += TR - (tc * kMcCols) / kMcRows;
-  if (Nl <= kMcLdsNl)
-    k_mode_coupling<true><<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
-  else
-    k_mode_coupling<false><<<nb, 256, 0, s>>>(Nl, W, nW, coef, coef + kTjG, M);
-  DSHT_LAUNCH_CHECK();
-  return DSHT_OK;
+  return Nl >= kMcRows8Nl ? launch_mode_coupling_rows<8>(Nl, W, nW, coef, M, s)
+                          : launch_mode_coupling_rows<4>(Nl, W, nW, coef, M, s);
 }
 
 int tj_max_n() { return kTjMaxN; }
