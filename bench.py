@@ -61,46 +61,121 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(theta, phi, w, Nl, Nx, xmax=0.75):
-    """The reference's csht do_transform (OpenMP C) on a sample of the
-    workload's points: equatorial points directly, polar points after the
-    90-degree rotation; tables are built before timing.  The O(Nl^3) alm
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
+def cpu_baseline(theta, phi, w, Nl, Nx, N_full, sizes=(20000, 100000), reps=3, xmax=0.75):
+    """The reference's csht do_transform (OpenMP C, compiled into oracle/_ref)
+    on samples of the workload's points: equatorial points directly, polar
+    points after the 90-degree rotation; tables built before timing.  Each
+    sample size runs `reps` times (median); the time per transform is fitted
+    as t = t_fixed + n * t_point (csht contracts every bin for each call,
+    a fixed O(Nlm Nx) cost, and scatters O(n Nl)), and the full workload's
+    rate is N_full * Nlm / (t_fixed + N_full * t_point).  The O(Nl^3) alm
     rotation (healpy in the reference) is not included."""
     import numpy as np
 
     from oracle import cref, refalg
     if not cref.available():
         return None
-    n = len(theta)
     log(f"cpu baseline: csht make_table Nl={Nl} Nx={Nx} (not timed)")
     c = cref.CRef(Nl, Nx, xmax)
-    x = np.abs(np.cos(theta))
-    pol, equ = x >= xmax, x < xmax
-    tp, pp = refalg.rotate_points(theta[pol], phi[pol])
-    log(f"cpu baseline: timing csht on {n} points")
-    t0 = time.perf_counter()
-    c(theta[equ], phi[equ], w[equ])
-    c(tp, pp, w[pol])
-    dt = time.perf_counter() - t0
+    med = {}
+    for n in sizes:
+        t, p, ww = theta[:n], phi[:n], w[:n]
+        x = np.abs(np.cos(t))
+        pol, equ = x >= xmax, x < xmax
+        tp, pp = refalg.rotate_points(t[pol], p[pol])
+        runs = []
+        for r in range(reps):
+            log(f"cpu baseline: csht on {n} points, run {r + 1}/{reps}")
+            t0 = time.perf_counter()
+            c(t[equ], p[equ], ww[equ])
+            c(tp, pp, ww[pol])
+            runs.append(time.perf_counter() - t0)
+        med[n] = float(np.median(runs))
+    n0, n1 = sizes
+    t_point = (med[n1] - med[n0]) / (n1 - n0)
+    t_fixed = max(0.0, med[n0] - n0 * t_point)
+    t_full = t_fixed + N_full * t_point
     Nlm = Nl * (Nl + 1) // 2
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
-    except OSError:
-        pass
     return {
-        "value": n * Nlm / dt,
+        "value": N_full * Nlm / t_full,
         "unit": "points*modes/s",
         "cores": cores,
         "nproc": os.cpu_count(),
-        "cpu_model": model,
+        "cpu_model": _cpu_model(),
         "kind": "reference",
-        "sample": (f"{n} points of the same workload (Nl={Nl}, Nx={Nx}); csht/sht_helper.c "
-                   f"do_transform via its csht/sht.py driver, both branches, tables prebuilt, "
-                   f"alm rotation excluded; {dt:.2f} s"),
+        "sample": (f"csht/sht_helper.c do_transform (its csht/sht.py driver, both branches, "
+                   f"tables prebuilt, alm rotation excluded) on the first {n0} and {n1} points "
+                   f"of the same workload (Nl={Nl}, Nx={Nx}), median of {reps} runs each: "
+                   f"{med[n0]:.2f} s and {med[n1]:.2f} s; fitted fixed {t_fixed:.2f} s + "
+                   f"{t_point * 1e6:.1f} us/point, so {t_full:.1f} s for all {N_full} points"),
+        "median_s": {str(k): v for k, v in med.items()},
+        "fixed_s": t_fixed,
+        "per_point_us": t_point * 1e6,
+        "full_workload_s": t_full,
     }
+
+
+def cpu_numpy_c1(reps=3):
+    """The numpy restatement of the reference's numpy path (oracle.refalg,
+    FP64) on the full C1 workload (Nl=256, Nx=512, 1e5 uniform points), tables
+    built before timing; median of reps runs on one process."""
+    import numpy as np
+
+    from oracle import refalg
+    rng = np.random.default_rng(1)
+    n = 100_000
+    t = np.arccos(rng.uniform(-1, 1, n))
+    p = rng.uniform(0, 2 * np.pi, n)
+    w = rng.uniform(0.5, 1.5, n)
+    T = refalg.make_tables(256, 512, 0.75)
+    runs = []
+    for r in range(reps):
+        log(f"cpu baseline: numpy oracle on C1, run {r + 1}/{reps}")
+        t0 = time.perf_counter()
+        refalg.direct_sht(T, t, p, w)
+        runs.append(time.perf_counter() - t0)
+    med = float(np.median(runs))
+    return {"value": n * 32896 / med, "unit": "points*modes/s", "median_s": med, "runs": len(runs),
+            "cores": 1, "kind": "port",
+            "sample": "oracle.refalg.direct_sht (numpy restatement of sht/sht.py:112-283, "
+                      "FP64) on the whole C1 workload: Nl=256, Nx=512, 1e5 uniform points"}
+
+
+def method_error(Nl, Nx, theta, phi, W, n, dflt_type="float32"):
+    """Error of the device transform against the exact FP64 direct sum
+    (oracle.exact, no spline) on the first n points: max |da| / max |a| and
+    rms, worst over the weight rows W (S, N)."""
+    import numpy as np
+    import torch
+
+    from directsht_amd.sht import DirectSHT
+    from oracle import exact
+    t = theta[:n].contiguous()
+    p = phi[:n].contiguous()
+    Wn = W[:, :n].contiguous()
+    ex = exact.exact_alm(Nl, t.cpu().numpy(), p.cpu().numpy(), Wn.cpu().numpy())
+    sht = DirectSHT(Nl, Nx, dflt_type=dflt_type)
+    if Wn.shape[0] == 1:
+        a = sht.transform_device(t, p, Wn[0].contiguous())[0][None, :]
+    else:
+        a = sht.transform_sets_device(t, p, Wn)[0]
+    a = a.cpu().numpy()
+    d = np.abs(a - ex)
+    mx = float(np.max(d.max(axis=1) / np.abs(ex).max(axis=1)))
+    rms = float(np.max(np.linalg.norm(a - ex, axis=1) / np.linalg.norm(ex, axis=1)))
+    del sht
+    torch.cuda.empty_cache()
+    return {"max_rel": mx, "rms_rel": rms, "points": n, "sets": int(Wn.shape[0]),
+            "vs": "exact FP64 direct sum (oracle.exact)", "plan": dflt_type}
 
 
 def _timed(fn, reps, sync):
@@ -122,19 +197,29 @@ def _err(a, ref):
     return {"max_rel": float(mx), "rms_rel": float(rms)}
 
 
-def bench_weight_sets(reps=3):
-    """Many weight sets on shared points: 2e6 uniform points, 64 weight sets
-    (set 0 constant, 63 lognormal-modulated with distinct seeds), Nl=768,
-    Nx=1536, FP32.  Plain = 64 single-set transforms; batched = one
+def bench_weight_sets(reps=3, headline=False, S=64):
+    """Many weight sets on shared points.  Default (the weight-set
+    configuration): 2e6 uniform points, Nl=768, Nx=1536.  headline=True: the
+    1e7 lognormal-cap points of the plain workload at Nl=1000, Nx=2048.  S
+    weight sets: set 0 constant, the others lognormal-modulated with
+    distinct seeds; FP32.  Plain = S single-set transforms; batched = one
     dsht_transform_sets call (binning once, one table stream, fp16x3 MFMA
-    contraction; also with the FP32 GEMV contraction per set)."""
+    contraction; for the default configuration also with the FP32 GEMV
+    contraction per set).  Errors against the FP64 plan per set, and against
+    the exact direct sum on a subsample."""
     import torch
 
     from directsht_amd import mocks
     from directsht_amd.sht import DirectSHT
-    Nl, Nx, N, S = 768, 1536, 2_000_000, 64
-    log(f"weight sets: {N} uniform points, {S} weight sets, Nl={Nl}")
-    th, ph, _ = mocks.uniform_sphere(N, seed=3)
+    if headline:
+        Nl, Nx, N = 1000, 2048, 10_000_000
+        th, ph, _ = mocks.lognormal_cap_points(N, seed=2)
+        what = f"{N} lognormal cap points (the plain workload's)"
+    else:
+        Nl, Nx, N = 768, 1536, 2_000_000
+        th, ph, _ = mocks.uniform_sphere(N, seed=3)
+        what = f"{N} uniform points"
+    log(f"weight sets: {what}, {S} weight sets, Nl={Nl}")
     W = mocks.lognormal_weight_sets(th, ph, S, seed0=300)
     sync = torch.cuda.synchronize
     sht = DirectSHT(Nl, Nx, dflt_type="float32")
@@ -149,24 +234,33 @@ def bench_weight_sets(reps=3):
     t_plain = _timed(plain, reps, sync)
     t_mat = _timed(lambda: sht.transform_sets_device(th, ph, W, out=outS), reps, sync)
     a_mat = outS.clone()
-    sht.configure(contract="vector")
-    t_vec = _timed(lambda: sht.transform_sets_device(th, ph, W, out=outS), reps, sync)
-    a_vec = outS.clone()
-    del sht
+    t_vec = a_vec = None
+    if not headline:
+        sht.configure(contract="vector")
+        t_vec = _timed(lambda: sht.transform_sets_device(th, ph, W, out=outS), reps, sync)
+        a_vec = outS.clone()
+    del sht, outS
     torch.cuda.empty_cache()
     s64 = DirectSHT(Nl, Nx, dflt_type="float64")
     a64, _ = s64.transform_sets_device(th, ph, W)
     sync()
     Nlm = Nl * (Nl + 1) // 2
     pm = N * Nlm * S
-    res = {"workload": f"{N} uniform points x {S} weight sets (1 constant, {S - 1} lognormal "
+    res = {"workload": f"{what} x {S} weight sets (1 constant, {S - 1} lognormal "
                        f"sigma=0.5, seeds 301..), Nl={Nl}, Nx={Nx}, FP32",
            "plain_ms": t_plain * 1e3, "batched_ms": t_mat * 1e3,
-           "batched_vector_contract_ms": t_vec * 1e3,
            "plain_points_modes_per_s": pm / t_plain, "batched_points_modes_per_s": pm / t_mat,
            "speedup": t_plain / t_mat,
-           "error_vs_fp64": {"mfma_fp16x3": _err(a_mat, a64), "fp32_gemv": _err(a_vec, a64)}}
-    del s64
+           "error_vs_fp64": {"mfma_fp16x3": _err(a_mat, a64)}}
+    if a_vec is not None:
+        res["batched_vector_contract_ms"] = t_vec * 1e3
+        res["error_vs_fp64"]["fp32_gemv"] = _err(a_vec, a64)
+    del s64, a64, a_mat, a_vec
+    torch.cuda.empty_cache()
+    if not headline:
+        log("weight sets: error against the exact direct sum (1000 points, 2 sets)")
+        res["method_error_vs_exact"] = method_error(Nl, Nx, th, ph, W[:2], 1000)
+    del W, th, ph
     torch.cuda.empty_cache()
     return res
 
@@ -220,7 +314,13 @@ def bench_uneven(nsets=200, reps=3):
            "plain_points_modes_per_s": pm / t_plain, "batched_points_modes_per_s": pm / t_batch,
            "setup_s": t_setup, "calls": len(bt.calls), "streams": bt.nstreams,
            "max_rel_vs_single": worst}
-    del bt, plans, sets
+    del bt, plans
+    torch.cuda.empty_cache()
+    big = next(i for i in range(nsets) if int(nls[i]) == 1024 and int(sizes[i]) >= 1000)
+    th, ph, w, nl = sets[big]
+    log("uneven: error against the exact direct sum (Nl=1024 set, 500 points)")
+    res["method_error_vs_exact"] = method_error(1024, 2048, th, ph, w[None, :], 500)
+    del sets
     torch.cuda.empty_cache()
     return res
 
@@ -409,14 +509,55 @@ def bench_serving(nreq=2100, warm=100, rate=50.0):
             futs.append(srv.submit(th[a:a + n], ph[a:a + n], w[a:a + n], Nl=int(nls[i])))
             if (i + 1) % 500 == 0:
                 log(f"serving: {i + 1} submitted")
-        for f in futs:
-            f.result(timeout=600)
+        res = {}
+        for i, f in enumerate(futs):
+            a = f.result(timeout=600)
+            if i >= warm and (i - warm) % 100 == 0:
+                res[i] = a
+        del futs
         done = srv.completed[warm:]
         srv.completed = done
         st = srv.stats()
+        # capacity: the next requests of the same stream submitted all at
+        # once (no arrival gaps), aggregate rate from first submit to last
+        # result
+        nsat = min(200, nreq)
+        log(f"serving: {nsat} requests submitted at once (saturated)")
+        srv.completed = []
+        t0 = time.perf_counter()
+        sfut = [srv.submit(th[offs[i]:offs[i] + int(sizes[i])], ph[offs[i]:offs[i] + int(sizes[i])],
+                           w[offs[i]:offs[i] + int(sizes[i])], Nl=int(nls[i])) for i in range(nsat)]
+        for f in sfut:
+            f.result(timeout=600)
+        t_sat = time.perf_counter() - t0
+        pm_sat = sum(int(sizes[i]) * (int(nls[i]) * (int(nls[i]) + 1) // 2) for i in range(nsat))
+        st["saturated"] = {"requests": nsat, "wall_s": t_sat,
+                           "points_modes_per_s": pm_sat / t_sat,
+                           "p50_ms": srv.stats()["p50_ms"], "p99_ms": srv.stats()["p99_ms"]}
     st["workload"] = (f"Poisson {rate}/s, {nreq - warm} timed requests after {warm} warm-up, "
                       f"N log-uniform [1e4, 5e7], Nl in {{256,512,1024,1536}}, Nx=2Nl, FP32, "
                       f"device-resident inputs, alm returned to the host")
+    # served results against FP64 plans on the same points (every 100th
+    # request), and the method error against the exact direct sum
+    from directsht_amd.sht import DirectSHT
+    worst = {"max_rel": 0.0, "rms_rel": 0.0}
+    plans = {}
+    for i in range(warm, nreq, 100):
+        nl, a, n = int(nls[i]), offs[i], int(sizes[i])
+        if nl not in plans:
+            plans[nl] = DirectSHT(nl, 2 * nl, dflt_type="float64")
+        ref = plans[nl].transform_device(th[a:a + n], ph[a:a + n], w[a:a + n])[0].cpu().numpy()
+        got = np.asarray(res[i])
+        worst["max_rel"] = max(worst["max_rel"],
+                               float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))))
+        worst["rms_rel"] = max(worst["rms_rel"],
+                               float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    worst["requests"] = len(range(warm, nreq, 100))
+    st["served_error_vs_fp64"] = worst
+    del plans
+    torch.cuda.empty_cache()
+    log("serving: error against the exact direct sum (Nl=1536, 300 points)")
+    st["method_error_vs_exact"] = method_error(1536, 3072, th, ph, w[None, :], 300)
     del th, ph, w
     torch.cuda.empty_cache()
     return st
@@ -440,7 +581,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2100)
     ap.add_argument("--skip-downstream", action="store_true",
                     help="skip the spectra / mode-coupling / 3j timings")
-    ap.add_argument("--only", choices=("weight_sets", "uneven", "serving", "downstream"),
+    ap.add_argument("--only", choices=("weight_sets", "weight_sets_headline", "uneven", "serving",
+                                       "downstream"),
                     help="run only this workload and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after timing, run a 1 GiB device copy 3 times (the known-size "
@@ -461,6 +603,7 @@ def main():
     torch.cuda.set_device(0)
     if args.only:
         fn = {"weight_sets": bench_weight_sets, "uneven": bench_uneven,
+              "weight_sets_headline": lambda: bench_weight_sets(headline=True, S=64),
               "downstream": bench_downstream,
               "serving": lambda: bench_serving(nreq=args.serve_requests,
                                                warm=min(100, args.serve_requests // 10))}[args.only]
@@ -501,13 +644,17 @@ def main():
     log(f"{ms:.3f} ms/step; stages {stage_avg}")
 
     # Dominant kernel: the matrix-core scatter.  Algorithmic work per
-    # (point, order m): the complex products w h_k(t) e^{-i m phi} for the four
-    # Hermite weights k, as the real GEMM [(k, m1, re|im) x point] .
-    # [point x (re|im, m2)] with m = 1024 mt + 32 m1 + m2: 2*256*64/1024 = 32
-    # flop, done three times (fp16 hi*hi, hi*lo, lo*hi) = 96 MFMA flop.
-    scat_flop = 96.0 * N * Nl
-    scat_tf = scat_flop / (stage_avg["scatter"] * 1e-3) / 1e12
-    valu_tf = 22.0 * N * Nl / (stage_avg["scatter"] * 1e-3) / 1e12
+    # (point, order m): 22 flop -- the four complex multiply-adds
+    # w h_k(t) e^{-i m phi} into the accumulators (16) and the complex step of
+    # e^{-i m phi} (6) -- so 22 N Nl per transform, priced against the dense
+    # fp16 matrix peak the kernel runs on.  The kernel's instruction count is
+    # higher: the real GEMM [(k, m1, re|im) x point] . [point x (re|im, m2)]
+    # with m = 1024 mt + 32 m1 + m2 costs 32 flop per (point, m), three times
+    # for the fp16 hi*hi, hi*lo, lo*hi products = 96 MFMA flop, reported
+    # beside it.
+    scat_s = stage_avg["scatter"] * 1e-3
+    alg_tf = 22.0 * N * Nl / scat_s / 1e12
+    mfma_tf = 96.0 * N * Nl / scat_s / 1e12
     # contraction stage (k_contract_y<float> + k_add_der): algorithmic bytes =
     # the FP32 Y-only table, one float per (l, m, node) over the 2 ceil(Nx/4)
     # node pairs of the layout
@@ -541,12 +688,15 @@ def main():
         "alm_per_s": Nlm / (ms * 1e-3),
         "stage_ms": stage_avg,
         "roofline": {"kernel": "k_scatter_mfma", "bound": "mfma", "engine": "MFMA fp16 (x3 split)",
-                     "achieved": scat_tf, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": scat_tf / PEAK_FP16_TFLOPS, "traffic": tr("k_scatter_mfma"),
-                     "algorithmic": "96 MFMA flop per (point, m): 32-flop real GEMM form of "
-                                    "4 complex products, x3 for the fp16 hi/lo split",
-                     "valu_equivalent_tflops": valu_tf,
-                     "valu_equivalent_frac": valu_tf / PEAK_FP32_TFLOPS},
+                     "achieved": alg_tf, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": alg_tf / PEAK_FP16_TFLOPS, "traffic": tr("k_scatter_mfma"),
+                     "algorithmic": "22 flop per (point, m): 4 complex multiply-adds + the "
+                                    "complex step of e^{-i m phi}; 22 N Nl per transform",
+                     "mfma_instruction_tflops": mfma_tf,
+                     "mfma_instruction_frac": mfma_tf / PEAK_FP16_TFLOPS,
+                     "mfma_instruction_flop": "96 per (point, m): 32-flop real GEMM form, "
+                                              "x3 for the fp16 hi/lo split",
+                     "fp32_vector_frac": alg_tf / PEAK_FP32_TFLOPS},
         "roofline_contract": {"kernel": "k_contract_y<float> + k_add_der", "bound": "hbm",
                               "achieved": con_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                               "frac": con_gbs / PEAK_HBM_GBS, "traffic": tr("k_contract_y"),
@@ -577,20 +727,42 @@ def main():
             "rms_rel": float(d.norm() / a64.norm())}
         del s64
         torch.cuda.empty_cache()
+        log("accuracy: error against the exact direct sum (1000 points)")
+        result["method_error_vs_exact"] = method_error(Nl, Nx, th, ph, w[None, :], 1000)
 
     if not args.no_cpu_baseline:
         n = min(args.cpu_sample, N)
         sample = [a[:n].cpu().numpy() for a in (th, ph, w)]
         try:
-            result["cpu_baseline"] = cpu_baseline(*sample, Nl, Nx)
+            result["cpu_baseline"] = cpu_baseline(*sample, Nl, Nx, N, sizes=(n // 5, n))
         except Exception as e:  # reported, never fatal for the GPU number
             log(f"cpu baseline failed: {e!r}")
             result["cpu_baseline"] = None
         if result.get("cpu_baseline"):
             result["cpu_baseline"]["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+        try:
+            c1 = cpu_numpy_c1()
+            from directsht_amd.sht import DirectSHT as _D
+            import numpy as _np
+            rng = _np.random.default_rng(1)
+            t1 = torch.from_numpy(_np.arccos(rng.uniform(-1, 1, 100_000))).cuda()
+            p1 = torch.from_numpy(rng.uniform(0, 2 * _np.pi, 100_000)).cuda()
+            w1 = torch.from_numpy(rng.uniform(0.5, 1.5, 100_000)).cuda()
+            s1 = _D(256, 512, dflt_type="float64")
+            g1 = _timed(lambda: s1.transform_device(t1, p1, w1), 20, torch.cuda.synchronize)
+            c1["gpu_fp64_ms"] = g1 * 1e3
+            c1["gpu_over_cpu"] = c1["median_s"] / g1
+            result["cpu_baseline_c1_numpy"] = c1
+            del s1
+        except Exception as e:
+            log(f"numpy C1 baseline failed: {e!r}")
     if not args.skip_batched:
         result["batched"] = {}
-        for name, fn in (("weight_sets", bench_weight_sets), ("uneven", bench_uneven)):
+        for name, fn in (("weight_sets", bench_weight_sets), ("uneven", bench_uneven),
+                         ("weight_sets_headline_16",
+                          lambda: bench_weight_sets(headline=True, S=16)),
+                         ("weight_sets_headline_64",
+                          lambda: bench_weight_sets(headline=True, S=64))):
             try:
                 result["batched"][name] = fn()
                 log(f"{name}: {result['batched'][name]}")

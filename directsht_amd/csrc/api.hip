@@ -286,8 +286,14 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
     set_error("dsht: need Nl >= 2, Nx >= 2 and 0 < xmax < 1");
     return DSHT_EINVAL;
   }
-  if (Nl > 2048) {
-    set_error("dsht: Nl > 2048 is not supported");
+  if (Nx > kMaxNx) {
+    set_error("dsht: Nx > " + std::to_string(kMaxNx) + " is not supported");
+    return DSHT_EINVAL;
+  }
+  // tables alone (make_table) have no band limit beyond memory; transforms
+  // are bounded by the polar rotation's per-degree LDS rows
+  if (alloc_tables && Nl > kMaxNl) {
+    set_error("dsht: transforms need Nl <= " + std::to_string(kMaxNl));
     return DSHT_EINVAL;
   }
   if (precision != DSHT_FP64 && precision != DSHT_FP32) {
@@ -804,8 +810,9 @@ int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {
 
 int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table_bytes,
                        int64_t* work_bytes) {
-  if (Nl < 2 || Nx < 2) {
-    set_error("dsht_plan_estimate: need Nl >= 2, Nx >= 2");
+  if (Nl < 2 || Nx < 2 || Nl > kMaxNl || Nx > kMaxNx || Np < 0) {
+    set_error("dsht_plan_estimate: need 2 <= Nl <= " + std::to_string(kMaxNl) + ", 2 <= Nx <= " +
+              std::to_string(kMaxNx) + " and Np >= 0");
     return DSHT_EINVAL;
   }
   std::vector<GroupDesc> groups;

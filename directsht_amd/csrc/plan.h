@@ -14,6 +14,12 @@ namespace dsht {
 // the Y-only table can hold two pairs of a row in one 16-byte vector.
 inline int node_pairs(int Nx) { return 2 * ((Nx + 3) / 4); }
 
+// Band and grid limits of transform plans: the polar rotation stages 4 Nl
+// doubles per degree in LDS (128 KB at Nl = 4096); Nx bounds the segment and
+// node-pair counts well inside int range.
+constexpr int kMaxNl = 4096;
+constexpr int kMaxNx = 1 << 20;
+
 struct Workspace {
   int64_t cap_points = 0;  // capacity of the sorted-point arrays
   int64_t cap_chunks = 0;  // capacity of per-chunk arrays

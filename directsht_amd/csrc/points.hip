@@ -206,16 +206,44 @@ __global__ __launch_bounds__(1024) void k_scan(const int* __restrict__ in, int n
   }
 }
 
+// max |w| over the valid points of a block into *wmax (float bits of a
+// non-negative value: their order as unsigned integers is the float order),
+// one atomic per block; non-finite weights are left out (they make the sums
+// non-finite by themselves).  The matrix-core scatter scales its fp16
+// operands by the power of two this bound gives.
+__device__ inline void block_wmax(float v, unsigned* __restrict__ wmax) {
+  __shared__ float red[16];
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();  // red[] may still be read by a previous call
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) b = fmaxf(b, red[i]);
+    if (b > 0.f) atomicMax(wmax, __float_as_uint(b));
+  }
+}
+
+__device__ inline float abs_w(double w, int seg, int nseg) {
+  const float a = (float)fabs(w);
+  return (seg < nseg && a <= 3.0e38f) ? a : 0.f;
+}
+
 // Place each point's 64-byte record at its sorted position.  The order inside
 // a segment is that of the atomics (the sums over a segment do not depend on
 // it beyond rounding).
 __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restrict__ theta,
                                                const double* __restrict__ phi,
                                                const double* __restrict__ wt, PointParams p,
-                                               int* __restrict__ cursor, PointRec* __restrict__ rec) {
+                                               int* __restrict__ cursor, PointRec* __restrict__ rec,
+                                               unsigned* __restrict__ wmax) {
+  float vmax = 0.f;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
-    Classified c = classify(theta[i], phi[i], wt ? wt[i] : 1.0, p);
+    const double wi = wt ? wt[i] : 1.0;
+    Classified c = classify(theta[i], phi[i], wi, p);
+    vmax = fmaxf(vmax, abs_w(wi, c.seg, p.nseg));
     const int pos = atomicAdd(&cursor[c.seg], 1);
     double2* d = reinterpret_cast<double2*>(rec + pos);
     d[0] = make_double2(c.ere, c.eim);
@@ -224,6 +252,7 @@ __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restr
     reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)i);
     if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
   }
+  block_wmax(vmax, wmax);
 }
 
 // Segment containing sorted position i: the last s with seg_start[s] <= i
@@ -362,13 +391,15 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
                                                              const double* __restrict__ wt,
                                                              PointParams p,
                                                              const int* __restrict__ toff,
-                                                             PointRec* __restrict__ rec) {
+                                                             PointRec* __restrict__ rec,
+                                                             unsigned* __restrict__ wmax) {
   extern __shared__ int lo[];
   const int nb = p.nseg + 1;
   const int* row = toff + (int64_t)blockIdx.x * nb;
   for (int i = threadIdx.x; i < nb; i += kSortThreads) lo[i] = row[i];
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
+  float vmax = 0.f;
   for (int64_t i0 = b + threadIdx.x; i0 < e; i0 += kSortThreads * kSortBatch) {
     double th[kSortBatch], ph[kSortBatch], ww[kSortBatch];
 #pragma unroll
@@ -382,6 +413,7 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
     for (int u = 0; u < kSortBatch; ++u) {
       if (i0 + (int64_t)u * kSortThreads < e) {
         const Classified c = classify(th[u], ph[u], ww[u], p);
+        vmax = fmaxf(vmax, abs_w(ww[u], c.seg, p.nseg));
         const int pos = atomicAdd(&lo[c.seg], 1);
         double2* d = reinterpret_cast<double2*>(rec + pos);
         d[0] = make_double2(c.ere, c.eim);
@@ -392,13 +424,15 @@ __global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int til
       }
     }
   }
+  block_wmax(vmax, wmax);
 }
 
 __global__ __launch_bounds__(256) void k_clear2(int* __restrict__ a, int na, int* __restrict__ b,
-                                                int nb) {
+                                                int nb, unsigned* __restrict__ c) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < na) a[i] = 0;
   if (i < nb) b[i] = 0;
+  if (i == 0) c[0] = 0u;
 }
 
 // One weight set into the sorted order of the points: wsort[pos] = w[idx].
@@ -425,7 +459,7 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
                 pl.acos_mxmax};
   // counters are cleared by a kernel, not a memset node, so the transform can
   // be captured in a HIP graph and replayed
-  k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, flags, 4);
+  k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, flags, 4, ws.wmax);
   DSHT_LAUNCH_CHECK();
   const int nb = pl.nseg + 1;
   const size_t lds = sizeof(int) * nb;
@@ -456,9 +490,10 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
     if (tiled) {
       k_col_offsets<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_start);
       DSHT_LAUNCH_CHECK();
-      k_tile_place<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, wt, p, ws.thist, ws.rec);
+      k_tile_place<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, wt, p, ws.thist, ws.rec,
+                                                     ws.wmax);
     } else {
-      k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec);
+      k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec, ws.wmax);
     }
     DSHT_LAUNCH_CHECK();
   }

@@ -21,6 +21,8 @@
 // One workgroup per l, one thread per m; FP64 throughout.  If a polar point
 // failed the basic_sht range check after rotation, the reference returns
 // zeros for the whole transform (sht/sht.py:141-149); so does this kernel.
+#include <mutex>
+
 #include "common.h"
 #include "plan.h"
 
@@ -400,6 +402,16 @@ int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride
                           int* status, hipStream_t s) {
   if (nsets == 1) {
     const size_t lds = sizeof(double) * 4 * (size_t)pl.Nl;
+    if (lds > 64 * 1024) {  // beyond the default dynamic LDS of a launch
+      static std::once_flag once;
+      static hipError_t attr = hipSuccess;
+      std::call_once(once, [] {
+        attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rotate_combine<1>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double) * 4 * kMaxNl));
+      });
+      DSHT_HIP(attr);
+    }
     k_rotate_combine<1><<<pl.Nl, 256, lds, s>>>(pl.ws.a_equ, pl.ws.a_pol, alm, pl.Nl, 1, flags,
                                                 flag_stride, status, pl.rot_top, pl.rot_exp,
                                                 pl.rot_AB, pl.rot_iB);

@@ -131,7 +131,8 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
                                                           const int* __restrict__ run_base,
                                                           const double* __restrict__ wsort,
                                                           float* __restrict__ part,
-                                                          int* __restrict__ run_seg) {
+                                                          int* __restrict__ run_seg,
+                                                          const unsigned* __restrict__ wmax) {
   constexpr int kWaves = M1 / 4;              // row tiles: 4 k * M1 m1 * (re, im) / 32
   constexpr int kThreads = 64 * kWaves;
   constexpr int kBlk = KB;                    // points per staged block
@@ -172,6 +173,20 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
   const int bA = 2 * w + gA;
   const int kA = bA / kMBlocks;
   const int m1A = 8 * (bA % kMBlocks) + (ia & 7);
+
+  // Range of the fp16 operands: the staged weights w h_k are scaled by
+  // 2^-ex, with 2^ex the power of two at or above max |w| of the transform
+  // (|h_k| <= 1 and |E| = 1, so every operand is at most 1 in magnitude and
+  // the split keeps its ~2^-22 relative precision for any weight range), and
+  // each run's sums are scaled back by 2^ex, exactly.
+  int ex = 0;
+  {
+    const float mx = __uint_as_float(wmax[0]);
+    if (mx > 0.f && mx <= 3.0e38f) (void)frexpf(mx, &ex);
+    ex = max(-120, min(120, ex));
+  }
+  const double w_down = ldexp(1.0, -ex);
+  const float w_up = ldexpf(1.0f, ex);
 
   floatx16 acc[2];  // [column tile re | im], accumulating the current run
   const floatx16 zero = {};
@@ -248,7 +263,7 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
   auto write_h = [&](int nbuf) {
 #pragma unroll
     for (int j = 0; j < kHPer; ++j)
-      sH[nbuf][gg * kHPer + j][gp] = (float)(hold_ok * hold_w * hold_h[j]);
+      sH[nbuf][gg * kHPer + j][gp] = (float)(hold_ok * hold_w * w_down * hold_h[j]);
   };
   // chain steps of window j of the block being generated into buffer nbuf
   auto gen_slice_b = [&](int j, int nbuf) {
@@ -320,8 +335,8 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
       for (int d = 0; d < 4; ++d) {
         const int m = kMT * mt + 32 * (8 * (bb % kMBlocks) + 4 * bq + d) + (l & 31);
         if (m < Nl) {
-          const float vre = acc[0][8 * g + d] - acc[1][8 * g + 4 + d];
-          const float vim = acc[1][8 * g + d] + acc[0][8 * g + 4 + d];
+          const float vre = (acc[0][8 * g + d] - acc[1][8 * g + 4 + d]) * w_up;
+          const float vim = (acc[1][8 * g + d] + acc[0][8 * g + 4 + d]) * w_up;
           *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + kk) * Nl + m) * 2) =
               make_float2(vre, vim);
         }
@@ -475,27 +490,60 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
 }
 
 template <int M1, int KB>
-static int launch_m1(Plan& pl, int nchunks, const double* w, bool gather, hipStream_t s) {
+static int launch_m1(Plan& pl, int nchunks, const double* w, bool gather, const unsigned* wmax,
+                     hipStream_t s) {
   dim3 grid(nchunks, (pl.Nl + 32 * M1 - 1) / (32 * M1));
   if (gather)
     k_scatter_mfma<M1, KB, true><<<grid, 16 * M1, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg,
                                                          pl.chunk, pl.Nl, pl.ws.run_base, w,
-                                                         (float*)pl.ws.part, pl.ws.run_seg);
+                                                         (float*)pl.ws.part, pl.ws.run_seg, wmax);
   else
     k_scatter_mfma<M1, KB, false><<<grid, 16 * M1, 0, s>>>(pl.ws.rec, pl.ws.seg_start, pl.nseg,
                                                           pl.chunk, pl.Nl, pl.ws.run_base, w,
-                                                          (float*)pl.ws.part, pl.ws.run_seg);
+                                                          (float*)pl.ws.part, pl.ws.run_seg, wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
 
-// m tile of 256, 512 or 1024 orders: the narrowest that covers Nl, or 1024
+// *wmax = max |w[i]| over the finite entries of one weight set (float bits)
+__global__ __launch_bounds__(256) void k_absmax(int64_t n, const double* __restrict__ w,
+                                                unsigned* __restrict__ wmax) {
+  __shared__ float red[4];
+  float v = 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float a = (float)fabs(w[i]);
+    if (a <= 3.0e38f) v = fmaxf(v, a);
+  }
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float b = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (b > 0.f) atomicMax(wmax, __float_as_uint(b));
+  }
+}
+
+__global__ void k_set_u32(unsigned* __restrict__ p, unsigned v) { *p = v; }
+
+// m tile of 256, 512 or 1024 orders: the narrowest that covers Nl, or 1024.
+// The operand scale comes from ws.wmax[0]: the points pass's max |w| for the
+// records' own weights; for a gathered weight set, its own max, reduced here.
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
-  if (pl.Nl <= 256) return launch_m1<8, 64>(pl, nchunks, w, gather, s);
-  if (pl.Nl <= 512) return launch_m1<16, 64>(pl, nchunks, w, gather, s);
-  return launch_m1<32, 128>(pl, nchunks, w, gather, s);
+  const unsigned* wmax = pl.ws.wmax;
+  if (gather || w) {
+    // records hold the basis alone (unit weights): the set's weights set the range
+    k_set_u32<<<1, 1, 0, s>>>(pl.ws.wmax, 0u);
+    DSHT_LAUNCH_CHECK();
+    const int blocks = (int)std::min<int64_t>((Np + 255) / 256, 2048);
+    k_absmax<<<blocks, 256, 0, s>>>(Np, w, pl.ws.wmax);
+    DSHT_LAUNCH_CHECK();
+  }
+  if (pl.Nl <= 256) return launch_m1<8, 64>(pl, nchunks, w, gather, wmax, s);
+  if (pl.Nl <= 512) return launch_m1<16, 64>(pl, nchunks, w, gather, wmax, s);
+  return launch_m1<32, 128>(pl, nchunks, w, gather, wmax, s);
 }
 
 }  // namespace dsht

@@ -9,7 +9,7 @@ Request  (32-byte header + 24 N bytes):
     0  magic  b"DSHQ"
     4  u16    version (1)
     6  u16    flags (0)
-    8  i32    Nl (2 .. 2048)
+    8  i32    Nl (2 .. 4096)
    12  i32    Nx (0: the server's default, nx_factor * Nl)
    16  i64    N, the number of points
    24  u64    request id (echoed in the response)
@@ -35,7 +35,8 @@ REQ_MAGIC, RESP_MAGIC, VERSION = b"DSHQ", b"DSHR", 1
 OK, MALFORMED, REJECTED, FAILED = 0, 1, 2, 3
 _HDR = struct.Struct("<4sHHiiqQ")  # 32 bytes
 HEADER_BYTES = _HDR.size
-MAX_NL = 2048
+MAX_NL = 4096   # the library's transform band limit
+MAX_NX = 1 << 20  # its grid limit
 
 
 def encode_request(theta, phi, wt, Nl, Nx=0, rid=0):
@@ -60,8 +61,8 @@ def decode_request(buf):
         raise ValueError(f"unsupported request version {ver}")
     if not 2 <= Nl <= MAX_NL:
         raise ValueError(f"Nl={Nl} outside 2 .. {MAX_NL}")
-    if Nx != 0 and Nx < 2:
-        raise ValueError(f"Nx={Nx} must be 0 (default) or at least 2")
+    if Nx != 0 and not 2 <= Nx <= MAX_NX:
+        raise ValueError(f"Nx={Nx} must be 0 (default) or in 2 .. {MAX_NX}")
     if n < 0 or len(mv) != HEADER_BYTES + 24 * n:
         raise ValueError(f"request length {len(mv)} does not hold {n} points")
     arr = np.frombuffer(mv, dtype="<f8", count=3 * n, offset=HEADER_BYTES)

@@ -60,8 +60,12 @@ extern "C" {
 
 /* Fill caller-owned host arrays Yv, Yd (each Nl(Nl+1)/2 * Nx doubles, healpy
  * row order, row length Nx) with Y_lm(x_j, 0) and dY_lm/dx at
- * x_j = j*xmax/(Nx-1).  Built by the FP64 recurrence on the GPU.
- * Replaces csht/sht_helper.c:18-116.  Returns 0. */
+ * x_j = j*xmax/(Nx-1).  Built by the FP64 recurrence on the GPU.  Any
+ * Nl >= 2 (memory permitting), 2 <= Nx <= 2^20, 0 < xmax < 1.
+ * Replaces csht/sht_helper.c:18-116.  Returns 0 on success, 1 for arguments
+ * outside these ranges or a failed device allocation or copy, with the
+ * reason in dsht_last_error() (the caller's arrays are then left as they
+ * were; the reference returns 0 unconditionally). */
 int make_table(int Nl, int Nx, double xmax, double Yv[], double Yd[]);
 
 /* Direct SHT of Np points with cos(theta) = cost[i] (all of one sign,
@@ -72,8 +76,10 @@ int make_table(int Nl, int Nx, double xmax, double Yv[], double Yd[]);
  * does.  The reference requires cost sorted by |cost|; this implementation
  * sorts on the device and accepts any order.  The tables are uploaded on every
  * call (the plan API keeps them resident).  Replaces csht/sht_helper.c:176-286.
- * Returns 0, or 1 if the device work failed (the reference returns 1 on
- * allocation failure). */
+ * Band limit 2 <= Nl <= 4096 (the transform kernels'), 2 <= Nx <= 2^20.
+ * Returns 0, or 1 for arguments outside these ranges or failed device work,
+ * with the reason in dsht_last_error() and carr, sarr untouched (the
+ * reference returns 1 on allocation failure). */
 int do_transform(int Nl, int Nx, double xmax, double Yv[], double Yd[], int Np,
                  double cost[], double phi[], double wt[], double carr[], double sarr[]);
 
@@ -86,7 +92,7 @@ typedef struct dsht_plan dsht_plan;
 /* Build the FP64 Ylm/dYlm spline tables of DirectSHT(Nl, Nx, xmax) on the
  * current device (sht/sht.py:32-56) in the layout the contraction streams,
  * keeping an FP64 copy (precision DSHT_FP64) or an FP32 copy (DSHT_FP32).
- * Requires Nl >= 2, Nx >= 2, 0 < xmax < 1. */
+ * Requires 2 <= Nl <= 4096, 2 <= Nx <= 2^20, 0 < xmax < 1. */
 int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** plan);
 int dsht_plan_destroy(dsht_plan* plan);
 

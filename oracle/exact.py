@@ -27,49 +27,59 @@ def _indx(ell, m, Nl):
 
 
 def ylm_values(Nl, theta):
-    """Y_lm(theta, 0) for all healpy rows, shape (Nlm, N)."""
+    """Y_lm(theta, 0) for all healpy rows, shape (Nlm, N).  The recurrence in
+    l runs for all orders m at once (rows with the same l - m)."""
     theta = np.asarray(theta, dtype=np.float64)
     x, s = np.cos(theta), np.sin(theta)
     Nlm = Nl * (Nl + 1) // 2
     out = np.empty((Nlm, theta.size))
+    ms = np.arange(Nl)
+    diag = _indx(ms, ms, Nl)
     pmm = np.full(theta.size, 1.0 / np.sqrt(4 * np.pi))
-    for m in range(Nl):
-        if m > 0:
-            pmm = -np.sqrt((2 * m + 1) / (2.0 * m)) * s * pmm
-        out[_indx(m, m, Nl)] = pmm
-        if m + 1 < Nl:
-            p1 = np.sqrt(2 * m + 3.0) * x * pmm
-            out[_indx(m + 1, m, Nl)] = p1
-            p2 = pmm
-            for ell in range(m + 2, Nl):
-                a = np.sqrt((4.0 * ell * ell - 1) / (ell * ell - m * m))
-                b = np.sqrt(((ell - 1.0) ** 2 - m * m) / (4.0 * (ell - 1) ** 2 - 1))
-                p = a * (x * p1 - b * p2)
-                out[_indx(ell, m, Nl)] = p
-                p2, p1 = p1, p
+    out[diag[0]] = pmm
+    for m in range(1, Nl):
+        pmm = -np.sqrt((2 * m + 1) / (2.0 * m)) * s * pmm
+        out[diag[m]] = pmm
+    if Nl > 1:
+        m = ms[:-1]
+        out[_indx(m + 1, m, Nl)] = np.sqrt(2 * m + 3.0)[:, None] * x[None, :] * out[diag[:-1]]
+    for k in range(2, Nl):
+        m = ms[:Nl - k]
+        ell = (m + k).astype(np.float64)
+        a = np.sqrt((4.0 * ell * ell - 1) / (ell * ell - m * m))
+        b = np.sqrt(((ell - 1.0) ** 2 - m * m) / (4.0 * (ell - 1) ** 2 - 1))
+        out[_indx(m + k, m, Nl)] = a[:, None] * (x[None, :] * out[_indx(m + k - 1, m, Nl)]
+                                                 - b[:, None] * out[_indx(m + k - 2, m, Nl)])
     return out
 
 
-def exact_alm(Nl, theta, phi, w, chunk=4096):
-    """sum_i w_i Y*_lm(theta_i, phi_i) in healpy order (complex128)."""
+def exact_alm(Nl, theta, phi, w, chunk=None):
+    """sum_i w_i Y*_lm(theta_i, phi_i) in healpy order (complex128); w may be
+    (S, N) weight rows sharing the points, giving (S, Nlm).  Points go
+    through the Ylm recurrence `chunk` at a time (default: about 1 GB of
+    Ylm values per chunk)."""
     theta = np.asarray(theta, dtype=np.float64)
     phi = np.asarray(phi, dtype=np.float64)
     w = np.asarray(w, dtype=np.float64)
+    one = w.ndim == 1
+    W = np.atleast_2d(w)
     Nlm = Nl * (Nl + 1) // 2
-    alm = np.zeros(Nlm, dtype=np.complex128)
+    if chunk is None:
+        chunk = int(max(16, min(4096, (1 << 27) // max(Nlm, 1))))
+    alm = np.zeros((W.shape[0], Nlm), dtype=np.complex128)
     for i0 in range(0, theta.size, chunk):
         sl = slice(i0, i0 + chunk)
         Y = ylm_values(Nl, theta[sl])
         for m in range(Nl):
-            e = w[sl] * np.exp(-1j * m * phi[sl])
+            e = W[:, sl] * np.exp(-1j * m * phi[sl])[None, :]
             r = slice(_indx(m, m, Nl), _indx(Nl - 1, m, Nl) + 1)
-            alm[r] += Y[r] @ e
-    return alm
+            alm[:, r] += e @ Y[r].T
+    return alm[0] if one else alm
 
 
 def exact_alm_multi(Nl, theta, phi, W):
     """exact_alm for several weight vectors W (S, N) sharing the points."""
-    return np.stack([exact_alm(Nl, theta, phi, w) for w in np.atleast_2d(W)])
+    return exact_alm(Nl, theta, phi, np.atleast_2d(W))
 
 
 def synth_rings(alm, Nl, x):

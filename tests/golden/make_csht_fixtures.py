@@ -21,6 +21,17 @@ Fixtures
                          no point lies on a node: csht bins by a scan
                          (csht/sht_helper.c:199-203) but interpolates with
                          (int)(ax/dx) (:208), which disagree at nodes)
+  csht_c2_equ.npz        the headline configuration Nl=1000, Nx=2048: the
+                         equatorial part (0.3 <= cos theta < 0.75) of 12000
+                         points drawn by the lognormal-cap generator
+                         (directsht_amd.mocks.lognormal_cap_points on the CPU,
+                         its field synthesized by oracle.exact.synth_rings,
+                         seed 2, w = 1), do_transform alm at every 37th
+                         healpy index, and table rows at every 997th index
+                         (every 16th node)
+  csht_nl2100.npz        band limit above 2048: Nl=2100, Nx=40, 1500 seeded
+                         equatorial points with weights U[0.5, 1.5]; alm at
+                         every 101st index and table rows at every 4999th
 """
 import os
 import sys
@@ -71,6 +82,53 @@ def main():
                         Nl_rand=64, Nx_rand=128, theta=theta, phi=phi, wt=wt, alm_rand=alm_rand,
                         Nl_det=48, Nx_det=100, theta_det=theta_d, phi_det=phi_d, wt_det=wt_d,
                         alm_det=alm_det, xmax=0.75)
+    # (d) the headline configuration; the generator's field is synthesized
+    # on the host by the exact oracle's ring synthesis instead of the device
+    # kernel (same alm and grid), and the points are drawn by torch on the CPU
+    import torch
+
+    from directsht_amd import mocks
+    from oracle import exact
+
+    def synthesize_map_cpu(alm, Nl, x, nphi, device=None):
+        F = torch.from_numpy(exact.synth_rings(np.asarray(alm), Nl, np.asarray(x)))
+        spec = torch.zeros((F.shape[0], nphi // 2 + 1), dtype=torch.complex128)
+        spec[:, :Nl] = F
+        spec[:, 0] = spec[:, 0].real
+        return torch.fft.irfft(spec, n=nphi, dim=1) * nphi
+
+    mocks.synthesize_map = synthesize_map_cpu
+    th, ph, _ = mocks.lognormal_cap_points(12000, seed=2, device="cpu")
+    th, ph = th.numpy(), ph.numpy()
+    equ = np.abs(np.cos(th)) < 0.75
+    th, ph = th[equ], ph[equ]
+    wt = np.ones_like(th)
+    Nl, Nx = 1000, 2048
+    c = cref.CRef(Nl, Nx, 0.75)
+    alm = c(th, ph, wt)
+    Yv, Yd = c.tables()
+    aidx = np.arange(0, c.Nlm, 37)
+    ridx = np.arange(0, c.Nlm, 997)
+    cols = np.arange(0, Nx, 16)
+    np.savez_compressed(os.path.join(HERE, "csht_c2_equ.npz"), Nl=Nl, Nx=Nx, xmax=0.75,
+                        theta=th, phi=ph, wt=wt, aidx=aidx, alm=alm[aidx], ridx=ridx, cols=cols,
+                        Yv=Yv[ridx][:, cols], Yd=Yd[ridx][:, cols])
+    del c, Yv, Yd, torch
+    # (e) a band limit above 2048
+    Nl, Nx = 2100, 40
+    rng = np.random.default_rng(2100)
+    N = 1500
+    theta = np.arccos(rng.uniform(-0.74, 0.74, N))
+    phi = rng.uniform(0, 2 * np.pi, N)
+    wt = rng.uniform(0.5, 1.5, N)
+    c = cref.CRef(Nl, Nx, 0.75)
+    alm = c(theta, phi, wt)
+    Yv, Yd = c.tables()
+    aidx = np.arange(0, c.Nlm, 101)
+    ridx = np.arange(0, c.Nlm, 4999)
+    np.savez_compressed(os.path.join(HERE, "csht_nl2100.npz"), Nl=Nl, Nx=Nx, xmax=0.75,
+                        theta=theta, phi=phi, wt=wt, aidx=aidx, alm=alm[aidx], ridx=ridx,
+                        Yv=Yv[ridx], Yd=Yd[ridx])
     print("wrote fixtures to", HERE)
 
 

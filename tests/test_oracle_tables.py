@@ -108,3 +108,19 @@ def test_exact_ylm_vs_scipy():
     for ell, m in [(0, 0), (1, 1), (17, 3), (119, 0), (119, 60), (119, 119), (80, 79)]:
         ref = special.sph_harm_y(ell, m, th, 0.0).real
         assert np.max(np.abs(Y[refalg.indx(ell, m, Nl)] - ref)) < 1e-12
+
+
+def test_headline_table_rows_vs_csht(golden_dir):
+    """Rows of the Nl=1000, Nx=2048 tables at every 16th node against the
+    reference's C make_table (tests/golden/csht_c2_equ.npz): the recurrences
+    are per column, so the oracle evaluated at those nodes alone gives the
+    same values."""
+    d = np.load(f"{golden_dir}/csht_c2_equ.npz")
+    Nl, Nx = int(d["Nl"]), int(d["Nx"])
+    xx = refalg.x_grid(Nx, 0.75)[d["cols"]]
+    Yv = refalg.compute_Plm_table(Nl, xx)
+    Yd = refalg.compute_der_table(Nl, xx, Yv)
+    Yv, Yd = refalg.norm_ext(Yv, Nl), refalg.norm_ext(Yd, Nl)
+    for mine, ref in ((Yv[d["ridx"]], d["Yv"]), (Yd[d["ridx"]], d["Yd"])):
+        scale = np.maximum(np.max(np.abs(ref), axis=1, keepdims=True), 1e-300)
+        assert np.max(np.abs(mine - ref) / scale) < 1e-12

@@ -39,7 +39,8 @@ def test_response_round_trip():
         wire.encode_response(1, Nl, a[:-1])
 
 
-@pytest.mark.parametrize("mutate", ["magic", "version", "short", "long", "nl", "nx", "header"])
+@pytest.mark.parametrize("mutate", ["magic", "version", "short", "long", "nl", "nx", "nx_huge",
+                                    "header"])
 def test_malformed_requests(mutate):
     f = bytearray(wire.encode_request(np.ones(3), np.ones(3), np.ones(3), 32))
     if mutate == "magic":
@@ -51,9 +52,11 @@ def test_malformed_requests(mutate):
     elif mutate == "long":
         f += b"\0" * 8
     elif mutate == "nl":
-        struct.pack_into("<i", f, 8, 4096)
+        struct.pack_into("<i", f, 8, 4097)
     elif mutate == "nx":
         struct.pack_into("<i", f, 12, 1)
+    elif mutate == "nx_huge":  # segment and pair counts of such a grid overflow int32
+        struct.pack_into("<i", f, 12, 1 << 30)
     elif mutate == "header":
         f = f[:20]
     with pytest.raises(ValueError):
