@@ -41,6 +41,8 @@ SIGNATURES = {
                                       ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),
     "dsht_plan_get_tables": (ct.c_int, [_vp, _vp, _vp]),
     "dsht_transform": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, _vp, _vp, _vp, ct.c_int, _vp]),
+    "dsht_transform_noalloc": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, _vp, _vp, _vp, ct.c_int,
+                                          _vp]),
     "dsht_transform_sets": (ct.c_int, [_vp, ct.c_int64, _vp, _vp, ct.c_int, _vp, ct.c_int64, _vp,
                                        _vp, ct.c_int, _vp]),
     "dsht_transform_many": (ct.c_int, [_vp, ct.c_int, ct.POINTER(ct.c_int64), ct.POINTER(_vp),

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -109,6 +110,10 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
 }
 
 static void free_workspace(Workspace& ws) {
+  if (ws.fixed) {  // carved from the fixed buffer: nothing to free
+    ws = Workspace{};
+    return;
+  }
   for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
                   (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
@@ -142,6 +147,58 @@ static int64_t rhs_per_pair(const Plan& pl, int nsets) {
   return use_mfma_contract(pl, nsets) ? std::max<int64_t>(24, 16 * (int64_t)nsets) : 24;
 }
 
+// Lay out every workspace array of a transform with capacities `want`
+// through alloc(ptr, bytes) (device allocation, or carving from a fixed
+// buffer), and record the capacities.
+template <class Alloc>
+static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, int sets,
+                            int psets, bool with_w, int64_t rhs_elems, Alloc&& alloc) {
+  const int fsz = pl.precision == DSHT_FP32 ? 4 : 8;
+  int rc = DSHT_OK;
+  auto get = [&](auto*& ptr, int64_t nbytes) {
+    void* q = nullptr;
+    if (rc == DSHT_OK) rc = alloc(&q, nbytes);
+    ptr = static_cast<std::remove_reference_t<decltype(ptr)>>(q);
+  };
+  get(ws.rec, want.points * (int64_t)sizeof(PointRec));
+  get(ws.run_count, (want.chunks + 1) * (int64_t)sizeof(int));
+  get(ws.run_base, (want.chunks + 1) * (int64_t)sizeof(int));
+  get(ws.run_seg, want.runs * (int64_t)sizeof(int));
+  get(ws.part, psets * want.runs * (int64_t)pl.Nl * 8 * fsz);
+  get(ws.seg_count, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_start, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_cursor, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_run_lo, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_run_hi, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.ne_rank, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.thist, want.tiles * (pl.nseg + 1) * (int64_t)sizeof(int));
+  get(ws.rhs, rhs_elems * fsz);
+  get(ws.rmax, sets * (int64_t)sizeof(unsigned));
+  get(ws.wmax, sets * (int64_t)sizeof(unsigned));
+  if (psets > 1) {
+    ws.ldws = (want.points + 63) / 64 * 64 + 64;
+    get(ws.sidx, want.points * (int64_t)sizeof(int));
+    // sorted rows [psets][ldws], then the point-major copy [cap_points][psets]
+    get(ws.wsf, 2 * psets * ws.ldws * (int64_t)sizeof(float));
+  }
+  if (with_w) get(ws.wsort, want.points * (int64_t)sizeof(double));
+  get(ws.a_equ, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
+  get(ws.a_pol, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
+  if (sets > 1) get(ws.a_tmp, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
+  get(ws.a_der, (int64_t)kContractSplits * pl.Nlm * 8 * (int64_t)sizeof(double));
+  get(ws.flags, 4 * sets * (int64_t)sizeof(int));
+  if (rc) return rc;
+  ws.cap_points = want.points;
+  ws.cap_chunks = want.chunks;
+  ws.cap_runs = want.runs;
+  ws.cap_tiles = want.tiles;
+  ws.cap_sets = sets;
+  ws.cap_part_sets = psets;
+  ws.has_wsort = with_w;
+  ws.rhs_elems = rhs_elems;
+  return DSHT_OK;
+}
+
 static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false,
                             int part_sets = 1) {
   const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk_max);
@@ -151,6 +208,10 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
       z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort) &&
       rhs_need <= ws.rhs_elems && part_sets <= ws.cap_part_sets)
     return DSHT_OK;
+  if (ws.fixed) {
+    set_error("dsht: transform larger than the fixed workspace it was given");
+    return DSHT_EINVAL;
+  }
   DSHT_HIP(hipDeviceSynchronize());
   WsSizes want{std::max(z.points, ws.cap_points), std::max(z.chunks, ws.cap_chunks),
                std::max(z.runs, ws.cap_runs), std::max(z.tiles, ws.cap_tiles)};
@@ -159,42 +220,74 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   const bool with_w = wsort || ws.has_wsort;
   const int64_t rhs_elems = std::max(rhs_need, ws.rhs_elems);
   free_workspace(ws);
-  const int fsz = pl.precision == DSHT_FP32 ? 4 : 8;
-  DSHT_ALLOC(ws.rec, want.points * sizeof(PointRec));
-  DSHT_ALLOC(ws.run_count, (want.chunks + 1) * sizeof(int));
-  DSHT_ALLOC(ws.run_base, (want.chunks + 1) * sizeof(int));
-  DSHT_ALLOC(ws.run_seg, want.runs * sizeof(int));
-  DSHT_ALLOC(ws.part, psets * want.runs * (int64_t)pl.Nl * 8 * fsz);
-  DSHT_ALLOC(ws.seg_count, (pl.nseg + 2) * sizeof(int));
-  DSHT_ALLOC(ws.seg_start, (pl.nseg + 2) * sizeof(int));
-  DSHT_ALLOC(ws.seg_cursor, (pl.nseg + 2) * sizeof(int));
-  DSHT_ALLOC(ws.seg_run_lo, (pl.nseg + 2) * sizeof(int));
-  DSHT_ALLOC(ws.seg_run_hi, (pl.nseg + 2) * sizeof(int));
-  DSHT_ALLOC(ws.ne_rank, (pl.nseg + 2) * sizeof(int));
-  DSHT_ALLOC(ws.thist, want.tiles * (pl.nseg + 1) * sizeof(int));
-  DSHT_ALLOC(ws.rhs, rhs_elems * fsz);
-  DSHT_ALLOC(ws.rmax, sets * sizeof(unsigned));
-  DSHT_ALLOC(ws.wmax, sets * sizeof(unsigned));
-  if (psets > 1) {
-    ws.ldws = (want.points + 63) / 64 * 64 + 64;
-    DSHT_ALLOC(ws.sidx, want.points * sizeof(int));
-    // sorted rows [psets][ldws], then the point-major copy [cap_points][psets]
-    DSHT_ALLOC(ws.wsf, 2 * psets * ws.ldws * sizeof(float));
+  auto dev_alloc = [&](void** q, int64_t nbytes) -> int {
+    const size_t n = (size_t)std::max<int64_t>(nbytes, 16);
+    hipError_t e = hipMalloc(q, n);
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc(") + std::to_string(nbytes) + " bytes): " +
+                hipGetErrorString(e));
+      return DSHT_ENOMEM;
+    }
+    ws.bytes += nbytes;
+    return DSHT_OK;
+  };
+  return layout_workspace(pl, ws, want, sets, psets, with_w, rhs_elems, dev_alloc);
+}
+
+// A workspace for one single-set transform of Np points carved from the
+// fixed device buffer [base, base + cap): the same arrays as ensure_workspace
+// lays out, each 256-byte aligned.  Nothing is allocated.
+// *need receives the bytes the layout takes (also when they exceed cap).
+static int carve_workspace(const Plan& pl, int64_t Np, char* base, int64_t cap, Workspace& ws,
+                           int64_t* need) {
+  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk_max);
+  const int64_t rhs_need = (int64_t)pl.Nl * 2 * pl.Npair * rhs_per_pair(pl, 1);
+  int64_t off = 0;
+  auto bump = [&](void** q, int64_t nbytes) -> int {
+    const int64_t n = align256(std::max<int64_t>(nbytes, 16));
+    *q = off + n <= cap ? static_cast<void*>(base + off) : nullptr;
+    off += n;
+    return DSHT_OK;
+  };
+  ws = Workspace{};
+  layout_workspace(pl, ws, z, 1, 1, false, rhs_need, bump);
+  ws.fixed = true;
+  if (need) *need = off;
+  return off <= cap ? DSHT_OK : DSHT_ENOMEM;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed scratch buffer of dsht_transform_noalloc: kScratchBytes of device
+// memory in the library's code object, held on every device the library is
+// loaded on until the process ends.  Host code never passes the symbol
+// itself to a kernel: its device address, one per device, comes from
+// hipGetSymbolAddress on that device's first call.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchBytes = (int64_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_dev[kMaxDevices] = {};  // per device: address of `scratch`
+
+// The calling thread's current device must be `dev`.
+static int scratch_address(int dev, char** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("dsht_transform_noalloc: device index beyond the supported range");
+    return DSHT_EINVAL;
   }
-  if (with_w) DSHT_ALLOC(ws.wsort, want.points * sizeof(double));
-  DSHT_ALLOC(ws.a_equ, pl.Nlm * 2 * sets * sizeof(double));
-  DSHT_ALLOC(ws.a_pol, pl.Nlm * 2 * sets * sizeof(double));
-  if (sets > 1) DSHT_ALLOC(ws.a_tmp, pl.Nlm * 2 * sets * sizeof(double));
-  DSHT_ALLOC(ws.a_der, (int64_t)kContractSplits * pl.Nlm * 8 * sizeof(double));
-  DSHT_ALLOC(ws.flags, 4 * sets * sizeof(int));
-  ws.cap_points = want.points;
-  ws.cap_chunks = want.chunks;
-  ws.cap_runs = want.runs;
-  ws.cap_tiles = want.tiles;
-  ws.cap_sets = sets;
-  ws.cap_part_sets = psets;
-  ws.has_wsort = with_w;
-  ws.rhs_elems = rhs_elems;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("dsht_transform_noalloc: hipGetSymbolAddress(scratch): ") +
+                hipGetErrorString(e));
+      return DSHT_EHIP;
+    }
+    g_scratch_dev[dev] = static_cast<char*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
   return DSHT_OK;
 }
 
@@ -870,6 +963,42 @@ int dsht_transform(dsht_plan* p, int64_t Np, const double* theta, const double* 
   }
   DeviceGuard g(p->pl.device);
   return run_transform(p->pl, Np, theta, phi, wt, alm, status, mode, (hipStream_t)stream);
+}
+
+int dsht_transform_noalloc(dsht_plan* p, int64_t Np, const double* theta, const double* phi,
+                           const double* wt, double* alm, int* status, int mode, void* stream) {
+  if (!p) {
+    set_error("dsht_transform_noalloc: null plan");
+    return DSHT_EINVAL;
+  }
+  if (mode != DSHT_MODE_CALL && mode != DSHT_MODE_BASIC) {
+    set_error("dsht_transform_noalloc: bad mode");
+    return DSHT_EINVAL;
+  }
+  if (Np < 0 || Np >= (int64_t)1 << 31) {
+    set_error("dsht_transform_noalloc: need 0 <= Np < 2^31");
+    return DSHT_EINVAL;
+  }
+  Plan& pl = p->pl;
+  DeviceGuard g(pl.device);
+  char* base = nullptr;
+  int rc = scratch_address(pl.device, &base);
+  if (rc) return rc;
+  Workspace fixed;
+  int64_t need = 0;
+  if (carve_workspace(pl, Np, base, kScratchBytes, fixed, &need) != DSHT_OK) {
+    set_error("dsht_transform_noalloc: a transform of " + std::to_string(Np) + " points at Nl=" +
+              std::to_string(pl.Nl) + ", Nx=" + std::to_string(pl.Nx) + " needs " +
+              std::to_string(need) + " bytes of scratch; the fixed buffer holds " +
+              std::to_string(kScratchBytes) + " (use dsht_transform)");
+    return DSHT_EINVAL;
+  }
+  // the plan's own workspace is set aside (not freed, not grown) for the call
+  const Workspace own = pl.ws;
+  pl.ws = fixed;
+  rc = run_transform(pl, Np, theta, phi, wt, alm, status, mode, (hipStream_t)stream);
+  pl.ws = own;
+  return rc;
 }
 
 int dsht_transform_sets(dsht_plan* p, int64_t Np, const double* theta, const double* phi,

@@ -65,6 +65,8 @@ struct Workspace {
                               // contraction per node range (one set at a time)
   int* flags = nullptr;       // [cap_sets][4]: status bits, n_polar, unused, unused
   int64_t bytes = 0;
+  bool fixed = false;         // carved from a caller-independent fixed buffer: never
+                              // grown or freed (dsht_transform_noalloc)
 };
 
 // Frees a plan's table arrays (on the plan's device) when destroyed.  Also

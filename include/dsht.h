@@ -140,6 +140,23 @@ int dsht_plan_get_tables(dsht_plan* plan, double* Yv, double* Yd);
 int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
                    const double* wt, double* alm, int* status, int mode, void* stream);
 
+/* dsht_transform without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * transform (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results as
+ * dsht_transform; the plan's tables are used as they are, and the transform's
+ * workspace is carved from a fixed 64 MiB device buffer inside libdsht.so
+ * (one per device, held until the process ends) instead of the plan's
+ * workspace, which is neither used, grown nor freed.  This covers Nl <= 256:
+ * e.g. up to about 3.6e5 points at Nl=256, Nx=512 (FP32) or 9.7e5 at Nl=64,
+ * Nx=128.  Larger transforms return DSHT_EINVAL with the bytes they would
+ * need in dsht_last_error(), and leave alm untouched.  Every call on a device
+ * shares that device's buffer: calls whose work overlaps on one device (other
+ * streams, other threads, other plans) race, so serialize them.  The buffer
+ * keeps the last call's contents; the kernels clear what must start at zero. */
+int dsht_transform_noalloc(dsht_plan* plan, int64_t Np, const double* theta, const double* phi,
+                           const double* wt, double* alm, int* status, int mode, void* stream);
+
 /* Batched transforms (no counterpart in the reference, which transforms one
  * weight vector of one point set per call; these share the resident tables
  * and read them once for the whole batch).  At most 64 sets per call.
