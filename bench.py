@@ -581,7 +581,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2100)
     ap.add_argument("--skip-downstream", action="store_true",
                     help="skip the spectra / mode-coupling / 3j timings")
-    ap.add_argument("--only", choices=("weight_sets", "weight_sets_headline", "uneven", "serving",
+    ap.add_argument("--only", choices=("weight_sets", "weight_sets_headline",
+                                       "weight_sets_headline_16", "uneven", "serving",
                                        "downstream"),
                     help="run only this workload and print its JSON (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -604,6 +605,7 @@ def main():
     if args.only:
         fn = {"weight_sets": bench_weight_sets, "uneven": bench_uneven,
               "weight_sets_headline": lambda: bench_weight_sets(headline=True, S=64),
+              "weight_sets_headline_16": lambda: bench_weight_sets(headline=True, S=16),
               "downstream": bench_downstream,
               "serving": lambda: bench_serving(nreq=args.serve_requests,
                                                warm=min(100, args.serve_requests // 10))}[args.only]

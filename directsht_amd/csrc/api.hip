@@ -587,7 +587,11 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   if ((rc = mark(0))) return rc;
   if (a.shared) {
     const int64_t Np = a.Np[0];
-    pl.chunk = work_chunk(pl.chunk_max, Np);
+    // the weight-set scatter takes longer work items: each run (a stretch of
+    // one segment inside a work item) writes nsets partials that the fold
+    // reads back, and there are about one run per work item plus one per
+    // segment (fewer items: smaller capacities than the workspace holds)
+    pl.chunk = work_chunk(sets_scatter ? std::max(pl.chunk_max, kSetsChunk) : pl.chunk_max, Np);
     ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
     if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
     const bool gathers = scatter_gathers(pl);

@@ -201,6 +201,7 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
 // kSetsScatterMin sets): one scatter of all sets, set s's run partials at
 // part + s * part_set_stride(pl)
 constexpr int kSetsScatterMin = 2;
+constexpr int kSetsChunk = 16384;  // largest work item of the weight-set scatter
 inline bool uses_sets_scatter(const Plan& pl, int nsets) {
   return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX && nsets >= kSetsScatterMin;
 }

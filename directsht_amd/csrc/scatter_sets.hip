@@ -25,7 +25,11 @@
 // transpose to point-major rows and a gather of whole rows, which also finds
 // max |w_s|), so the scatter reads them coalesced.
 //
-// Workgroups: (chunk of sorted points, block of 64 orders), the order blocks
+// Two kernels: k_scatter_sets below (sets as rows, 33..64 sets) and
+// k_scatter_wsets further down (rows (k, set), up to 32 sets), chosen in
+// launch_scatter_sets by the set count on measured times.
+//
+// k_scatter_sets workgroups: (chunk of sorted points, block of 64 orders), the order blocks
 // of a chunk on one XCD.  One workgroup of 4 wavefronts, two per CU (each
 // wavefront's 128 accumulator registers use half of a SIMD's register file);
 // wavefront w owns 16 orders m = 64 my + 16 w + (0..15), i.e. four 32-column
@@ -439,6 +443,354 @@ __global__ __launch_bounds__(kThreads, 1) void k_scatter_sets(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Weight-set scatter with the Hermite index on the set side:
+//   C[(k, s)][(m, re|im)] = sum_p A[(k, s)][p] Z[p][(m, re|im)],
+//   A[(k, s)][p] = w_s(p) h_k(p) 2^-e_s,   Z[p][(m, c)] = (Re | Im) e^{-i m phi_p}.
+// A has 4 SP rows (SP = 8 RTA sets, padded), built and split into fp16 hi/lo
+// once per staged block in LDS and read by every wavefront; Z is built once
+// per block in LDS in FP32 and each wavefront splits only its own columns.
+// So the vector work per matrix-core product is ~3-4 instructions (the A
+// products and splits are shared by 128 orders, the Z chains by 4 SP rows),
+// against ~7 when each wavefront forms h_k z for its orders and splits every
+// weight itself.
+//
+// Workgroup: (chunk of sorted points, block of 128 orders), NW wavefronts;
+// wavefront w owns orders 128 my + 16 CT w + (0 .. 16 CT - 1) as CT 32-column
+// tiles, and all RTA row tiles: RTA CT accumulator tiles of 16 registers,
+// independent, so consecutive MFMAs never wait on each other.  Blocks of
+// kWBlk = 32 points: while block i is multiplied out, block i+1's A and Z are
+// built into the other buffer from registers loaded at the start of the
+// block; one barrier per block.
+// ---------------------------------------------------------------------------
+constexpr int kWBlk = 32;                 // points per staged block
+constexpr int kWPg = kWBlk / 8;           // 8-point groups (MFMA K slices) per block
+constexpr int kWOrders = 128;             // orders per workgroup
+// Z in LDS, FP32: value (point p, column col) at
+//   (2 pg + hf) * kZPg + (col + col / 32) * 4 + c4,   p = 8 pg + 4 hf + c4,
+// so a lane's 8 K values are two 16-byte reads, the reads of 32 consecutive
+// columns are contiguous, and the stores of 32 points at one column hit 32
+// different banks (kZPg = 4 mod 32).
+constexpr int kZPg = 1060;
+constexpr int kZBuf = 8 * kZPg;           // floats per buffer
+
+__device__ inline int zidx(int p, int col) {
+  return ((p >> 2) * kZPg) + (col + (col >> 5)) * 4 + (p & 3);
+}
+
+template <int RTA, int CT, int NW>
+__global__ __launch_bounds__(64 * NW, 1) void k_scatter_wsets(
+    const PointRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg, int chunk,
+    int Nl, const int* __restrict__ run_base, const float* __restrict__ wsf, int64_t ldws,
+    int nsets, const unsigned* __restrict__ smax, int nchunks, int nmy, float* __restrict__ part,
+    int64_t set_stride, int* __restrict__ run_seg) {
+  static_assert(16 * CT * NW == kWOrders, "the wavefronts cover 128 orders");
+  constexpr int kT = 64 * NW;           // threads
+  constexpr int SP = 8 * RTA;           // sets per row block (padded)
+  constexpr int NR = 4 * SP;            // A rows, r = k SP + s
+  constexpr int kElems = NR * kWPg;     // A elements (row, 8-point group) per block
+  constexpr int kPerThr = (kElems + kT - 1) / kT;
+  constexpr int kZG = kT / kWBlk;       // z chain groups per point
+  constexpr int kZO = kWOrders / kZG;   // orders per chain
+  constexpr int kRP = RTA >= 2 ? 2 : 1;  // row tiles per MFMA round
+  // workgroup b runs on XCD b % 8; the order blocks of one chunk go to one
+  // XCD, one after another, so they share its L2 for the chunk's records
+  // and weights
+  const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+  const int c = 8 * (slot / nmy) + xcd, my = slot % nmy;
+  if (c >= nchunks) return;
+  const int tid = threadIdx.x, l = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nvalid = seg_start[nseg];
+  const int b = c * chunk, e = min(nvalid, b + chunk);
+  if (b >= e) return;
+
+  __shared__ __attribute__((aligned(16))) half8 sA[2][2][kWPg][NR];  // [buf][hi|lo][pg][row]
+  __shared__ __attribute__((aligned(16))) float sZ[2][kZBuf];
+  __shared__ __attribute__((aligned(16))) float sH[2][4][kWBlk];     // h_k of a block's points
+  __shared__ float sScale[2][SP];                                    // 2^-e_s, 2^e_s
+
+  if (tid < SP) {
+    int ex = 0;
+    if (tid < nsets) {
+      const float mx = __uint_as_float(smax[tid]);
+      if (mx > 0.f && mx < 3.0e38f) (void)frexpf(mx, &ex);
+    }
+    sScale[0][tid] = ldexpf(1.0f, -ex);
+    sScale[1][tid] = ldexpf(1.0f, ex);
+  }
+
+  // ---- staging roles
+  // z chains: thread (point zp, group zg) makes orders kZO zg .. kZO zg + kZO - 1;
+  // h loads: thread (point zp, k = zg) for zg < 4;
+  // A elements: element q = tid + kT j (j < kPerThr): row q % NR, group q / NR
+  const int zp = tid & (kWBlk - 1), zg = tid / kWBlk;
+  double se_re = 1.0, se_im = 0.0;  // e^{-i phi} of point zp of the block being staged
+  float4 sw[kPerThr][2];            // weights of element j's 8 points (unscaled)
+  double pe_re = 1.0, pe_im = 0.0;  // the same, loaded a block earlier (prefetch)
+  float4 pw[kPerThr][2];
+  float ph = 0.f;                   // h_zg of point zp of a later block
+  // records and weights of the block at pblk into the prefetch registers
+  // (past e: zeros); take() moves them to the staging registers
+  auto load_stage = [&](int pblk) {
+    const int p = pblk + zp;
+    const bool ok = p < e;
+    const PointRec* pr = rec + (ok ? p : b);
+    pe_re = ok ? pr->ere : 1.0;
+    pe_im = ok ? pr->eim : 0.0;
+#pragma unroll
+    for (int j = 0; j < kPerThr; ++j) {
+      const int q = tid + kT * j;
+      const int row = q % NR, pg = q / NR;
+      const int s = row % SP;
+      float4 a = {0.f, 0.f, 0.f, 0.f}, bq = {0.f, 0.f, 0.f, 0.f};
+      if (q < kElems && s < nsets && pblk < e) {
+        // rows hold every position a block reads, zero past the valid points
+        const float4* wr = reinterpret_cast<const float4*>(wsf + s * ldws + pblk + 8 * pg);
+        a = wr[0];
+        bq = wr[1];
+      }
+      pw[j][0] = a;
+      pw[j][1] = bq;
+    }
+  };
+  auto take = [&]() {
+    se_re = pe_re;
+    se_im = pe_im;
+#pragma unroll
+    for (int j = 0; j < kPerThr; ++j) {
+      sw[j][0] = pw[j][0];
+      sw[j][1] = pw[j][1];
+    }
+  };
+  auto load_h = [&](int pblk) {
+    const int p = pblk + zp;
+    if (zg < 4) {
+      float v = 0.f;
+      if (p < e) {
+        const PointRec* pr = rec + p;
+        v = (float)(zg == 0 ? pr->h0 : zg == 1 ? pr->h1 : zg == 2 ? pr->h2 : pr->h3);
+      }
+      ph = v;
+    }
+  };
+  // A elements j0 .. j1-1 of the staged block into buffer nb (h from sH[nb])
+  auto stage_a = [&](int nb, int j0, int j1) {
+#pragma unroll
+    for (int j = j0; j < j1; ++j) {
+      const int q = tid + kT * j;
+      if (q < kElems) {
+        const int row = q % NR, pg = q / NR;
+        const int k = row / SP, s = row % SP;
+        const float sc = sScale[0][s];
+        const float4 h0 = *reinterpret_cast<const float4*>(&sH[nb][k][8 * pg]);
+        const float4 h1 = *reinterpret_cast<const float4*>(&sH[nb][k][8 * pg + 4]);
+        float x[8] = {sw[j][0].x * h0.x, sw[j][0].y * h0.y, sw[j][0].z * h0.z, sw[j][0].w * h0.w,
+                      sw[j][1].x * h1.x, sw[j][1].y * h1.y, sw[j][1].z * h1.z, sw[j][1].w * h1.w};
+        uint32_t hi[4], lo[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) split2_f16(x[2 * u] * sc, x[2 * u + 1] * sc, hi[u], lo[u]);
+        sA[nb][0][pg][row] = __builtin_bit_cast(half8, (u32x4){hi[0], hi[1], hi[2], hi[3]});
+        sA[nb][1][pg][row] = __builtin_bit_cast(half8, (u32x4){lo[0], lo[1], lo[2], lo[3]});
+      }
+    }
+  };
+  // z chain of the staged block: seed at order 128 my + kZO zg, then steps
+  cplx zst{1.0, 0.0};
+  auto z_seed = [&]() {
+    const cplx e1{se_re, se_im};
+    cplx z{1.0, 0.0}, pw2 = e1;
+    int nn = kWOrders * my + kZO * zg;
+    while (nn) {
+      if (nn & 1) z = cmul(z, pw2);
+      pw2 = csq(pw2);
+      nn >>= 1;
+    }
+    zst = z;
+  };
+  auto z_steps = [&](int nb, int q0, int q1) {
+    const cplx e1{se_re, se_im};
+    float* zb = sZ[nb];
+#pragma unroll
+    for (int q = q0; q < q1; ++q) {
+      const int zc = 2 * (kZO * zg + q);
+      zb[zidx(zp, zc)] = (float)zst.x;
+      zb[zidx(zp, zc + 1)] = (float)zst.y;
+      zst = cmul(zst, e1);
+    }
+  };
+
+  // ---- products
+  const int col = l & 31, hq = l >> 5;
+  floatx16 acc[RTA][CT];
+  const floatx16 zero = {};
+#pragma unroll
+  for (int rt = 0; rt < RTA; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = zero;
+
+  int run = __builtin_amdgcn_readfirstlane(run_base[c]);
+  int cur = __builtin_amdgcn_readfirstlane(rec[b].seg);
+  int run_end = min(e, seg_start[cur + 1]);
+
+  // lane l, register r of tile (rt, ct) holds C[row 32 rt + 8 (r/4) + 4 (l/32) + r%4]
+  // [column 32 ct + l%32]: order 128 my + 16 (CT wv + ct) + (l%32)/2, component l%2
+  // Row 32 rt + 8 (r/4) + 4 (l/32) + r%4 = k SP + s: per lane, s = s_lane +
+  // (32 rt) % SP + 8 (r/4) + r%4 with s_lane = 4 (l/32), and k = (32 rt +
+  // 8 (r/4)) / SP, so every store is this lane's base plus a uniform offset.
+  auto finish_run = [&](int p) {
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" ::: "memory");
+    int s_lane = 4 * hq;
+    asm volatile("" : "+v"(s_lane));  // keeps the per-lane base inside this branch
+    const float* scl = &sScale[1][s_lane];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int m = kWOrders * my + 16 * (CT * wv + ct) + (col >> 1);
+      if (m < Nl) {
+        float* q = part + ((int64_t)run * 4 * Nl + m) * 2 + (col & 1) + s_lane * set_stride;
+#pragma unroll
+        for (int rt = 0; rt < RTA; ++rt)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int row0 = 32 * rt + 8 * (r >> 2) + (r & 3);  // row less 4 (l/32)
+            const int k = row0 / SP, sc = row0 % SP;
+            if (sc + s_lane < nsets) {
+              float v;
+              asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(v) : "a"(acc[rt][ct][r]));
+              q[sc * set_stride + (int64_t)k * Nl * 2] = v * scl[sc];
+            }
+          }
+      }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RTA; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = zero;
+    if (tid == 0 && my == 0) run_seg[run] = cur;
+    if (p < e) {
+      ++run;
+      cur = __builtin_amdgcn_readfirstlane(rec[p].seg);
+      run_end = min(e, seg_start[cur + 1]);
+    }
+  };
+  auto pin_acc = [&]() {
+#pragma unroll
+    for (int rt = 0; rt < RTA; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) asm volatile("" : "+a"(acc[rt][ct]));
+  };
+  // one pass of window wdx of buffer nb over points [plo, phi_) of the
+  // current run (the whole window when not masked): Z split once, then the
+  // row tiles in rounds of kRP with their A slices loaded just before
+  auto mfma_pass = [&](int nb, int wdx, int blk, bool masked, int plo, int phi_) {
+    const int pg = 2 * wdx + hq;
+    half8 bh[CT], bl[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+      const int zc = 32 * (CT * wv + ct) + col;
+      const float* zr = &sZ[nb][(2 * pg) * kZPg + (zc + (zc >> 5)) * 4];
+      const float4 z0 = *reinterpret_cast<const float4*>(zr);
+      const float4 z1 = *reinterpret_cast<const float4*>(zr + kZPg);
+      float x[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
+      if (masked) {
+        const int rel = blk + 8 * pg - plo;
+        const unsigned len = (unsigned)(phi_ - plo);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = (unsigned)(rel + u) < len ? x[u] : 0.f;
+      }
+      split8_f16(x, bh[ct], bl[ct]);
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < RTA; r0 += kRP) {
+      half8 ah[kRP], al[kRP];
+#pragma unroll
+      for (int u = 0; u < kRP; ++u) {
+        ah[u] = sA[nb][0][pg][32 * (r0 + u) + col];
+        al[u] = sA[nb][1][pg][32 * (r0 + u) + col];
+      }
+#pragma unroll
+      for (int u = 0; u < kRP; ++u)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          acc[r0 + u][ct] =
+              __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[u], bh[ct], acc[r0 + u][ct], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < kRP; ++u)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          acc[r0 + u][ct] =
+              __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[u], bl[ct], acc[r0 + u][ct], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < kRP; ++u)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+          acc[r0 + u][ct] =
+              __builtin_amdgcn_mfma_f32_32x32x16_f16(al[u], bh[ct], acc[r0 + u][ct], 0, 0, 0);
+    }
+  };
+
+  // ---- prologue: block 0 staged into buffer 0; block 1's h in sH[1] and
+  // its records and weights in the staging registers; block 2's loading
+  load_stage(b);
+  load_h(b);
+  if (zg < 4) sH[0][zg][zp] = ph;
+  take();
+  load_stage(b + kWBlk);
+  __syncthreads();  // sH[0], sScale
+  stage_a(0, 0, kPerThr);
+  z_seed();
+  z_steps(0, 0, kZO);
+  load_h(b + kWBlk);
+  if (zg < 4) sH[1][zg][zp] = ph;
+  take();
+  load_stage(b + 2 * kWBlk);
+  __syncthreads();
+
+  int buf = 0;
+  for (int blk = b; blk < e; blk += kWBlk, buf ^= 1) {
+    const bool next = blk + kWBlk < e;  // uniform: stage the next block
+    const int bn = min(kWBlk, e - blk);
+    load_h(blk + 2 * kWBlk);
+#pragma unroll
+    for (int wdx = 0; wdx < 2; ++wdx) {
+      const int wlo = blk + 16 * wdx;
+      if (16 * wdx < bn) {  // uniform
+        // one pass per run the window meets (one for almost all windows),
+        // the points outside the run zeroed: a single code path keeps every
+        // accumulator in one register set
+        const int whi = blk + min(16 * wdx + 16, bn);
+        int p = wlo;
+        do {
+          const int vhi = min(whi, run_end);
+          mfma_pass(buf, wdx, blk, true, p, vhi);
+          p = vhi;
+          if (p == run_end) finish_run(p);
+          pin_acc();
+        } while (p < whi);
+      }
+      // half of the next block's staging after each window
+      if (next) {
+        if (wdx == 0) {
+          stage_a(buf ^ 1, 0, (kPerThr + 1) / 2);
+          z_seed();
+          z_steps(buf ^ 1, 0, kZO / 2);
+        } else {
+          stage_a(buf ^ 1, (kPerThr + 1) / 2, kPerThr);
+          z_steps(buf ^ 1, kZO / 2, kZO);
+        }
+      }
+      pin_acc();
+    }
+    // block blk + 2 kWBlk's h into the buffer block blk used (its A is
+    // built), its records and weights into the staging registers; block
+    // blk + 3 kWBlk's loads in flight during the next block
+    if (zg < 4) sH[buf][zg][zp] = ph;
+    take();
+    load_stage(blk + 3 * kWBlk);
+    __syncthreads();
+  }
+}
+
 int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
                         hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
@@ -473,10 +825,23 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
                                             ws.run_base, ws.wsf, ws.ldws, nsets, ws.wmax, nchunks,
                                             nmy, (float*)ws.part, stride, ws.run_seg);
   };
-  if (nsets > 32)
+  // up to 32 sets: the set-side layout (4 k x 8, 16 or 32 sets per row
+  // block, no padding to 32 sets, A shared by all orders); more: sets as
+  // rows, h_k z formed per wavefront (measured faster from 33 sets on)
+  if (nsets <= 32) {
+    const int nmy2 = (pl.Nl + kWOrders - 1) / kWOrders;
+    const int64_t nwg2 = (int64_t)8 * ((nchunks + 7) / 8) * nmy2;
+    auto go2 = [&](auto kern) {
+      kern<<<(unsigned)nwg2, 256, 0, s>>>(ws.rec, ws.seg_start, pl.nseg, pl.chunk, pl.Nl,
+                                          ws.run_base, ws.wsf, ws.ldws, nsets, ws.wmax, nchunks,
+                                          nmy2, (float*)ws.part, stride, ws.run_seg);
+    };
+    if (nsets > 16) go2(k_scatter_wsets<4, 2, 4>);
+    else if (nsets > 8) go2(k_scatter_wsets<2, 2, 4>);
+    else go2(k_scatter_wsets<1, 2, 4>);
+  } else {
     go(k_scatter_sets<2>);
-  else
-    go(k_scatter_sets<1>);
+  }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
