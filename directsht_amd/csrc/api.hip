@@ -76,17 +76,14 @@ struct WsSizes {
 };
 
 static WsSizes ws_sizes(int Nx, int64_t Np, int chunk) {
-  const int tile = sort_tile(Np);
   WsSizes z;
   z.points = std::max<int64_t>(Np, 1);
   // the most work items of any transform up to Np points (work_chunk)
   z.chunks = std::max<int64_t>((Np + chunk - 1) / chunk,
                                std::min<int64_t>((Np + kChunkMin - 1) / kChunkMin, kChunkTarget));
   z.runs = z.chunks + num_segments(Nx) + 1;
-  // tiles of the largest transform up to Np points: sort_tile() keeps every
-  // size at or below ~512 tiles, and sizes below 2^21 use ceil(Np / 4096)
-  z.tiles = std::max<int64_t>(1, std::max<int64_t>((Np + tile - 1) / tile,
-                                                   std::min<int64_t>((Np + 4095) / 4096, 512)));
+  // tiles of the largest transform up to Np points
+  z.tiles = sort_tiles_cap(Np);
   return z;
 }
 
@@ -102,6 +99,7 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk) {
   b += align256(z.runs * (int64_t)Nl * 8 * (precision == DSHT_FP32 ? 4 : 8));
   b += 6 * align256((nseg + 2) * sizeof(int));
   b += align256(z.tiles * (nseg + 1) * sizeof(int));
+  b += align256(z.points * (int64_t)sizeof(int2));
   b += align256((int64_t)Nl * 2 * Npair * 24 * (precision == DSHT_FP32 ? 4 : 8));
   b += 2 * align256(Nlm * 2 * sizeof(double));
   b += align256((int64_t)kContractSplits * Nlm * 8 * sizeof(double));
@@ -117,7 +115,7 @@ static void free_workspace(Workspace& ws) {
   for (void* p : {(void*)ws.rec, (void*)ws.run_count, (void*)ws.run_base, (void*)ws.run_seg,
                   ws.part, (void*)ws.seg_count, (void*)ws.seg_start, (void*)ws.seg_cursor,
                   (void*)ws.seg_run_lo, (void*)ws.seg_run_hi,
-                  (void*)ws.ne_rank, (void*)ws.thist, ws.rhs, (void*)ws.rmax, (void*)ws.wmax, (void*)ws.wsort,
+                  (void*)ws.ne_rank, (void*)ws.thist, (void*)ws.segrank, ws.rhs, (void*)ws.rmax, (void*)ws.wmax, (void*)ws.wsort,
                   (void*)ws.sidx, (void*)ws.wsf,
                   (void*)ws.a_equ, (void*)ws.a_pol, (void*)ws.a_tmp, (void*)ws.a_der, (void*)ws.flags})
     if (p) (void)hipFree(p);
@@ -172,6 +170,7 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
   get(ws.seg_run_hi, (pl.nseg + 2) * (int64_t)sizeof(int));
   get(ws.ne_rank, (pl.nseg + 2) * (int64_t)sizeof(int));
   get(ws.thist, want.tiles * (pl.nseg + 1) * (int64_t)sizeof(int));
+  get(ws.segrank, want.points * (int64_t)sizeof(int2));
   get(ws.rhs, rhs_elems * fsz);
   get(ws.rmax, sets * (int64_t)sizeof(unsigned));
   get(ws.wmax, sets * (int64_t)sizeof(unsigned));

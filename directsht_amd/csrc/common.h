@@ -74,15 +74,34 @@ __host__ __device__ inline int64_t hp_index(int ell, int m, int Nl) {
 // 1 = polar (rotated); hemisphere 0: x >= 0, 1: x < 0; bin j in [0, Nx-2].
 // Points that the reference drops (NaN theta) get seg = nseg (sorted last,
 // never scattered).
-__host__ __device__ // Input points per counting-sort tile: at least 4096, and no more than ~512
-// tiles (two 1024-thread tile blocks per CU fill the chip in one round).
-inline int sort_tile(int64_t Np) {
-  int64_t t = (Np + 511) / 512;
-  t = (t + 1023) / 1024 * 1024;
-  return (int)(t < 4096 ? 4096 : t);
+// Input points per counting-sort tile: one wavefront ranks a tile's points
+// in input order, so about kSortTiles tiles (a multiple of 256 points each,
+// at least 256) fill the device in one round.
+constexpr int kSortTiles = 1280;
+constexpr int kSortRound = 256;  // points a wavefront takes per step (4 per lane)
+__host__ __device__ inline int sort_tile(int64_t Np) {
+  int64_t t = (Np + kSortTiles - 1) / kSortTiles;
+  t = (t + kSortRound - 1) / kSortRound * kSortRound;
+  return (int)(t < kSortRound ? kSortRound : t);
+}
+// tile rows of the histogram matrix for any transform of up to Np points
+inline int64_t sort_tiles_cap(int64_t Np) {
+  const int64_t t = (Np + kSortRound - 1) / kSortRound;
+  return t < 1 ? 1 : (t < kSortTiles ? t : kSortTiles);
 }
 
 inline int num_segments(int Nx) { return 4 * (Nx - 1); }
+
+// *p = max(*p, v) for non-negative floats kept as their bits (which order
+// like the floats).  The value is read first and the atomic skipped when it
+// cannot raise it: atomics on one address serialize, and once the maximum is
+// in place most callers would only repeat it.  A stale read can only be
+// smaller than the current value, so no update is lost.
+__device__ inline void atomic_max_pos(unsigned* p, float v) {
+  if (!(v > 0.f)) return;
+  const unsigned cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  if (__float_as_uint(v) > cur) atomicMax(p, __float_as_uint(v));
+}
 
 // Per-point record after sorting: one 64-byte line per point, written once by
 // the placement pass and read by the scatter.

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
     if (threadIdx.x == 0) {
       float bm = wmax[0];
       for (int i = 1; i < kFoldNp; ++i) bm = fmaxf(bm, wmax[i]);
-      if (bm > 0.f) atomicMax(rmax, __float_as_uint(bm));
+      atomic_max_pos(rmax, bm);
     }
   }
 }
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_fold_sets(
     }
     __syncthreads();
   }
-  if (tid < nsets && smx[tid] > 0.f) atomicMax(rmax + tid, __float_as_uint(smx[tid]));
+  if (tid < nsets) atomic_max_pos(rmax + tid, smx[tid]);
 }
 
 static int fold_impl(Plan& pl, int nsets, int set0, int count, hipStream_t s, const void* part,

@@ -44,6 +44,8 @@ struct Workspace {
   int* seg_run_hi = nullptr;  // [nseg + 2]: one past its last run (0 if empty)
   int* ne_rank = nullptr;     // [nseg + 1]: non-empty segments before s
   int* thist = nullptr;       // [cap_tiles][nseg + 1]: tile histograms, then offsets
+  int2* segrank = nullptr;    // [2][cap_points] ints: each input point's segment, then its
+                              // rank among its tile's points of that segment (input order)
   void* rhs = nullptr;        // node coefficients (float or double): plain contraction
                               // [Nl][2][Npair][24] (one set at a time); matrix-core
                               // contraction and caller tables [Nl][2][Npair][cap_sets][16]

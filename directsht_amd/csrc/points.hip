@@ -9,11 +9,23 @@
 //   * hemisphere split x >= 0 / x < 0                     sht/sht.py:193
 //   * bin j = digitize(|x|, x_grid) - 1, t = (|x|-x_j)/dx sht/sht.py:212,221
 //   * Hermite weights w * {h0..h3}(t)                     sht/sht.py:239-241
-// The segment id (branch, hemisphere, bin) is counted into a histogram
-// (LDS-privatized, one global atomic per non-empty bucket per block), scanned,
-// and each point's record is placed into its segment's range of the sorted
-// arrays.  Points whose cos(theta) is NaN are dropped, as the reference's two
-// np.nonzero masks drop them.
+// The points are sorted by segment id (branch, hemisphere, bin) with a
+// counting sort whose order within a segment is the input order, so a
+// transform's sums -- and its FP32 rounding -- are the same on every call:
+//   k_segments:     each point's segment id (the FP64 classification);
+//   k_rank_hist:    one wavefront per tile of consecutive input points, in
+//                   input order, finds each point's rank among the tile's
+//                   points of its segment (same-segment lanes by ballots over
+//                   the segment's bits, running counts in the wavefront's own
+//                   LDS row: no atomics), and writes the tile's histogram row;
+//   k_col_sum, k_scan, k_col_offsets: segment starts and per-(tile, segment)
+//                   write offsets;
+//   k_place_ranked: every point's record at offset + rank (fully parallel).
+// Points whose cos(theta) is NaN are dropped, as the reference's two
+// np.nonzero masks drop them (sorted last, never scattered).  Grids with more
+// segments than a wavefront's LDS row holds (16-bit counts, Nx > 8192) fall
+// back to global atomics (k_hist, k_place), whose order within a segment is
+// not fixed.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -114,37 +126,24 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
   return c;
 }
 
-// Histogram of segment ids (+1 bucket for dropped points) and the status /
-// polar-count reductions.  flags[0] |= status bits, flags[1] += polar points.
-template <bool kLds>
+// Fallback histogram for grids with more segments than k_rank_hist's LDS row
+// holds: segment ids (+1 bucket for dropped points) counted with global
+// atomics, and the status / polar-count reductions (flags[0] |= status bits,
+// flags[1] += polar points).
 __global__ __launch_bounds__(256) void k_hist(int64_t Np, const double* __restrict__ theta,
-                                              const double* __restrict__ phi,
-                                              const double* __restrict__ wt, PointParams p,
-                                              int* __restrict__ seg_count, int* __restrict__ flags) {
-  extern __shared__ int lds_hist[];
-  const int nb = p.nseg + 1;
-  if (kLds) {
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) lds_hist[i] = 0;
-    __syncthreads();
-  }
+                                              const double* __restrict__ phi, PointParams p,
+                                              int* __restrict__ seg_count,
+                                              int* __restrict__ flags) {
   int flag = 0, npol = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
     Classified c = classify(theta[i], phi[i], 0.0, p);  // weights do not move points
     flag |= c.flag;
     npol += c.polar;
-    if (kLds) atomicAdd(&lds_hist[c.seg], 1);
-    else atomicAdd(&seg_count[c.seg], 1);
+    atomicAdd(&seg_count[c.seg], 1);
   }
   if (flag) atomicOr(&flags[0], flag);
   if (npol) atomicAdd(&flags[1], npol);
-  if (kLds) {
-    __syncthreads();
-    for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-      int v = lds_hist[i];
-      if (v) atomicAdd(&seg_count[i], v);
-    }
-  }
 }
 
 // Exclusive scans of n ints by one workgroup of 1024 threads: out[i] =
@@ -221,7 +220,7 @@ __device__ inline void block_wmax(float v, unsigned* __restrict__ wmax) {
   if (threadIdx.x == 0) {
     float b = red[0];
     for (int i = 1; i < (int)(blockDim.x >> 6); ++i) b = fmaxf(b, red[i]);
-    if (b > 0.f) atomicMax(wmax, __float_as_uint(b));
+    atomic_max_pos(wmax, b);
   }
 }
 
@@ -286,57 +285,109 @@ __global__ __launch_bounds__(256) void k_count_runs(const int* __restrict__ seg_
   run_count[c] = ne_rank[se] - ne_rank[sb] + 1;
 }
 
-// ---- tile-based counting sort (no global atomics on segment counters) ----
-// Points are cut into tiles of `tile` consecutive input points (sort_tile():
-// at most ~512 tiles, so the 1024-thread tile blocks fill the chip in one
-// round).  Pass 1 writes each tile's LDS histogram as a row of
-// thist[tile][seg]; the column pass turns the rows into per-(tile, segment)
-// write offsets (tiles in order, after the segment's start); pass 2
-// re-classifies the tile and places each record with an LDS atomic on the
-// tile's own offset row.  Each thread takes kSortBatch points per step so
-// their loads are in flight together.
-constexpr int kSortThreads = 1024;
-constexpr int kSortBatch = 4;
+// ---- counting sort with ranks in input order ----
+// Tiles of sort_tile(Np) consecutive input points (a multiple of 256), one
+// wavefront each, taking 4 x 64 points a step; the next step's coordinates
+// are loaded while the current one is ranked.
+constexpr int kRankLdsMax = 64 * 1024;  // the wavefront's count row (Nx <= 8192)
 
-__global__ __launch_bounds__(kSortThreads) void k_tile_hist(int64_t Np, int tile,
-                                                            const double* __restrict__ theta,
-                                                            const double* __restrict__ phi,
-                                                            PointParams p, int* __restrict__ thist,
-                                                            int* __restrict__ flags) {
-  extern __shared__ int lh[];
-  const int nb = p.nseg + 1;
-  for (int i = threadIdx.x; i < nb; i += kSortThreads) lh[i] = 0;
+// Segment id of every point (input order), and the status / polar-count
+// reductions (flags[0] |= status bits, flags[1] += polar points): the FP64
+// classification at full occupancy, so the ranking below is integer work.
+__global__ __launch_bounds__(256) void k_segments(int64_t Np, const double* __restrict__ theta,
+                                                  const double* __restrict__ phi, PointParams p,
+                                                  int* __restrict__ seg_of_pt,
+                                                  int* __restrict__ flags) {
+  __shared__ int red[2][4];
+  int flag = 0, npol = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const Classified c = classify(theta[i], phi[i], 0.0, p);  // weights do not move points
+    seg_of_pt[i] = c.seg;
+    flag |= c.flag;
+    npol += c.polar;
+  }
+  // one atomic per block (a few thousand blocks): same-address atomics serialize
+  for (int off = 32; off > 0; off >>= 1) {
+    flag |= __shfl_xor(flag, off);
+    npol += __shfl_xor(npol, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = flag;
+    red[1][threadIdx.x >> 6] = npol;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int f = red[0][0] | red[0][1] | red[0][2] | red[0][3];
+    const int n = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    if (f) atomicOr(&flags[0], f);
+    if (n) atomicAdd(&flags[1], n);
+  }
+}
+
+// One wavefront per tile: ranks in input order (4 x 64 points a step, the
+// next step's segment ids loaded while the current ones are ranked) and the
+// tile's histogram row.
+__global__ __launch_bounds__(64) void k_rank_hist(int64_t Np, int tile,
+                                                  const int* __restrict__ seg_of_pt, int nb,
+                                                  int nbits, int* __restrict__ thist,
+                                                  int* __restrict__ rank_of_pt) {
+  // counts below 2^16 (a tile holds fewer points): half the LDS of int
+  // counters, so twice as many tiles are resident at once
+  extern __shared__ unsigned short cnt[];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < nb; i += 64) cnt[i] = 0;
   __syncthreads();
   const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
-  int flag = 0, npol = 0;
-  for (int64_t i0 = b + threadIdx.x; i0 < e; i0 += kSortThreads * kSortBatch) {
-    double th[kSortBatch], ph[kSortBatch];
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  // segment ids two steps ahead; the loads are unconditional (index clamped
+  // into the tile, validity from the index) so that the compiler can count
+  // them and wait for the oldest only
+  int sg[4], n1[4], n2[4];
+  auto load = [&](int64_t base, int (&v)[4]) {
 #pragma unroll
-    for (int u = 0; u < kSortBatch; ++u) {
-      const int64_t i = i0 + (int64_t)u * kSortThreads;
-      th[u] = i < e ? theta[i] : 0.0;
-      ph[u] = i < e ? phi[i] : 0.0;
+    for (int j = 0; j < 4; ++j) v[j] = seg_of_pt[min(base + 64 * j + lane, e - 1)];
+  };
+  load(b, n1);
+  load(b + kSortRound, n2);
+  for (int64_t base = b; base < e; base += kSortRound) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sg[j] = base + 64 * j + lane < e ? n1[j] : -1;
+      n1[j] = n2[j];
     }
+    load(base + 2 * kSortRound, n2);
 #pragma unroll
-    for (int u = 0; u < kSortBatch; ++u) {
-      if (i0 + (int64_t)u * kSortThreads < e) {
-        const Classified c = classify(th[u], ph[u], 0.0, p);  // weights do not move points
-        flag |= c.flag;
-        npol += c.polar;
-        atomicAdd(&lh[c.seg], 1);
+    for (int j = 0; j < 4; ++j) {
+      const int s = sg[j];
+      const bool act = s >= 0;
+      // lanes holding the same segment: agree on every bit
+      uint64_t peers = __ballot(act);
+      for (int bt = 0; bt < nbits; ++bt) {
+        const bool bit = (s >> bt) & 1;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      if (act) {
+        const int rank = __popcll(peers & below);
+        // every lane reads its count before the lowest peer writes the new one
+        // (one wavefront: its LDS operations complete in program order)
+        const int old = cnt[s];
+        if (rank == 0) cnt[s] = (unsigned short)(old + __popcll(peers));
+        rank_of_pt[base + 64 * j + lane] = old + rank;
       }
     }
   }
-  if (flag) atomicOr(&flags[0], flag);
-  if (npol) atomicAdd(&flags[1], npol);
   __syncthreads();
   int* row = thist + (int64_t)blockIdx.x * nb;
-  for (int i = threadIdx.x; i < nb; i += kSortThreads) row[i] = lh[i];
+  for (int i = lane; i < nb; i += 64) row[i] = cnt[i];
 }
 
-// Column passes over thist: 16 segments per block, each column split over 16
-// threads (tile ranges), so no thread walks more than ntiles/16 rows.
-constexpr int kColSplit = 16;
+// Column passes over thist: 32 segments per block (a half-wavefront reads
+// 128 contiguous bytes of a row), each column split over 32 threads (tile
+// ranges of ntiles/32 rows, loads unrolled so they are in flight together).
+constexpr int kColSeg = 32;
+constexpr int kColSplit = 32;
 
 __device__ inline int col_partial(const int* __restrict__ thist, int ntiles, int nb, int s, int q,
                                   int& t0, int& t1) {
@@ -344,17 +395,20 @@ __device__ inline int col_partial(const int* __restrict__ thist, int ntiles, int
   t0 = min(ntiles, q * per);
   t1 = min(ntiles, t0 + per);
   int acc = 0;
-  if (s < nb)
+  if (s < nb) {
+#pragma unroll 8
     for (int t = t0; t < t1; ++t) acc += thist[(int64_t)t * nb + s];
+  }
   return acc;
 }
 
 // seg_count[s] = sum over tiles of thist[t][s]
-__global__ __launch_bounds__(256) void k_col_sum(const int* __restrict__ thist, int ntiles, int nb,
-                                                 int* __restrict__ seg_count) {
-  __shared__ int part[kColSplit][16];
-  const int sl = threadIdx.x % 16, q = threadIdx.x / 16;
-  const int s = blockIdx.x * 16 + sl;
+__global__ __launch_bounds__(kColSeg * kColSplit) void k_col_sum(const int* __restrict__ thist,
+                                                                 int ntiles, int nb,
+                                                                 int* __restrict__ seg_count) {
+  __shared__ int part[kColSplit][kColSeg + 1];
+  const int sl = threadIdx.x % kColSeg, q = threadIdx.x / kColSeg;
+  const int s = blockIdx.x * kColSeg + sl;
   int t0, t1;
   part[q][sl] = col_partial(thist, ntiles, nb, s, q, t0, t1);
   __syncthreads();
@@ -366,11 +420,12 @@ __global__ __launch_bounds__(256) void k_col_sum(const int* __restrict__ thist, 
 }
 
 // thist[t][s] <- seg_start[s] + sum_{t' < t} thist[t'][s]
-__global__ __launch_bounds__(256) void k_col_offsets(int* __restrict__ thist, int ntiles, int nb,
-                                                     const int* __restrict__ seg_start) {
-  __shared__ int part[kColSplit][16];
-  const int sl = threadIdx.x % 16, q = threadIdx.x / 16;
-  const int s = blockIdx.x * 16 + sl;
+__global__ __launch_bounds__(kColSeg * kColSplit) void k_col_offsets(int* __restrict__ thist,
+                                                                     int ntiles, int nb,
+                                                                     const int* __restrict__ seg_start) {
+  __shared__ int part[kColSplit][kColSeg + 1];
+  const int sl = threadIdx.x % kColSeg, q = threadIdx.x / kColSeg;
+  const int s = blockIdx.x * kColSeg + sl;
   int t0, t1;
   part[q][sl] = col_partial(thist, ntiles, nb, s, q, t0, t1);
   __syncthreads();
@@ -385,44 +440,35 @@ __global__ __launch_bounds__(256) void k_col_offsets(int* __restrict__ thist, in
   }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_tile_place(int64_t Np, int tile,
-                                                             const double* __restrict__ theta,
-                                                             const double* __restrict__ phi,
-                                                             const double* __restrict__ wt,
-                                                             PointParams p,
-                                                             const int* __restrict__ toff,
-                                                             PointRec* __restrict__ rec,
-                                                             unsigned* __restrict__ wmax) {
-  extern __shared__ int lo[];
+// Each point's record at its tile's offset for its segment plus its rank.
+// Grid (blocks per tile, tiles): block (x, t) takes the points
+// t tile + 256 x + (0 .. 255).
+__global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile,
+                                                      const double* __restrict__ theta,
+                                                      const double* __restrict__ phi,
+                                                      const double* __restrict__ wt, PointParams p,
+                                                      const int* __restrict__ toff,
+                                                      const int* __restrict__ seg_of_pt,
+                                                      const int* __restrict__ rank_of_pt,
+                                                      PointRec* __restrict__ rec,
+                                                      unsigned* __restrict__ wmax) {
   const int nb = p.nseg + 1;
-  const int* row = toff + (int64_t)blockIdx.x * nb;
-  for (int i = threadIdx.x; i < nb; i += kSortThreads) lo[i] = row[i];
-  __syncthreads();
-  const int64_t b = (int64_t)blockIdx.x * tile, e = min(Np, b + tile);
+  const int t = blockIdx.y;
+  const int64_t b = (int64_t)t * tile, e = min(Np, b + tile);
+  const int64_t i = b + (int64_t)blockIdx.x * 256 + threadIdx.x;
   float vmax = 0.f;
-  for (int64_t i0 = b + threadIdx.x; i0 < e; i0 += kSortThreads * kSortBatch) {
-    double th[kSortBatch], ph[kSortBatch], ww[kSortBatch];
-#pragma unroll
-    for (int u = 0; u < kSortBatch; ++u) {
-      const int64_t i = i0 + (int64_t)u * kSortThreads;
-      th[u] = i < e ? theta[i] : 0.0;
-      ph[u] = i < e ? phi[i] : 0.0;
-      ww[u] = i < e ? (wt ? wt[i] : 1.0) : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kSortBatch; ++u) {
-      if (i0 + (int64_t)u * kSortThreads < e) {
-        const Classified c = classify(th[u], ph[u], ww[u], p);
-        vmax = fmaxf(vmax, abs_w(ww[u], c.seg, p.nseg));
-        const int pos = atomicAdd(&lo[c.seg], 1);
-        double2* d = reinterpret_cast<double2*>(rec + pos);
-        d[0] = make_double2(c.ere, c.eim);
-        d[1] = make_double2(c.h0, c.h1);
-        d[2] = make_double2(c.h2, c.h3);
-        reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)(i0 + (int64_t)u * kSortThreads));
-        if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
-      }
-    }
+  if (i < e) {
+    const double wi = wt ? wt[i] : 1.0;
+    const int2 sr = make_int2(seg_of_pt[i], rank_of_pt[i]);
+    const Classified c = classify(theta[i], phi[i], wi, p);
+    vmax = abs_w(wi, sr.x, p.nseg);
+    const int pos = toff[(int64_t)t * nb + sr.x] + sr.y;
+    double2* d = reinterpret_cast<double2*>(rec + pos);
+    d[0] = make_double2(c.ere, c.eim);
+    d[1] = make_double2(c.h0, c.h1);
+    d[2] = make_double2(c.h2, c.h3);
+    reinterpret_cast<int2*>(d + 3)[0] = make_int2(sr.x, (int)i);
+    if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
   }
   block_wmax(vmax, wmax);
 }
@@ -462,22 +508,27 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
   k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, flags, 4, ws.wmax);
   DSHT_LAUNCH_CHECK();
   const int nb = pl.nseg + 1;
-  const size_t lds = sizeof(int) * nb;
+  const size_t lds = sizeof(unsigned short) * nb;
   const int tile = sort_tile(Np);
   const int ntiles = (int)((Np + tile - 1) / tile);
-  const bool tiled = lds <= 64 * 1024 && ntiles <= ws.cap_tiles;
+  const bool ranked = lds <= (size_t)kRankLdsMax && ntiles <= ws.cap_tiles;
+  int nbits = 0;
+  while ((1 << nbits) < nb) ++nbits;
   const int threads = 256;
   const int64_t want = (Np + threads - 1) / threads;
   const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, 256 * 16));
   if (Np > 0) {
-    if (tiled) {
-      k_tile_hist<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, p, ws.thist, flags);
+    if (ranked) {
+      int* seg_of_pt = reinterpret_cast<int*>(ws.segrank);
+      int* rank_of_pt = seg_of_pt + ws.cap_points;
+      k_segments<<<blocks, threads, 0, s>>>(Np, theta, phi, p, seg_of_pt, flags);
       DSHT_LAUNCH_CHECK();
-      k_col_sum<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_count);
-    } else if (lds <= 64 * 1024) {
-      k_hist<true><<<blocks, threads, lds, s>>>(Np, theta, phi, wt, p, ws.seg_count, flags);
+      k_rank_hist<<<ntiles, 64, lds, s>>>(Np, tile, seg_of_pt, nb, nbits, ws.thist, rank_of_pt);
+      DSHT_LAUNCH_CHECK();
+      k_col_sum<<<(nb + kColSeg - 1) / kColSeg, kColSeg * kColSplit, 0, s>>>(ws.thist, ntiles, nb,
+                                                                            ws.seg_count);
     } else {
-      k_hist<false><<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_count, flags);
+      k_hist<<<blocks, threads, 0, s>>>(Np, theta, phi, p, ws.seg_count, flags);
     }
     DSHT_LAUNCH_CHECK();
   }
@@ -487,11 +538,14 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
                             pl.nseg);
   DSHT_LAUNCH_CHECK();
   if (Np > 0) {
-    if (tiled) {
-      k_col_offsets<<<(nb + 15) / 16, 256, 0, s>>>(ws.thist, ntiles, nb, ws.seg_start);
+    if (ranked) {
+      k_col_offsets<<<(nb + kColSeg - 1) / kColSeg, kColSeg * kColSplit, 0, s>>>(
+          ws.thist, ntiles, nb, ws.seg_start);
       DSHT_LAUNCH_CHECK();
-      k_tile_place<<<ntiles, kSortThreads, lds, s>>>(Np, tile, theta, phi, wt, p, ws.thist, ws.rec,
-                                                     ws.wmax);
+      const int* seg_of_pt = reinterpret_cast<const int*>(ws.segrank);
+      k_place_ranked<<<dim3((tile + 255) / 256, ntiles), 256, 0, s>>>(
+          Np, tile, theta, phi, wt, p, ws.thist, seg_of_pt, seg_of_pt + ws.cap_points, ws.rec,
+          ws.wmax);
     } else {
       k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec, ws.wmax);
     }

@@ -520,7 +520,7 @@ __global__ __launch_bounds__(256) void k_absmax(int64_t n, const double* __restr
   __syncthreads();
   if (threadIdx.x == 0) {
     const float b = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    if (b > 0.f) atomicMax(wmax, __float_as_uint(b));
+    atomic_max_pos(wmax, b);
   }
 }
 

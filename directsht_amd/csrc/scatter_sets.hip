@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gather_sets(const float* __restrict__ w
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
     const int sl = hi + 4 * r;
-    if (lo == 0 && sl < nsets) atomicMax(smax + sl, __float_as_uint(m));
+    if (lo == 0 && sl < nsets) atomic_max_pos(smax + sl, m);
   }
 }
 

@@ -18,8 +18,9 @@ limit against resident tables:
     request's latency is submit -> alm on the host.
 
 Each request's result does not depend on what else is in flight: every
-transform has its own workspace and the kernels are deterministic given the
-sorted order (see the serving tests).
+transform has its own workspace, the points are sorted in input order within
+each segment, and every reduction runs in a fixed order, so a request gives
+the same bits alone and in the mix (a serving test checks this).
 """
 import collections
 import concurrent.futures as cf

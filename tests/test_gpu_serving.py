@@ -123,7 +123,11 @@ def test_server_alone_vs_mix(torch_cuda):
         res = [f.result(timeout=120) for f in futs]
         st = srv.stats()
     for i, (a, r) in enumerate(zip(res, ref)):
-        assert rel(a, r) < TOL["float32"], i
+        # the sort order is the input order within each segment and every
+        # reduction runs in a fixed order: the same bits alone and in the mix
+        a = a.cpu().numpy() if hasattr(a, "cpu") else np.asarray(a)
+        r = r.cpu().numpy() if hasattr(r, "cpu") else np.asarray(r)
+        assert np.array_equal(a, r), (i, rel(a, r))
     assert st["requests"] == n_req and st["p99_ms"] >= st["p50_ms"] > 0
 
 
