@@ -248,8 +248,8 @@ __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restr
     d[0] = make_double2(c.ere, c.eim);
     d[1] = make_double2(c.h0, c.h1);
     d[2] = make_double2(c.h2, c.h3);
-    reinterpret_cast<int2*>(d + 3)[0] = make_int2(c.seg, (int)i);
-    if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
+    const double uu = p.want_u ? c.u : 0.0;
+    reinterpret_cast<int4*>(d + 3)[0] = make_int4(c.seg, (int)i, __double2loint(uu), __double2hiint(uu));
   }
   block_wmax(vmax, wmax);
 }
@@ -442,7 +442,10 @@ __global__ __launch_bounds__(kColSeg * kColSplit) void k_col_offsets(int* __rest
 
 // Each point's record at its tile's offset for its segment plus its rank.
 // Grid (blocks per tile, tiles): block (x, t) takes the points
-// t tile + 256 x + (0 .. 255).
+// t tile + 1024 x + 256 u + (0 .. 255), u < 4: each thread's four points'
+// loads are issued together, so more of the scattered record stores overlap
+// the reads and the FP64 classification.
+constexpr int kPlacePer = 4;
 __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile,
                                                       const double* __restrict__ theta,
                                                       const double* __restrict__ phi,
@@ -455,20 +458,36 @@ __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile,
   const int nb = p.nseg + 1;
   const int t = blockIdx.y;
   const int64_t b = (int64_t)t * tile, e = min(Np, b + tile);
-  const int64_t i = b + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i0 = b + (int64_t)blockIdx.x * (256 * kPlacePer) + threadIdx.x;
+  double th[kPlacePer], ph[kPlacePer], ww[kPlacePer];
+  int sg[kPlacePer], rk[kPlacePer];
+#pragma unroll
+  for (int u = 0; u < kPlacePer; ++u) {
+    const int64_t i = min(i0 + 256 * u, e - 1);  // clamped: loads are unconditional
+    th[u] = theta[i];
+    ph[u] = phi[i];
+    ww[u] = wt ? wt[i] : 1.0;
+    sg[u] = seg_of_pt[i];
+    rk[u] = rank_of_pt[i];
+  }
   float vmax = 0.f;
-  if (i < e) {
-    const double wi = wt ? wt[i] : 1.0;
-    const int2 sr = make_int2(seg_of_pt[i], rank_of_pt[i]);
-    const Classified c = classify(theta[i], phi[i], wi, p);
-    vmax = abs_w(wi, sr.x, p.nseg);
-    const int pos = toff[(int64_t)t * nb + sr.x] + sr.y;
-    double2* d = reinterpret_cast<double2*>(rec + pos);
-    d[0] = make_double2(c.ere, c.eim);
-    d[1] = make_double2(c.h0, c.h1);
-    d[2] = make_double2(c.h2, c.h3);
-    reinterpret_cast<int2*>(d + 3)[0] = make_int2(sr.x, (int)i);
-    if (p.want_u) reinterpret_cast<double*>(d + 3)[1] = c.u;
+#pragma unroll
+  for (int u = 0; u < kPlacePer; ++u) {
+    const int64_t i = i0 + 256 * u;
+    if (i < e) {
+      const Classified c = classify(th[u], ph[u], ww[u], p);
+      vmax = fmaxf(vmax, abs_w(ww[u], sg[u], p.nseg));
+      const int pos = toff[(int64_t)t * nb + sg[u]] + rk[u];
+      // the whole 64-byte line, in 16-byte stores (a partly written line
+      // costs a read-modify-write)
+      double2* d = reinterpret_cast<double2*>(rec + pos);
+      d[0] = make_double2(c.ere, c.eim);
+      d[1] = make_double2(c.h0, c.h1);
+      d[2] = make_double2(c.h2, c.h3);
+      const double uu = p.want_u ? c.u : 0.0;
+      reinterpret_cast<int4*>(d + 3)[0] =
+          make_int4(sg[u], (int)i, __double2loint(uu), __double2hiint(uu));
+    }
   }
   block_wmax(vmax, wmax);
 }
@@ -543,7 +562,7 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
           ws.thist, ntiles, nb, ws.seg_start);
       DSHT_LAUNCH_CHECK();
       const int* seg_of_pt = reinterpret_cast<const int*>(ws.segrank);
-      k_place_ranked<<<dim3((tile + 255) / 256, ntiles), 256, 0, s>>>(
+      k_place_ranked<<<dim3((tile + 256 * kPlacePer - 1) / (256 * kPlacePer), ntiles), 256, 0, s>>>(
           Np, tile, theta, phi, wt, p, ws.thist, seg_of_pt, seg_of_pt + ws.cap_points, ws.rec,
           ws.wmax);
     } else {
