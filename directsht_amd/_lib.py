@@ -36,6 +36,7 @@ SIGNATURES = {
     "dsht_plan_destroy": (ct.c_int, [_vp]),
     "dsht_plan_clone": (ct.c_int, [_vp, ct.POINTER(_vp)]),
     "dsht_plan_reserve": (ct.c_int, [_vp, ct.c_int64, ct.c_int]),
+    "dsht_plan_reserve_for": (ct.c_int, [_vp, ct.c_int64, ct.c_int, ct.c_int]),
     "dsht_plan_footprint": (ct.c_int, [_vp, ct.POINTER(ct.c_int64)]),
     "dsht_plan_estimate": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int64,
                                       ct.POINTER(ct.c_int64), ct.POINTER(ct.c_int64)]),

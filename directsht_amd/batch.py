@@ -173,7 +173,7 @@ class BatchTransform:
                 self.ws[wk][1] = max(self.ws[wk][1], max(self.sets[i][0].numel() for i in idx))
                 self.ws[wk][2] = max(self.ws[wk][2], len(idx))
         for sht, n, k in self.ws.values():
-            sht.reserve(n, k)
+            sht.reserve(n, k, weight_sets=False)
         self.call_out = [torch.empty((len(idx), key[0] * (key[0] + 1) // 2), dtype=torch.complex128,
                                      device=self.device) for key, idx in self.calls]
         self.call_status = [torch.zeros(len(idx), dtype=torch.int32, device=self.device)

@@ -147,10 +147,13 @@ static int64_t rhs_per_pair(const Plan& pl, int nsets) {
 
 // Lay out every workspace array of a transform with capacities `want`
 // through alloc(ptr, bytes) (device allocation, or carving from a fixed
-// buffer), and record the capacities.
+// buffer), and record the capacities.  psets > 1: weight-set slots of
+// slot.points points and slot.runs runs each.
 template <class Alloc>
 static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, int sets,
-                            int psets, bool with_w, int64_t rhs_elems, Alloc&& alloc) {
+                            int psets, bool with_w, int64_t rhs_elems, Alloc&& alloc,
+                            int segsets = 1, WsSizes slot = WsSizes{0, 0, 0, 0}) {
+  if (psets <= 1) slot = WsSizes{0, 0, 0, 0};
   const int fsz = pl.precision == DSHT_FP32 ? 4 : 8;
   int rc = DSHT_OK;
   auto get = [&](auto*& ptr, int64_t nbytes) {
@@ -162,22 +165,22 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
   get(ws.run_count, (want.chunks + 1) * (int64_t)sizeof(int));
   get(ws.run_base, (want.chunks + 1) * (int64_t)sizeof(int));
   get(ws.run_seg, want.runs * (int64_t)sizeof(int));
-  get(ws.part, psets * want.runs * (int64_t)pl.Nl * 8 * fsz);
-  get(ws.seg_count, (pl.nseg + 2) * (int64_t)sizeof(int));
-  get(ws.seg_start, (pl.nseg + 2) * (int64_t)sizeof(int));
-  get(ws.seg_cursor, (pl.nseg + 2) * (int64_t)sizeof(int));
-  get(ws.seg_run_lo, (pl.nseg + 2) * (int64_t)sizeof(int));
-  get(ws.seg_run_hi, (pl.nseg + 2) * (int64_t)sizeof(int));
-  get(ws.ne_rank, (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.part, std::max(want.runs, psets * slot.runs) * (int64_t)pl.Nl * 8 * fsz);
+  get(ws.seg_count, (int64_t)segsets * (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_start, (int64_t)segsets * (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_cursor, (int64_t)segsets * (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_run_lo, (int64_t)segsets * (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.seg_run_hi, (int64_t)segsets * (pl.nseg + 2) * (int64_t)sizeof(int));
+  get(ws.ne_rank, (int64_t)segsets * (pl.nseg + 2) * (int64_t)sizeof(int));
   get(ws.thist, want.tiles * (pl.nseg + 1) * (int64_t)sizeof(int));
   get(ws.segrank, want.points * (int64_t)sizeof(int2));
   get(ws.rhs, rhs_elems * fsz);
   get(ws.rmax, sets * (int64_t)sizeof(unsigned));
   get(ws.wmax, sets * (int64_t)sizeof(unsigned));
   if (psets > 1) {
-    ws.ldws = (want.points + 63) / 64 * 64 + 64;
+    ws.ldws = (slot.points + 63) / 64 * 64 + 64;
     get(ws.sidx, want.points * (int64_t)sizeof(int));
-    // sorted rows [psets][ldws], then the point-major copy [cap_points][psets]
+    // sorted rows [psets][ldws], then the point-major copy [slot.points][psets]
     get(ws.wsf, 2 * psets * ws.ldws * (int64_t)sizeof(float));
   }
   if (with_w) get(ws.wsort, want.points * (int64_t)sizeof(double));
@@ -193,19 +196,26 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
   ws.cap_tiles = want.tiles;
   ws.cap_sets = sets;
   ws.cap_part_sets = psets;
+  ws.cap_set_points = slot.points;
+  ws.cap_set_runs = slot.runs;
   ws.has_wsort = with_w;
   ws.rhs_elems = rhs_elems;
+  ws.cap_segsets = segsets;
   return DSHT_OK;
 }
 
-static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false,
-                            int part_sets = 1) {
-  const WsSizes z = ws_sizes(pl.Nx, Np, pl.chunk_max);
+// Grow the workspace to capacities z (and the other counts) at least;
+// part_sets > 1: as many weight-set slots, each for a transform within z.
+static int ensure_workspace_z(Plan& pl, const WsSizes& z, int nsets, bool wsort, int part_sets,
+                              int segsets) {
   Workspace& ws = pl.ws;
   const int64_t rhs_need = (int64_t)pl.Nl * 2 * pl.Npair * rhs_per_pair(pl, nsets);
+  const bool slots_ok =
+      part_sets <= 1 || (z.points <= ws.cap_set_points && z.runs <= ws.cap_set_runs);
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
       z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort) &&
-      rhs_need <= ws.rhs_elems && part_sets <= ws.cap_part_sets)
+      rhs_need <= ws.rhs_elems && part_sets <= ws.cap_part_sets && segsets <= ws.cap_segsets &&
+      slots_ok)
     return DSHT_OK;
   if (ws.fixed) {
     set_error("dsht: transform larger than the fixed workspace it was given");
@@ -218,6 +228,12 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
   const int psets = std::max(part_sets, ws.cap_part_sets);
   const bool with_w = wsort || ws.has_wsort;
   const int64_t rhs_elems = std::max(rhs_need, ws.rhs_elems);
+  const int ssets = std::max(segsets, ws.cap_segsets);
+  WsSizes slot{ws.cap_set_points, 0, ws.cap_set_runs, 0};
+  if (part_sets > 1) {
+    slot.points = std::max(slot.points, z.points);
+    slot.runs = std::max(slot.runs, z.runs);
+  }
   free_workspace(ws);
   auto dev_alloc = [&](void** q, int64_t nbytes) -> int {
     const size_t n = (size_t)std::max<int64_t>(nbytes, 16);
@@ -230,7 +246,50 @@ static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = fa
     ws.bytes += nbytes;
     return DSHT_OK;
   };
-  return layout_workspace(pl, ws, want, sets, psets, with_w, rhs_elems, dev_alloc);
+  return layout_workspace(pl, ws, want, sets, psets, with_w, rhs_elems, dev_alloc, ssets, slot);
+}
+
+static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false,
+                            int part_sets = 1) {
+  return ensure_workspace_z(pl, ws_sizes(pl.Nx, Np, pl.chunk_max), nsets, wsort, part_sets, 1);
+}
+
+// Point-set batches (dsht_transform_many) on the matrix-core engines run in
+// groups of consecutive sets sorted, scattered and folded together: at most
+// kManyGroup sets holding at most kManyPoints points (a larger set forms a
+// group of its own), and few enough sets that their run partials -- besides
+// the work items, up to nseg + 2 runs per set -- stay within kManyPartBytes.
+constexpr int64_t kManyPoints = (int64_t)1 << 23;
+constexpr int64_t kManyPartBytes = (int64_t)4 << 30;
+constexpr size_t kManyLdsMax = 64 * 1024;  // the rank pass's 16-bit count row
+
+static int many_group_cap(const Plan& pl) {
+  const int64_t per_set = (int64_t)(pl.nseg + 2) * pl.Nl * 8 * sizeof(float);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(kManyGroup, kManyPartBytes / per_set));
+}
+
+static bool uses_many(const Plan& pl, int nsets, int mode) {
+  return use_mfma_contract(pl, nsets) && pl.scatter == DSHT_SCATTER_MATRIX &&
+         mode != DSHT_MODE_CSHT && sizeof(unsigned short) * (pl.nseg + 1) <= kManyLdsMax &&
+         many_group_cap(pl) > 1;
+}
+
+// capacities that hold every group of any batch of nsets sets of up to Np
+// points each: a group holds at most T points, its work items are of
+// work_chunk(chunk_max, group points) points and its tiles of
+// sort_tile(group points), each set adding at most one partial item and
+// tile, and a set of n points has at most min(n, nseg + 2) runs beyond its
+// work items
+static WsSizes many_sizes(const Plan& pl, int64_t Np, int nsets) {
+  const int g = std::min(nsets, many_group_cap(pl));
+  const int64_t T = std::max<int64_t>(Np, std::min<int64_t>((int64_t)g * Np, kManyPoints));
+  const WsSizes one = ws_sizes(pl.Nx, T, pl.chunk_max);
+  WsSizes z;
+  z.points = one.points;
+  z.tiles = sort_tiles_cap(T) + g;
+  z.chunks = one.chunks + g;
+  z.runs = z.chunks + std::min<int64_t>(T, (int64_t)g * (pl.nseg + 2));
+  return z;
 }
 
 // A workspace for one single-set transform of Np points carved from the
@@ -564,8 +623,48 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     maxNp = std::max(maxNp, n);
   }
   const bool sets_scatter = a.shared && uses_sets_scatter(pl, a.nsets);
-  int rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl),
-                            sets_scatter ? a.nsets : 1);
+  const bool many = !a.shared && uses_many(pl, a.nsets, mode);
+  std::vector<ManySets> groups;
+  int rc = DSHT_OK;
+  if (many) {
+    WsSizes z{1, 1, 1, 1};
+    int maxg = 1;
+    const int cap = many_group_cap(pl);
+    for (int g0 = 0; g0 < a.nsets;) {
+      ManySets ms{};
+      int64_t tot = a.Np[g0];
+      int g = 1;
+      while (g0 + g < a.nsets && g < cap && tot + a.Np[g0 + g] <= kManyPoints) tot += a.Np[g0 + g++];
+      ms.nsets = g;
+      const int tile = sort_tile(tot);
+      const int chunk = work_chunk(pl.chunk_max, tot);
+      int64_t runs = 0;
+      for (int k = 0; k < g; ++k) {
+        const int n = (int)a.Np[g0 + k];
+        ms.th[k] = a.theta[g0 + k];
+        ms.ph[k] = a.phi[g0 + k];
+        ms.w[k] = a.wt[g0 + k];
+        ms.n[k] = n;
+        ms.tile[k] = tile;
+        ms.chunk[k] = chunk;
+        ms.off[k + 1] = ms.off[k] + n;
+        ms.tile_off[k + 1] = ms.tile_off[k] + (n + tile - 1) / tile;
+        ms.chunk_off[k + 1] = ms.chunk_off[k] + (n + chunk - 1) / chunk;
+        runs += std::min(n, pl.nseg + 2);
+      }
+      z.points = std::max<int64_t>(z.points, ms.off[g]);
+      z.tiles = std::max<int64_t>(z.tiles, ms.tile_off[g]);
+      z.chunks = std::max<int64_t>(z.chunks, ms.chunk_off[g]);
+      z.runs = std::max<int64_t>(z.runs, ms.chunk_off[g] + runs);
+      maxg = std::max(maxg, g);
+      groups.push_back(ms);
+      g0 += g;
+    }
+    rc = ensure_workspace_z(pl, z, a.nsets, false, 1, maxg);
+  } else {
+    rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl),
+                          sets_scatter ? a.nsets : 1);
+  }
   if (rc) return rc;
   const bool mfma = use_mfma_contract(pl, a.nsets);
   const float* full32 = nullptr;
@@ -616,6 +715,14 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       }
       if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
       if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
+    }
+  } else if (many) {
+    int g0 = 0;
+    for (const ManySets& ms : groups) {
+      if ((rc = launch_points_many(pl, ms, mode, ws.flags + 4 * g0, ws.wmax + g0, s))) return rc;
+      if ((rc = launch_scatter_mfma_many(pl, ms, ws.wmax + g0, s))) return rc;
+      if ((rc = launch_fold_many(pl, ms, g0, a.nsets, s))) return rc;
+      g0 += ms.nsets;
     }
   } else {
     for (int set = 0; set < a.nsets; ++set) {
@@ -875,19 +982,35 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   return DSHT_OK;
 }
 
-int dsht_plan_reserve(dsht_plan* p, int64_t Np, int nsets) {
-  if (!p || Np < 0 || Np >= (int64_t)1 << 31 || nsets < 1 || nsets > kMaxSets) {
-    set_error("dsht_plan_reserve: null plan, Np outside [0, 2^31) or nsets outside [1, 64]");
+int dsht_plan_reserve_for(dsht_plan* p, int64_t Np, int nsets, int kinds) {
+  if (!p || Np < 0 || Np >= (int64_t)1 << 31 || nsets < 1 || nsets > kMaxSets ||
+      kinds < 1 || kinds > (DSHT_RESERVE_POINT_SETS | DSHT_RESERVE_WEIGHT_SETS)) {
+    set_error("dsht_plan_reserve: null plan, Np outside [0, 2^31), nsets outside [1, 64] "
+              "or kinds not a non-empty combination of DSHT_RESERVE_*");
     return DSHT_EINVAL;
   }
   DeviceGuard g(p->pl.device);
-  int rc = ensure_workspace(p->pl, Np, nsets, nsets > 1 && !scatter_gathers(p->pl),
-                            uses_sets_scatter(p->pl, nsets) ? nsets : 1);
+  Plan& pl = p->pl;
+  int rc = DSHT_OK;
+  if (kinds & DSHT_RESERVE_WEIGHT_SETS)
+    rc = ensure_workspace(pl, Np, nsets, nsets > 1 && !scatter_gathers(pl),
+                          uses_sets_scatter(pl, nsets) ? nsets : 1);
+  if (rc == DSHT_OK && (kinds & DSHT_RESERVE_POINT_SETS)) {
+    rc = ensure_workspace(pl, Np, nsets);
+    // point-set batches (dsht_transform_many) sort groups of sets together
+    if (rc == DSHT_OK && uses_many(pl, nsets, DSHT_MODE_CALL))
+      rc = ensure_workspace_z(pl, many_sizes(pl, Np, nsets), nsets, false, 1,
+                              std::min(nsets, many_group_cap(pl)));
+  }
   if (rc == DSHT_OK && use_mfma_contract(p->pl, nsets)) {
     const float* full32 = nullptr;
     rc = ensure_full32(p->pl, &full32);  // so that later transforms allocate nothing
   }
   return rc;
+}
+
+int dsht_plan_reserve(dsht_plan* p, int64_t Np, int nsets) {
+  return dsht_plan_reserve_for(p, Np, nsets, DSHT_RESERVE_POINT_SETS | DSHT_RESERVE_WEIGHT_SETS);
 }
 
 int dsht_plan_footprint(const dsht_plan* p, int64_t* bytes) {

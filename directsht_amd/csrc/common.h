@@ -76,18 +76,24 @@ __host__ __device__ inline int64_t hp_index(int ell, int m, int Nl) {
 // never scattered).
 // Input points per counting-sort tile: one wavefront ranks a tile's points
 // in input order, so about kSortTiles tiles (a multiple of 256 points each,
-// at least 256) fill the device in one round.
+// at least 256) fill the device in one round.  The ranks count in 16 bits,
+// so a tile holds at most kSortTileMax points: transforms of more than
+// kSortTiles * kSortTileMax (about 8.4e7) points take more tiles.
 constexpr int kSortTiles = 1280;
 constexpr int kSortRound = 256;  // points a wavefront takes per step (4 per lane)
+constexpr int kSortTileMax = 65280;  // largest multiple of kSortRound below 2^16
 __host__ __device__ inline int sort_tile(int64_t Np) {
   int64_t t = (Np + kSortTiles - 1) / kSortTiles;
   t = (t + kSortRound - 1) / kSortRound * kSortRound;
-  return (int)(t < kSortRound ? kSortRound : t);
+  t = t < kSortRound ? kSortRound : t;
+  return (int)(t > kSortTileMax ? kSortTileMax : t);
 }
 // tile rows of the histogram matrix for any transform of up to Np points
 inline int64_t sort_tiles_cap(int64_t Np) {
   const int64_t t = (Np + kSortRound - 1) / kSortRound;
-  return t < 1 ? 1 : (t < kSortTiles ? t : kSortTiles);
+  const int64_t big = (Np + kSortTileMax - 1) / kSortTileMax;
+  const int64_t c = t < kSortTiles ? t : kSortTiles;
+  return c < 1 ? 1 : (c > big ? c : big);
 }
 
 inline int num_segments(int Nx) { return 4 * (Nx - 1); }

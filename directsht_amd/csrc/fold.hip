@@ -211,10 +211,14 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
 // (block, set).
 constexpr int kFoldSetChunk = 8;
 constexpr int kFoldRow = kFoldSetChunk * 16 + 4;  // LDS row: 4 (mod 32) dwords of padding
+// Weight sets share their runs (seg_stride 0, set s's partials part_stride
+// further on); the point sets of a transform_many group have runs of their own
+// (set s's run ranges seg_stride further on in seg_lo / seg_hi, part_stride
+// 0).  The sets' rows go to slots set0 .. set0 + nsets - 1 of ntot.
 __global__ __launch_bounds__(256) void k_fold_sets(
     const float* __restrict__ part, int64_t part_stride, const int* __restrict__ seg_lo,
-    const int* __restrict__ seg_hi, int Nl, int Nx, int Npair, double dx, int nsets,
-    float* __restrict__ rhs, unsigned* __restrict__ rmax) {
+    const int* __restrict__ seg_hi, int seg_stride, int Nl, int Nx, int Npair, double dx,
+    int nsets, int set0, int ntot, float* __restrict__ rhs, unsigned* __restrict__ rmax) {
   __shared__ __attribute__((aligned(16))) float tile[2][kFoldM][kFoldRow];  // [P][m][set, 16]
   __shared__ float smx[kMaxSets];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -223,26 +227,30 @@ __global__ __launch_bounds__(256) void k_fold_sets(
   const int np = blockIdx.x;
   const bool live = m < Nl;
   // run ranges of the 8 (node, hemisphere, half) slots of each branch
-  // (uniform: they depend on the node pair only)
+  // (uniform: they depend on the node pair and the set's run arrays only)
   int r0[2][8], r1[2][8];
+  auto slot_runs = [&](const int* lo, const int* hi) {
 #pragma unroll
-  for (int br = 0; br < 2; ++br)
+    for (int br = 0; br < 2; ++br)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int odd = q >> 2, h = (q >> 1) & 1, half = q & 1;
-      const int j = 2 * np + odd - half;  // bin n (v0, v1) or bin n-1 (v2, v3)
-      r0[br][q] = r1[br][q] = 0;
-      if (j >= 0 && j <= Nx - 2) {
-        const int sg = (br * 2 + h) * (Nx - 1) + j;
-        r0[br][q] = seg_lo[sg];
-        r1[br][q] = seg_hi[sg];
+      for (int q = 0; q < 8; ++q) {
+        const int odd = q >> 2, h = (q >> 1) & 1, half = q & 1;
+        const int j = 2 * np + odd - half;  // bin n (v0, v1) or bin n-1 (v2, v3)
+        r0[br][q] = r1[br][q] = 0;
+        if (j >= 0 && j <= Nx - 2) {
+          const int sg = (br * 2 + h) * (Nx - 1) + j;
+          r0[br][q] = lo[sg];
+          r1[br][q] = hi[sg];
+        }
       }
-    }
+  };
+  if (seg_stride == 0) slot_runs(seg_lo, seg_hi);
   for (int c0 = 0; c0 < nsets; c0 += kFoldSetChunk) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int sl = 2 * wv + u, set = c0 + sl;
       if (set >= nsets) break;  // uniform
+      if (seg_stride != 0) slot_runs(seg_lo + set * seg_stride, seg_hi + set * seg_stride);
       const float2* pv = reinterpret_cast<const float2*>(part + set * part_stride);
       float vmax = 0.f;
       float out[2][16];
@@ -308,9 +316,9 @@ __global__ __launch_bounds__(256) void k_fold_sets(
       const int row = f / per_row, within = f - row * per_row;
       const int ml = row >> 1, P = row & 1;
       if (m0 + ml < Nl)
-        *reinterpret_cast<float4*>(rhs + ((((int64_t)(m0 + ml) * 2 + P) * Npair + np) * nsets + c0) *
-                                             16 + 4 * within) =
-            *reinterpret_cast<const float4*>(&tile[P][ml][4 * within]);
+        *reinterpret_cast<float4*>(
+            rhs + ((((int64_t)(m0 + ml) * 2 + P) * Npair + np) * ntot + set0 + c0) * 16 +
+            4 * within) = *reinterpret_cast<const float4*>(&tile[P][ml][4 * within]);
     }
     __syncthreads();
   }
@@ -370,8 +378,33 @@ int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64
   DSHT_LAUNCH_CHECK();
   dim3 grid(pl.Npair, (pl.Nl + kFoldM - 1) / kFoldM);
   k_fold_sets<<<grid, 256, 0, s>>>((const float*)part, part_stride, ws.seg_run_lo,
-                                                ws.seg_run_hi, pl.Nl, pl.Nx, pl.Npair, pl.dx,
-                                                nsets, (float*)ws.rhs, ws.rmax);
+                                   ws.seg_run_hi, 0, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets, 0,
+                                   nsets, (float*)ws.rhs, ws.rmax);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_fold_many(Plan& pl, const ManySets& ms, int set0, int nsets, hipStream_t s) {
+  Workspace& ws = pl.ws;
+  if (!use_mfma_contract(pl, nsets) || set0 + ms.nsets > nsets || ms.nsets > ws.cap_segsets) {
+    set_error("dsht: the point-set group fold serves the matrix-core contraction");
+    return DSHT_EINVAL;
+  }
+  const int stride = pl.nseg + 2;
+  const int nchunks = ms.chunk_off[ms.nsets];
+  const int* run_total = ws.run_base + nchunks;  // scan total = number of runs
+  const int nkeys = ms.nsets * stride;
+  k_seg_clear<<<(std::max(nkeys, ms.nsets) + 255) / 256, 256, 0, s>>>(
+      nkeys, ws.seg_run_lo, ws.seg_run_hi, ws.rmax + set0, ms.nsets);
+  DSHT_LAUNCH_CHECK();
+  const int64_t max_runs = (int64_t)nchunks + nkeys;
+  k_seg_runs<<<(int)((max_runs + 255) / 256), 256, 0, s>>>(ws.run_seg, run_total, ws.seg_run_lo,
+                                                          ws.seg_run_hi);
+  DSHT_LAUNCH_CHECK();
+  dim3 grid(pl.Npair, (pl.Nl + kFoldM - 1) / kFoldM);
+  k_fold_sets<<<grid, 256, 0, s>>>((const float*)ws.part, 0, ws.seg_run_lo, ws.seg_run_hi, stride,
+                                   pl.Nl, pl.Nx, pl.Npair, pl.dx, ms.nsets, set0, nsets,
+                                   (float*)ws.rhs, ws.rmax + set0);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

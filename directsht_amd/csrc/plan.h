@@ -20,23 +20,58 @@ inline int node_pairs(int Nx) { return 2 * ((Nx + 3) / 4); }
 constexpr int kMaxNl = 4096;
 constexpr int kMaxNx = 1 << 20;
 
+// Point sets of one group of a dsht_transform_many call, processed together
+// (kManyGroup at most): passed by value to the group's kernels.  Set s has
+// n[s] points (device arrays th, ph, w); its sorted records occupy global
+// positions off[s] .. off[s] + n[s]; its sort tiles are global tiles
+// tile_off[s] .. tile_off[s+1] - 1 of tile[s] points, its scatter work items
+// global chunks chunk_off[s] .. chunk_off[s+1] - 1 of chunk[s] points.
+constexpr int kManyGroup = 16;
+struct ManySets {
+  int nsets;
+  const double* th[kManyGroup];
+  const double* ph[kManyGroup];
+  const double* w[kManyGroup];
+  int n[kManyGroup];
+  int tile[kManyGroup];
+  int chunk[kManyGroup];
+  int off[kManyGroup + 1];
+  int tile_off[kManyGroup + 1];
+  int chunk_off[kManyGroup + 1];
+};
+
+// the set of global index x given the sets' first indices first[0..nsets]
+__host__ __device__ inline int many_set_of(const int* first, int nsets, int x) {
+  int lo = 0, hi = nsets - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 struct Workspace {
   int64_t cap_points = 0;  // capacity of the sorted-point arrays
   int64_t cap_chunks = 0;  // capacity of per-chunk arrays
   int64_t cap_runs = 0;    // capacity of the run partials
   int64_t cap_tiles = 0;   // capacity of the tile histogram matrix (rows)
   int cap_sets = 0;        // sets of a batched transform the per-set arrays hold
+  int cap_segsets = 1;     // point sets whose segment arrays are held at once (many path)
   bool has_wsort = false;  // wsort allocated (weight-set transforms)
   int cur_chunks = 0;      // scatter chunks of the transform being enqueued
   PointRec* rec = nullptr;    // [cap_points] sorted point records
   int* run_count = nullptr;   // [cap_chunks + 1]: runs per chunk
   int* run_base = nullptr;    // [cap_chunks + 1]: exclusive scan of run_count
   int* run_seg = nullptr;     // [cap_runs]: segment of each run
-  void* part = nullptr;       // [cap_part_sets][cap_runs][4][Nl][2]: per-run v_k(m),
-                              // k = 0..3, complex; double (FP64 scatter) or float
-                              // (FP32 scatter); one slot per weight set for the
-                              // weight-set scatter, else one
+  void* part = nullptr;       // per-run v_k(m), k = 0..3, complex, [4][Nl][2] per
+                              // run; double (FP64 scatter) or float (FP32 scatter):
+                              // [cap_runs] runs, or for the weight-set scatter
+                              // [cap_part_sets][cap_set_runs] (one slot per set),
+                              // whichever is larger
   int cap_part_sets = 0;      // slots of part
+  int64_t cap_set_points = 0; // points of a weight-set transform (wsf rows)
+  int64_t cap_set_runs = 0;   // runs of a part slot of a weight-set transform
   int* seg_count = nullptr;   // [nseg + 2]
   int* seg_start = nullptr;   // [nseg + 2]
   int* seg_cursor = nullptr;  // [nseg + 2]
@@ -57,7 +92,7 @@ struct Workspace {
   float* wsf = nullptr;       // [cap_part_sets][ldws]: weight sets in sorted order,
                               // FP32, zero past the valid points (weight-set scatter);
                               // followed by the same size of scratch for the sort
-  int64_t ldws = 0;           // row stride of wsf: cap_points rounded up to 64, + 64
+  int64_t ldws = 0;           // row stride of wsf: cap_set_points rounded up to 64, + 64
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
   double* a_pol = nullptr;    // [cap_sets][Nlm][2]
@@ -162,6 +197,16 @@ int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipS
 // binning + counting sort; wt NULL = unit weights; flags: the set's 4 status ints
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, int* flags, hipStream_t s);
+// the group's point sets sorted into one record array (segment arrays and
+// flags per set: set s at s (nseg + 2) and flags + 4 s; run counts per
+// global chunk, run_base over all the group's chunks)
+int launch_points_many(Plan& pl, const ManySets& ms, int mode, int* flags, unsigned* wmax,
+                       hipStream_t s);
+// the matrix-core scatter of the group's chunks; runs keyed by
+// set (nseg + 2) + segment in run_seg
+int launch_scatter_mfma_many(Plan& pl, const ManySets& ms, const unsigned* wmax, hipStream_t s);
+// the fold of the group's sets into rhs slots set0 .. set0 + ms.nsets - 1 of nsets
+int launch_fold_many(Plan& pl, const ManySets& ms, int set0, int nsets, hipStream_t s);
 // ws.wsort[pos] = w[rec[pos].idx] for the sorted valid points
 int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s);
 // wsort NULL: the records' own weights
@@ -207,7 +252,7 @@ constexpr int kSetsChunk = 16384;  // largest work item of the weight-set scatte
 inline bool uses_sets_scatter(const Plan& pl, int nsets) {
   return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX && nsets >= kSetsScatterMin;
 }
-inline int64_t part_set_stride(const Plan& pl) { return pl.ws.cap_runs * (int64_t)pl.Nl * 8; }
+inline int64_t part_set_stride(const Plan& pl) { return pl.ws.cap_set_runs * (int64_t)pl.Nl * 8; }
 // w: nsets rows of ldw weights in the caller's input order
 int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
                         hipStream_t s);

@@ -517,6 +517,317 @@ int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s) 
   return DSHT_OK;
 }
 
+// ---- the point sets of a dsht_transform_many group, sorted in one pass ----
+// Each set's points go to its own range of the record array (global
+// positions off[s] ..), sorted by its own segments in input order, exactly as
+// launch_points sorts one set; tiles never span sets.  Segment arrays of set
+// s start at s (nseg + 2), its flags at 4 s, its max |w| at wmax[s].
+__global__ __launch_bounds__(256) void k_clear_many(int n_seg_all, int* __restrict__ seg_count,
+                                                    int nflags, int* __restrict__ flags,
+                                                    int nsets, unsigned* __restrict__ wmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_seg_all) seg_count[i] = 0;
+  if (i < nflags) flags[i] = 0;
+  if (i < nsets) wmax[i] = 0u;
+}
+
+// segment ids: one block per global tile
+__global__ __launch_bounds__(256) void k_segments_many(ManySets ms, PointParams p,
+                                                       int* __restrict__ seg_of_pt,
+                                                       int* __restrict__ flags) {
+  __shared__ int red[2][4];
+  const int t = blockIdx.x;
+  const int s = many_set_of(ms.tile_off, ms.nsets, t);
+  const int lt = t - ms.tile_off[s];
+  const int b = lt * ms.tile[s], e = min(ms.n[s], b + ms.tile[s]);
+  const double* __restrict__ th = ms.th[s];
+  const double* __restrict__ ph = ms.ph[s];
+  int flag = 0, npol = 0;
+  for (int i = b + threadIdx.x; i < e; i += 256) {
+    const Classified c = classify(th[i], ph[i], 0.0, p);
+    seg_of_pt[ms.off[s] + i] = c.seg;
+    flag |= c.flag;
+    npol += c.polar;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    flag |= __shfl_xor(flag, off);
+    npol += __shfl_xor(npol, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = flag;
+    red[1][threadIdx.x >> 6] = npol;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int f = red[0][0] | red[0][1] | red[0][2] | red[0][3];
+    const int n = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    if (f) atomicOr(&flags[4 * s], f);
+    if (n) atomicAdd(&flags[4 * s + 1], n);
+  }
+}
+
+// ranks in input order, one wavefront per global tile (as k_rank_hist)
+__global__ __launch_bounds__(64) void k_rank_hist_many(ManySets ms,
+                                                       const int* __restrict__ seg_of_pt, int nb,
+                                                       int nbits, int* __restrict__ thist,
+                                                       int* __restrict__ rank_of_pt) {
+  extern __shared__ unsigned short cnt[];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < nb; i += 64) cnt[i] = 0;
+  __syncthreads();
+  const int t = blockIdx.x;
+  const int s = many_set_of(ms.tile_off, ms.nsets, t);
+  const int lt = t - ms.tile_off[s];
+  const int64_t b = ms.off[s] + (int64_t)lt * ms.tile[s];
+  const int64_t e = ms.off[s] + min(ms.n[s], (lt + 1) * ms.tile[s]);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int sg[4], n1[4], n2[4];
+  auto load = [&](int64_t base, int (&v)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = seg_of_pt[min(base + 64 * j + lane, e - 1)];
+  };
+  if (b < e) {
+    load(b, n1);
+    load(b + kSortRound, n2);
+  }
+  for (int64_t base = b; base < e; base += kSortRound) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sg[j] = base + 64 * j + lane < e ? n1[j] : -1;
+      n1[j] = n2[j];
+    }
+    load(base + 2 * kSortRound, n2);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int sgj = sg[j];
+      const bool act = sgj >= 0;
+      uint64_t peers = __ballot(act);
+      for (int bt = 0; bt < nbits; ++bt) {
+        const bool bit = (sgj >> bt) & 1;
+        const uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      if (act) {
+        const int rank = __popcll(peers & below);
+        const int old = cnt[sgj];
+        if (rank == 0) cnt[sgj] = (unsigned short)(old + __popcll(peers));
+        rank_of_pt[base + 64 * j + lane] = old + rank;
+      }
+    }
+  }
+  __syncthreads();
+  int* row = thist + (int64_t)t * nb;
+  for (int i = lane; i < nb; i += 64) row[i] = cnt[i];
+}
+
+// column sums / offsets over each set's tiles (grid.y = set)
+__global__ __launch_bounds__(kColSeg * kColSplit) void k_col_sum_many(ManySets ms,
+                                                                      const int* __restrict__ thist,
+                                                                      int nb, int seg_stride,
+                                                                      int* __restrict__ seg_count) {
+  __shared__ int part[kColSplit][kColSeg + 1];
+  const int set = blockIdx.y;
+  const int sl = threadIdx.x % kColSeg, q = threadIdx.x / kColSeg;
+  const int sg = blockIdx.x * kColSeg + sl;
+  const int t0s = ms.tile_off[set], nt = ms.tile_off[set + 1] - t0s;
+  int t0, t1;
+  part[q][sl] = col_partial(thist + (int64_t)t0s * nb, nt, nb, sg, q, t0, t1);
+  __syncthreads();
+  if (q == 0 && sg < nb) {
+    int tot = 0;
+    for (int k = 0; k < kColSplit; ++k) tot += part[k][sl];
+    seg_count[set * seg_stride + sg] = tot;
+  }
+}
+
+__global__ __launch_bounds__(kColSeg * kColSplit) void k_col_offsets_many(
+    ManySets ms, int* __restrict__ thist, int nb, int seg_stride,
+    const int* __restrict__ seg_start) {
+  __shared__ int part[kColSplit][kColSeg + 1];
+  const int set = blockIdx.y;
+  const int sl = threadIdx.x % kColSeg, q = threadIdx.x / kColSeg;
+  const int sg = blockIdx.x * kColSeg + sl;
+  const int t0s = ms.tile_off[set], nt = ms.tile_off[set + 1] - t0s;
+  int* th = thist + (int64_t)t0s * nb;
+  int t0, t1;
+  part[q][sl] = col_partial(th, nt, nb, sg, q, t0, t1);
+  __syncthreads();
+  if (sg >= nb) return;
+  int run = seg_start[set * seg_stride + sg];
+  for (int k = 0; k < q; ++k) run += part[k][sl];
+  for (int t = t0; t < t1; ++t) {
+    const int64_t k = (int64_t)t * nb + sg;
+    const int c = th[k];
+    th[k] = run;
+    run += c;
+  }
+}
+
+// per set (block = set): seg_start = off[s] + exclusive scan of its counts,
+// ne_rank = non-empty segments before each
+__global__ __launch_bounds__(1024) void k_scan_many(ManySets ms, const int* __restrict__ in,
+                                                    int n, int stride, int* __restrict__ out,
+                                                    int* __restrict__ rank, int nrank) {
+  __shared__ int wsum[16], wcnt[16];
+  const int set = blockIdx.x;
+  in += set * stride;
+  out += set * stride;
+  rank += set * stride;
+  const int add = ms.off[set];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int per = (n + 1023) / 1024;
+  const int b = min(n, tid * per), e = min(n, b + per);
+  int sm = 0, c = 0;
+  for (int i = b; i < e; ++i) {
+    const int v = in[i];
+    sm += v;
+    c += v > 0;
+  }
+  const int si = wave_incl_scan(sm, lane), ci = wave_incl_scan(c, lane);
+  if (lane == 63) {
+    wsum[w] = si;
+    wcnt[w] = ci;
+  }
+  __syncthreads();
+  if (w == 0) {
+    const int a = lane < 16 ? wsum[lane] : 0, k = lane < 16 ? wcnt[lane] : 0;
+    const int as = wave_incl_scan(a, lane), ks = wave_incl_scan(k, lane);
+    if (lane < 16) {
+      wsum[lane] = as - a;
+      wcnt[lane] = ks - k;
+    }
+  }
+  __syncthreads();
+  int run = add + wsum[w] + si - sm, cnt = wcnt[w] + ci - c;
+  for (int i = b; i < e; ++i) {
+    const int v = in[i];
+    out[i] = run;
+    if (i <= nrank) rank[i] = cnt;
+    run += v;
+    cnt += v > 0;
+  }
+  if (tid == 1023) {
+    out[n] = run;
+    if (nrank == n) rank[n] = cnt;
+  }
+}
+
+// records at offset + rank: grid (blocks per tile, global tiles)
+__global__ __launch_bounds__(256) void k_place_many(ManySets ms, PointParams p,
+                                                    const int* __restrict__ toff,
+                                                    const int* __restrict__ seg_of_pt,
+                                                    const int* __restrict__ rank_of_pt,
+                                                    PointRec* __restrict__ rec,
+                                                    unsigned* __restrict__ wmax) {
+  const int nb = p.nseg + 1;
+  const int t = blockIdx.y;
+  const int s = many_set_of(ms.tile_off, ms.nsets, t);
+  const int lt = t - ms.tile_off[s];
+  const int b = lt * ms.tile[s], e = min(ms.n[s], b + ms.tile[s]);
+  const int i0 = b + blockIdx.x * (256 * kPlacePer) + threadIdx.x;
+  const double* __restrict__ th = ms.th[s];
+  const double* __restrict__ ph = ms.ph[s];
+  const double* __restrict__ ww = ms.w[s];
+  const int o = ms.off[s];
+  float vmax = 0.f;
+#pragma unroll
+  for (int u = 0; u < kPlacePer; ++u) {
+    const int i = i0 + 256 * u;
+    if (i < e) {
+      const double wi = ww[i];
+      const int sg = seg_of_pt[o + i], rk = rank_of_pt[o + i];
+      const Classified c = classify(th[i], ph[i], wi, p);
+      vmax = fmaxf(vmax, abs_w(wi, sg, p.nseg));
+      const int pos = toff[(int64_t)t * nb + sg] + rk;
+      double2* d = reinterpret_cast<double2*>(rec + pos);
+      d[0] = make_double2(c.ere, c.eim);
+      d[1] = make_double2(c.h0, c.h1);
+      d[2] = make_double2(c.h2, c.h3);
+      const double uu = p.want_u ? c.u : 0.0;
+      reinterpret_cast<int4*>(d + 3)[0] = make_int4(sg, i, __double2loint(uu), __double2hiint(uu));
+    }
+  }
+  block_wmax(vmax, wmax + s);
+}
+
+// runs of each global chunk (chunk c of set s covers global positions
+// off[s] + (c - chunk_off[s]) chunk[s] ..)
+__global__ __launch_bounds__(256) void k_count_runs_many(ManySets ms,
+                                                         const int* __restrict__ seg_start,
+                                                         const int* __restrict__ ne_rank, int nseg,
+                                                         int stride, int nchunks,
+                                                         int* __restrict__ run_count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchunks) return;
+  const int s = many_set_of(ms.chunk_off, ms.nsets, c);
+  const int* ss = seg_start + s * stride;
+  const int* nr = ne_rank + s * stride;
+  const int nvalid = ss[nseg];
+  const int b = ms.off[s] + (c - ms.chunk_off[s]) * ms.chunk[s];
+  const int e = min(nvalid, b + ms.chunk[s]);
+  if (b >= e) {
+    run_count[c] = 0;
+    return;
+  }
+  const int sb = seg_of(ss, nseg, b), se = seg_of(ss, nseg, e - 1);
+  run_count[c] = nr[se] - nr[sb] + 1;
+}
+
+int launch_points_many(Plan& pl, const ManySets& ms, int mode, int* flags, unsigned* wmax,
+                       hipStream_t s) {
+  Workspace& ws = pl.ws;
+  PointParams p{pl.Nx, pl.nseg, mode, uses_spread(pl) ? 1 : 0, pl.xmax, pl.dx, pl.acos_xmax,
+                pl.acos_mxmax};
+  const int nb = pl.nseg + 1, stride = pl.nseg + 2;
+  const int ntiles = ms.tile_off[ms.nsets], nchunks = ms.chunk_off[ms.nsets];
+  const int npts = ms.off[ms.nsets];
+  if (ms.nsets > ws.cap_segsets || ntiles > ws.cap_tiles || nchunks > ws.cap_chunks ||
+      npts > ws.cap_points || sizeof(unsigned short) * nb > (size_t)kRankLdsMax) {
+    set_error("dsht: point-set group beyond the workspace");
+    return DSHT_EINVAL;
+  }
+  int nbits = 0;
+  while ((1 << nbits) < nb) ++nbits;
+  const int nclear = std::max(ms.nsets * stride, 4 * ms.nsets);
+  k_clear_many<<<(nclear + 255) / 256, 256, 0, s>>>(ms.nsets * stride, ws.seg_count, 4 * ms.nsets,
+                                                    flags, ms.nsets, wmax);
+  DSHT_LAUNCH_CHECK();
+  int* seg_of_pt = reinterpret_cast<int*>(ws.segrank);
+  int* rank_of_pt = seg_of_pt + ws.cap_points;
+  int maxtile = 0;
+  for (int k = 0; k < ms.nsets; ++k) maxtile = std::max(maxtile, ms.tile[k]);
+  if (ntiles > 0) {
+    k_segments_many<<<ntiles, 256, 0, s>>>(ms, p, seg_of_pt, flags);
+    DSHT_LAUNCH_CHECK();
+    k_rank_hist_many<<<ntiles, 64, sizeof(unsigned short) * nb, s>>>(ms, seg_of_pt, nb, nbits,
+                                                                     ws.thist, rank_of_pt);
+    DSHT_LAUNCH_CHECK();
+    k_col_sum_many<<<dim3((nb + kColSeg - 1) / kColSeg, ms.nsets), kColSeg * kColSplit, 0, s>>>(
+        ms, ws.thist, nb, stride, ws.seg_count);
+    DSHT_LAUNCH_CHECK();
+  }
+  k_scan_many<<<ms.nsets, 1024, 0, s>>>(ms, ws.seg_count, pl.nseg + 1, stride, ws.seg_start,
+                                        ws.ne_rank, pl.nseg);
+  DSHT_LAUNCH_CHECK();
+  if (ntiles > 0) {
+    k_col_offsets_many<<<dim3((nb + kColSeg - 1) / kColSeg, ms.nsets), kColSeg * kColSplit, 0, s>>>(
+        ms, ws.thist, nb, stride, ws.seg_start);
+    DSHT_LAUNCH_CHECK();
+    const int bx = (maxtile + 256 * kPlacePer - 1) / (256 * kPlacePer);
+    k_place_many<<<dim3(bx, ntiles), 256, 0, s>>>(ms, p, ws.thist, seg_of_pt, rank_of_pt, ws.rec,
+                                                  wmax);
+    DSHT_LAUNCH_CHECK();
+  }
+  if (nchunks > 0) {
+    k_count_runs_many<<<(nchunks + 255) / 256, 256, 0, s>>>(ms, ws.seg_start, ws.ne_rank, pl.nseg,
+                                                            stride, nchunks, ws.run_count);
+    DSHT_LAUNCH_CHECK();
+  }
+  k_scan<<<1, 1024, 0, s>>>(ws.run_count, nchunks, ws.run_base, nullptr, nullptr, 0);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, int* flags, hipStream_t s) {
   Workspace& ws = pl.ws;

@@ -124,15 +124,17 @@ __device__ inline void pin(cplx& z) {
 // kGather: the weights are w[rec.idx] of a weight set in the caller's input
 // order (wsort = w), gathered here; otherwise wsort[sorted position] or, if
 // NULL, 1 (the records' own weights).
+// The scatter of one work item: sorted points [b, e) of a set whose segment
+// starts are seg_start, runs numbered from run0 (recorded in run_seg as
+// key_off + segment), m tile mt.
 template <int M1, int KB, bool kGather>
-__global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __restrict__ rec,
-                                                          const int* __restrict__ seg_start,
-                                                          int nseg, int chunk, int Nl,
-                                                          const int* __restrict__ run_base,
-                                                          const double* __restrict__ wsort,
-                                                          float* __restrict__ part,
-                                                          int* __restrict__ run_seg,
-                                                          const unsigned* __restrict__ wmax) {
+__device__ inline void scatter_mfma_item(const PointRec* __restrict__ rec,
+                                         const int* __restrict__ seg_start, const int b,
+                                         const int e, const int run0, const int key_off,
+                                         const int Nl, const int mt,
+                                         const double* __restrict__ wsort,
+                                         float* __restrict__ part, int* __restrict__ run_seg,
+                                         const unsigned* __restrict__ wmax) {
   constexpr int kWaves = M1 / 4;              // row tiles: 4 k * M1 m1 * (re, im) / 32
   constexpr int kThreads = 64 * kWaves;
   constexpr int kBlk = KB;                    // points per staged block
@@ -151,11 +153,7 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
                 "unsupported m tile / staging block");
   static_assert(kBStep * kNWin == kBPer && kEStep * kNWin == kEPer && kEStep >= 1,
                 "chain steps must spread evenly over the windows of a block");
-  const int c = blockIdx.x, mt = blockIdx.y;
   const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-  const int nvalid = seg_start[nseg];
-  const int b = c * chunk, e = min(nvalid, b + chunk);
-  if (b >= e) return;
 
   // two staging buffers: block i is multiplied out of one while block i+1 is
   // generated into the other.  B = e^{-i m2 phi} as fp16 hi/lo,
@@ -193,7 +191,7 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
   acc[0] = acc[1] = zero;
 
   // run tracking is wave-uniform (scalar registers, scalar branches)
-  int run = __builtin_amdgcn_readfirstlane(run_base[c]);
+  int run = __builtin_amdgcn_readfirstlane(run0);
   int cur = __builtin_amdgcn_readfirstlane(rec[b].seg);
   int run_end = min(e, seg_start[cur + 1]);
   int blk = b, bn = 0, buf = 0;  // staged block: first point, size, buffer
@@ -343,7 +341,7 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
       }
     }
     acc[0] = acc[1] = zero;
-    if (tid == 0 && mt == 0) run_seg[run] = cur;
+    if (tid == 0 && mt == 0) run_seg[run] = key_off + cur;
     if (p < e) {
       ++run;
       const int off = p - blk;  // point p may lie past this block
@@ -489,6 +487,41 @@ __global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __r
   }
 }
 
+template <int M1, int KB, bool kGather>
+__global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma(const PointRec* __restrict__ rec,
+                                                          const int* __restrict__ seg_start,
+                                                          int nseg, int chunk, int Nl,
+                                                          const int* __restrict__ run_base,
+                                                          const double* __restrict__ wsort,
+                                                          float* __restrict__ part,
+                                                          int* __restrict__ run_seg,
+                                                          const unsigned* __restrict__ wmax) {
+  const int c = blockIdx.x;
+  const int nvalid = seg_start[nseg];
+  const int b = c * chunk, e = min(nvalid, b + chunk);
+  if (b >= e) return;
+  scatter_mfma_item<M1, KB, kGather>(rec, seg_start, b, e, run_base[c], 0, Nl, blockIdx.y, wsort,
+                                     part, run_seg, wmax);
+}
+
+// the work items of a group of point sets (dsht_transform_many): global
+// chunk c belongs to set s = many_set_of(chunk_off, c)
+template <int M1, int KB>
+__global__ __launch_bounds__(16 * M1, 2) void k_scatter_mfma_many(
+    ManySets ms, const PointRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg,
+    int stride, int Nl, const int* __restrict__ run_base, float* __restrict__ part,
+    int* __restrict__ run_seg, const unsigned* __restrict__ wmax) {
+  const int c = blockIdx.x;
+  const int s = many_set_of(ms.chunk_off, ms.nsets, c);
+  const int* ss = seg_start + s * stride;
+  const int nvalid = ss[nseg];
+  const int b = ms.off[s] + (c - ms.chunk_off[s]) * ms.chunk[s];
+  const int e = min(nvalid, b + ms.chunk[s]);
+  if (b >= e) return;
+  scatter_mfma_item<M1, KB, false>(rec, ss, b, e, run_base[c], s * stride, Nl, blockIdx.y,
+                                   nullptr, part, run_seg, wmax + s);
+}
+
 template <int M1, int KB>
 static int launch_m1(Plan& pl, int nchunks, const double* w, bool gather, const unsigned* wmax,
                      hipStream_t s) {
@@ -503,6 +536,24 @@ static int launch_m1(Plan& pl, int nchunks, const double* w, bool gather, const 
                                                           (float*)pl.ws.part, pl.ws.run_seg, wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
+}
+
+template <int M1, int KB>
+static int launch_many_m1(Plan& pl, const ManySets& ms, const unsigned* wmax, hipStream_t s) {
+  const int nchunks = ms.chunk_off[ms.nsets];
+  dim3 grid(nchunks, (pl.Nl + 32 * M1 - 1) / (32 * M1));
+  k_scatter_mfma_many<M1, KB><<<grid, 16 * M1, 0, s>>>(ms, pl.ws.rec, pl.ws.seg_start, pl.nseg,
+                                                      pl.nseg + 2, pl.Nl, pl.ws.run_base,
+                                                      (float*)pl.ws.part, pl.ws.run_seg, wmax);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_scatter_mfma_many(Plan& pl, const ManySets& ms, const unsigned* wmax, hipStream_t s) {
+  if (ms.chunk_off[ms.nsets] == 0) return DSHT_OK;
+  if (pl.Nl <= 256) return launch_many_m1<8, 64>(pl, ms, wmax, s);
+  if (pl.Nl <= 512) return launch_many_m1<16, 64>(pl, ms, wmax, s);
+  return launch_many_m1<32, 128>(pl, ms, wmax, s);
 }
 
 // *wmax = max |w[i]| over the finite entries of one weight set (float bits)

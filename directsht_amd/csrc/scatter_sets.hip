@@ -797,7 +797,7 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
   if (nchunks == 0) return DSHT_OK;
   Workspace& ws = pl.ws;
   if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_part_sets || !ws.wsf ||
-      Np > ws.cap_points) {
+      Np > ws.cap_set_points) {
     set_error("dsht: the weight-set scatter takes 1..64 sets within the workspace");
     return DSHT_EINVAL;
   }

@@ -88,13 +88,18 @@ class DirectSHT:
         other._Yv = other._Yd = None
         return other
 
-    def reserve(self, npoints, nsets=1):
+    def reserve(self, npoints, nsets=1, point_sets=True, weight_sets=True):
         """Size the workspace for transforms of up to ``npoints`` points (per
         set) and ``nsets`` sets now, so later transforms neither allocate nor
-        synchronize."""
+        synchronize: point-set transforms (``__call__``, ``transform_device``,
+        ``transform_many_device``) and/or weight-set transforms
+        (``transform_sets_device``)."""
+        kinds = (1 if point_sets else 0) | (2 if weight_sets else 0)
+        if not kinds:
+            raise ValueError("reserve: nothing to reserve for")
         with torch.cuda.device(self.device):
-            _lib.check(_lib.load().dsht_plan_reserve(self._plan, int(npoints), int(nsets)),
-                       "dsht_plan_reserve")
+            _lib.check(_lib.load().dsht_plan_reserve_for(self._plan, int(npoints), int(nsets),
+                                                         kinds), "dsht_plan_reserve")
 
     def footprint(self):
         """HBM bytes held: tables (shared with clones) + own workspace."""

@@ -112,6 +112,16 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** clone);
  * first such transform). */
 int dsht_plan_reserve(dsht_plan* plan, int64_t Np, int nsets);
 
+/* dsht_plan_reserve for some kinds of transform only (no counterpart in the
+ * reference; it spares batched callers the workspace of the kinds they never
+ * run): DSHT_RESERVE_POINT_SETS for dsht_transform and dsht_transform_many
+ * (up to nsets point sets of up to Np points each), DSHT_RESERVE_WEIGHT_SETS
+ * for dsht_transform_sets (up to nsets weight sets on Np shared points), or
+ * both (= dsht_plan_reserve). */
+#define DSHT_RESERVE_POINT_SETS 1
+#define DSHT_RESERVE_WEIGHT_SETS 2
+int dsht_plan_reserve_for(dsht_plan* plan, int64_t Np, int nsets, int kinds);
+
 /* Bytes of HBM the plan holds: its tables (shared with its clones, see
  * dsht_plan_clone; the Y-only table, plus the FP32 {Y, Y'} table once a
  * batched matrix-core contraction has built it) + its own current

@@ -116,3 +116,80 @@ def test_weight_sets_lognormal_c3_like(torch_cuda):
     torch.cuda.synchronize()
     for s in range(S):
         assert rel(a32[s], a64[s]) < 1e-6, s
+
+
+def test_many_point_sets_in_groups(torch_cuda):
+    """40 point sets at one band limit: three groups of up to 16 sets sorted,
+    scattered and folded together (FP32, matrix-core engines), with empty sets
+    between others, a set with NaN points (dropped), a polar-only and an
+    equatorial-only set; each set within 1e-6 of its single FP32 transform
+    and of the FP64 oracle."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    from oracle import refalg
+    rng = np.random.default_rng(40)
+    S, Nl = 40, 96
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(50), np.log(2e5), S))]
+    sizes[5] = sizes[17] = sizes[39] = 0
+    sets = []
+    for i, n in enumerate(sizes):
+        if i == 8:  # polar only
+            c = torch.empty(n, device="cuda", dtype=torch.float64).uniform_(0.8, 1.0)
+            th = torch.acos(c * torch.where(torch.rand(n, device="cuda") < 0.5, -1.0, 1.0))
+            ph = 2 * math.pi * torch.rand(n, device="cuda", dtype=torch.float64)
+        elif i == 9:  # equatorial only
+            th = torch.acos(torch.empty(n, device="cuda", dtype=torch.float64).uniform_(-0.7, 0.7))
+            ph = 2 * math.pi * torch.rand(n, device="cuda", dtype=torch.float64)
+        else:
+            th, ph = points(torch, 500 + i, n)
+        if i == 20 and n > 3:
+            th = th.clone()
+            th[::3] = float("nan")
+        w = 0.5 + torch.rand(n, device="cuda", dtype=torch.float64)
+        sets.append((th, ph, w))
+    sht = DirectSHT(Nl, 2 * Nl, dflt_type="float32")
+    alm, status = sht.transform_many_device(sets)
+    torch.cuda.synchronize()
+    assert int(status.max()) == 0
+    tables = refalg.make_tables(Nl, 2 * Nl, 0.75)
+    for i, (th, ph, w) in enumerate(sets):
+        if th.numel() == 0:
+            assert float(alm[i].abs().max()) == 0.0, i
+            continue
+        assert rel(alm[i], sht(th, ph, w)) < TOL["float32"], i
+        if i % 7 == 0 or i in (8, 9, 20):
+            t, p, ww = (a.cpu().numpy() for a in (th, ph, w))
+            keep = ~np.isnan(t)
+            ref = refalg.direct_sht(tables, t[keep], p[keep], ww[keep])
+            assert rel(alm[i], ref) < TOL["float32"], i
+
+
+def test_reserve_kinds_cover_their_transforms(torch_cuda):
+    """A point-set reservation holds every dsht_transform_many group of its
+    sizes, a weight-set reservation every dsht_transform_sets: the plan's
+    footprint does not change over transforms within them."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    n, S = 50000, 20
+    rng = np.random.default_rng(21)
+    sets = []
+    for i, m in enumerate(int(x) for x in rng.integers(1, n + 1, S)):
+        th, ph = points(torch, 700 + i, m)
+        sets.append((th, ph, 0.5 + torch.rand(m, device="cuda", dtype=torch.float64)))
+    base = DirectSHT(96, 192, dflt_type="float32")
+    sp = base.clone()
+    sp.reserve(n, S, weight_sets=False)
+    f0 = sp.footprint()
+    sp.transform_many_device(sets)
+    sp.transform_many_device(sets[:5])
+    torch.cuda.synchronize()
+    assert sp.footprint() == f0
+    sw = base.clone()
+    sw.reserve(n, S, point_sets=False)
+    f1 = sw.footprint()
+    th, ph = points(torch, 800, n)
+    W = 0.5 + torch.rand((S, n), device="cuda", dtype=torch.float64)
+    sw.transform_sets_device(th, ph, W)
+    sw.transform_sets_device(th[:1000], ph[:1000], W[:7, :1000].contiguous())
+    torch.cuda.synchronize()
+    assert sw.footprint() == f1
