@@ -186,7 +186,8 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
   if (with_w) get(ws.wsort, want.points * (int64_t)sizeof(double));
   get(ws.a_equ, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
   get(ws.a_pol, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
-  if (sets > 1) get(ws.a_tmp, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
+  // the batched rotation's input, in groups of up to 16 sets
+  if (sets > 1) get(ws.a_tmp, pl.Nlm * 2 * ((sets + 15) / 16 * 16) * (int64_t)sizeof(double));
   get(ws.a_der, (int64_t)kContractSplits * pl.Nlm * 8 * (int64_t)sizeof(double));
   get(ws.flags, 4 * sets * (int64_t)sizeof(int));
   if (rc) return rc;

@@ -96,8 +96,9 @@ struct Workspace {
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
   double* a_pol = nullptr;    // [cap_sets][Nlm][2]
-  double* a_tmp = nullptr;    // [cap_sets][Nlm][2] (cap_sets > 1): the batched
-                              // rotation's l-major copy of a_pol
+  double* a_tmp = nullptr;    // (cap_sets > 1) the batched rotation's copy of
+                              // a_pol, grouped: [cap_sets / G][Nlm][2][G] in l-major
+                              // order, G <= 16 (room for cap_sets rounded up to 16)
   double* a_der = nullptr;    // [kContractSplits][Nlm][8]: partial sums of the plain
                               // contraction per node range (one set at a time)
   int* flags = nullptr;       // [cap_sets][4]: status bits, n_polar, unused, unused

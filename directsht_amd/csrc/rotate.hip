@@ -231,17 +231,60 @@ __global__ __launch_bounds__(256) void k_tri_transpose(const double2* __restrict
   }
 }
 
-// Many sets: G sets per workgroup (up to 32), one thread per m (rounds of
-// 256), the recursion for each (l, m) generated once and applied to all G
-// sets.  The a'_{l m'} of the G sets are streamed through LDS in chunks of
-// kRotChunk values of m' (all threads walk m' = l .. 1 in the same order), so
-// the LDS footprint does not grow with l.
-constexpr int kRotChunk = 32;
-
-// Inputs and output in l-major order (k_tri_transpose): a_pol of the sets,
-// and a_equ, which is overwritten with the result (each thread reads its own
-// (l, m) entries before writing them).
+// healpy order [set][Nlm] -> grouped l-major order [set / G][Nlm][re|im][G]
+// (the batched rotation's input: for each (l, m') and component, the G sets
+// of a group are 8 G contiguous bytes).  One workgroup per tile of 16 l x 8 m
+// of one group: the G sets' tiles are read (consecutive l at fixed m) into
+// LDS and written out (consecutive sets, components and m at fixed l) as
+// whole 128-byte lines.  Sets past nsets are left unwritten.
 template <int G>
+__global__ __launch_bounds__(256) void k_tri_group(const double2* __restrict__ in,
+                                                   double* __restrict__ out, int Nl, int nsets) {
+  __shared__ double2 tile[G][8][16];  // [set][m - m0][l - l0]
+  const int l0 = 16 * blockIdx.x, m0 = 8 * blockIdx.y, grp = blockIdx.z;
+  if (m0 > l0 + 15) return;  // tile above the diagonal: no m <= l
+  const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < G * 128 / 256; ++r) {
+    const int e = t + 256 * r;
+    const int li = e & 15, mi = (e >> 4) & 7, g = e >> 7;
+    const int l = l0 + li, m = m0 + mi, set = grp * G + g;
+    if (l < Nl && m <= l && set < nsets) tile[g][mi][li] = in[set * Nlm + hp_index(l, m, Nl)];
+  }
+  __syncthreads();
+  double* dst = out + (int64_t)grp * Nlm * 2 * G;
+#pragma unroll
+  for (int r = 0; r < G * 256 / 256; ++r) {
+    const int e = t + 256 * r;
+    const int g = e % G, comp = (e / G) & 1, q = e / (2 * G);
+    const int mi = q & 7, li = q >> 3;
+    const int l = l0 + li, m = m0 + mi, set = grp * G + g;
+    if (l < Nl && m <= l && set < nsets) {
+      const double2 v = tile[g][mi][li];
+      dst[(((int64_t)l * (l + 1) / 2 + m) * 2 + comp) * G + g] = comp ? v.y : v.x;
+    }
+  }
+}
+
+// Many sets: G sets per workgroup, the recursion for each (l, m) generated
+// once and applied to all G sets.  Every lane of a wavefront needs the same
+// a'_{l m'} at each step, so the sets' values are read with scalar loads
+// (through the constant address space: uniform addresses, read-only data)
+// and enter the FMAs as scalar operands; no LDS.  The component a step reads
+// (Re or Im) follows the parity of m, so each wavefront takes m of one
+// parity: wavefront w of the block holds m = m0 + 2 (64 (w >> 1) + lane) +
+// (w & 1), and each thread C such values m, m + 256, ... (chains sharing
+// every scalar operand).  Accumulators are kept in the chain's scale 2^E
+// (the same power of two as its d values) and rescaled with them, which
+// is exact; the true sums are formed at the end.
+typedef const double __attribute__((address_space(4)))* kdptr;
+typedef const int __attribute__((address_space(4)))* kiptr;
+
+// Inputs: a_pol of the sets in grouped l-major order (k_tri_group), a_equ in
+// l-major order (k_tri_transpose), which is overwritten with the result
+// (each thread reads its own (l, m) entries before writing them).
+template <int G, int C>
 __global__ __launch_bounds__(256) void k_rotate_sets(const double* a_equ,
                                                      const double* __restrict__ a_pol,
                                                      double* alm, int Nl, int nsets,
@@ -254,101 +297,103 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* a_equ,
   const int ell = Nl - 1 - (int)blockIdx.x;  // the longest recursions first
   const int s0 = blockIdx.y * G;
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
-  // a'_{l m'}, m' = hi - row, as [row][re|im][set]: a lane reads its
-  // component for two sets per 16-byte LDS read
-  __shared__ __attribute__((aligned(16))) double sv[1][kRotChunk][2][G];
-  __shared__ double sAB[1][kRotChunk], sBi[1][kRotChunk];
-  __shared__ int smode[G];
-  if (threadIdx.x < G) {
-    const int set = s0 + threadIdx.x;
-    int md = 0;  // 0 not in block, 1 zeros, 2 equatorial only, 3 rotate and combine
-    if (set < nsets) {
-      const int fl = flags[set * flag_stride];
-      const int npol = flags[set * flag_stride + 1];
-      if (status && ell == 0) status[set] = fl;
-      md = (fl & DSHT_STATUS_POLAR_RANGE) ? 1 : (npol == 0 ? 2 : 3);
-    }
-    smode[threadIdx.x] = md;
-  }
-  __syncthreads();
+  const int64_t tri = (int64_t)ell * (ell + 1) / 2;
+  const kiptr fl = (kiptr)flags;
+  // per set: 0 = not in this block, 1 = zeros (polar range failure),
+  // 2 = equatorial part only (no polar point), 3 = rotate and combine
+  int mode[G];
   bool any_rot = false;
 #pragma unroll
-  for (int g = 0; g < G; ++g) any_rot |= smode[g] == 3;
-  const int64_t tri = (int64_t)ell * (ell + 1) / 2;
-  for (int m0 = 0; m0 <= ell; m0 += blockDim.x) {
-    const int m = m0 + threadIdx.x;
-    const bool live = m <= ell;
-    double accE[G], accO[G];
+  for (int g = 0; g < G; ++g) {
+    const int set = s0 + g;
+    mode[g] = 0;
+    if (set < nsets) {
+      const int f = fl[(int64_t)set * flag_stride];
+      const int npol = fl[(int64_t)set * flag_stride + 1];
+      mode[g] = (f & DSHT_STATUS_POLAR_RANGE) ? 1 : (npol == 0 ? 2 : 3);
+      any_rot |= mode[g] == 3;
+    }
+  }
+  if (status && ell == 0 && (int)threadIdx.x < G && s0 + (int)threadIdx.x < nsets)
+    status[s0 + threadIdx.x] = flags[(int64_t)(s0 + threadIdx.x) * flag_stride];
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int p = w & 1;                     // parity of this wavefront's m
+  const int wfirst = 128 * (w >> 1) + p;   // its smallest m - m0
+  const int mw = wfirst + 2 * lane;        // this thread's m - m0 (chain 0)
+  // a'_{l m'} of set s0 + g: xs[(2 m' + component) G + g] (sets past nsets:
+  // unused values)
+  const kdptr xs = (kdptr)a_pol + ((int64_t)blockIdx.y * Nlm + tri) * 2 * G;
+  const kdptr ab = (kdptr)rot_AB + tri;
+  const kdptr ib = (kdptr)rot_iB + tri;
+  for (int m0 = 0; m0 + wfirst <= ell; m0 += 256 * C) {
+    int mm[C], E[C];
+    bool act[C];
+    double cur[C], nxt[C], twom[C], accE[C][G], accO[C][G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) accE[g] = accO[g] = 0.0;
-    double cur = live ? rot_top[tri + m] : 0.0, nxt = 0.0;
-    int E = live ? rot_exp[tri + m] : 0;
-    const double twom = 2.0 * m;
-    const int cm = m & 1;  // component of even steps: re for even m, im for odd m
+    for (int c = 0; c < C; ++c) {
+      mm[c] = m0 + mw + 256 * c;
+      act[c] = mm[c] <= ell;
+      cur[c] = act[c] ? rot_top[tri + mm[c]] : 0.0;  // inactive chains stay 0
+      E[c] = act[c] ? rot_exp[tri + mm[c]] : 0;
+      nxt[c] = 0.0;
+      twom[c] = 2.0 * mm[c];
+#pragma unroll
+      for (int g = 0; g < G; ++g) accE[c][g] = accO[c][g] = 0.0;
+    }
     if (any_rot) {
-      for (int hi = ell; hi >= 1; hi -= kRotChunk) {
-        const int lo = max(1, hi - kRotChunk + 1);
-        const int n = hi - lo + 1;
-        const int b = 0;
-        __syncthreads();  // the previous chunk has been consumed
-        // consecutive threads take consecutive m' of one set: a contiguous
-        // stretch in l-major order
-        for (int e = threadIdx.x; e < n * G; e += blockDim.x) {
-          const int row = e % n, g = e / n;
-          double2 v = make_double2(0.0, 0.0);
-          if (smode[g] == 3)
-            v = *reinterpret_cast<const double2*>(a_pol + ((int64_t)(s0 + g) * Nlm + tri + hi - row) * 2);
-          sv[b][row][0][g] = v.x;
-          sv[b][row][1][g] = v.y;
-        }
-        for (int e = threadIdx.x; e < n; e += blockDim.x) {
-          sAB[b][e] = rot_AB[tri + hi - e];
-          sBi[b][e] = rot_iB[tri + hi - e];
-        }
-        __syncthreads();
-        if (live) {
-          for (int row = 0; row < n; ++row) {
-            // step s = l - m' (m' = hi - row): d_{m',m} times Re a' if s + m is
-            // even and Im a' if odd; even steps accumulate in accE, odd in accO
-            const double dv = ldexp(cur, E);
-            typedef double d2 __attribute__((ext_vector_type(2)));
-            if (((ell - hi + row) & 1) == 0) {  // uniform branch
-              const d2* v = reinterpret_cast<const d2*>(&sv[b][row][cm][0]);
+      // step s = l - m' contributes d_{m',m} a'_{m'}: the component of
+      // parity m + s (Re for even m + s), into accE for even s, accO for odd
+      // s; d_{m'-1} = d_m' (2m/B) - d_{m'+1} (A/B)
+      auto step = [&](int mq, int comp, double (&acc)[C][G]) {
+        double x[G];
 #pragma unroll
-              for (int g = 0; g < G; g += 2) {
-                const d2 a = v[g / 2];
-                accE[g] = fma(dv, a.x, accE[g]);
-                accE[g + 1] = fma(dv, a.y, accE[g + 1]);
-              }
-            } else {
-              const d2* v = reinterpret_cast<const d2*>(&sv[b][row][cm ^ 1][0]);
+        for (int g = 0; g < G; ++g) x[g] = xs[(2 * mq + comp) * G + g];
 #pragma unroll
-              for (int g = 0; g < G; g += 2) {
-                const d2 a = v[g / 2];
-                accO[g] = fma(dv, a.x, accO[g]);
-                accO[g + 1] = fma(dv, a.y, accO[g + 1]);
-              }
-            }
-            // d_{m'-1} = d_m' * (2m/B) - d_{m'+1} * (A/B); values stay below 2^1023
-            // since they are rescaled as soon as they pass 2^400
-            const double nw = fma(cur, twom * sBi[b][row], -nxt * sAB[b][row]);
-            nxt = cur;
-            cur = nw;
-            if (fmax(fabs(cur), fabs(nxt)) > kScaleHi) {
-              cur *= kScaleLo;
-              nxt *= kScaleLo;
-              E += 400;
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+          for (int g = 0; g < G; ++g) acc[c][g] = fma(cur[c], x[g], acc[c][g]);
+        const double b = ib[mq], a = ab[mq];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const double nw = fma(cur[c], twom[c] * b, -nxt[c] * a);
+          nxt[c] = cur[c];
+          cur[c] = nw;
+        }
+      };
+      int mp = ell;
+      while (mp >= 2) {
+        const int pairs = min(4, mp >> 1);
+        for (int q = 0; q < pairs; ++q) {
+          step(mp, p, accE);
+          step(mp - 1, p ^ 1, accO);
+          mp -= 2;
+        }
+        // at most 8 steps since the last check: growth per step is below 3l,
+        // so values stay below 2^400 * (3l)^8 < 2^1023
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          if (fmax(fabs(cur[c]), fabs(nxt[c])) > kScaleHi) {
+            cur[c] *= kScaleLo;
+            nxt[c] *= kScaleLo;
+            E[c] += 400;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+              accE[c][g] *= kScaleLo;
+              accO[c][g] *= kScaleLo;
             }
           }
-        }
       }
+      if (mp == 1) step(1, p, accE);
     }
-    if (live) {
-      const double d0 = ldexp(cur, E);  // d_{0,m}
-      const int64_t idx = tri + m, idx0 = tri;  // l-major
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      if (!act[c]) continue;
+      const double d0 = ldexp(cur[c], E[c]);  // d_{0,m}
+      const int64_t idx = tri + mm[c];          // l-major
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int md = smode[g];
+        const int md = mode[g];
         if (md == 0) continue;
         const int64_t o = ((int64_t)(s0 + g) * Nlm + idx) * 2;
         double re = 0.0, im = 0.0;
@@ -357,11 +402,11 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* a_equ,
           im = a_equ[o + 1];
         }
         if (md == 3) {
-          const double* a0 = a_pol + ((int64_t)(s0 + g) * Nlm + idx0) * 2;
-          const double sr = cm ? accO[g] : accE[g];
-          const double si = cm ? accE[g] : accO[g];
-          re += fma(d0, a0[0], 2.0 * sr);
-          im += fma(d0, a0[1], 2.0 * si);
+          // even m: Re collects the even steps; odd m: the odd steps
+          const double sr = ldexp(p ? accO[c][g] : accE[c][g], E[c]);
+          const double si = ldexp(p ? accE[c][g] : accO[c][g], E[c]);
+          re += fma(d0, xs[g], 2.0 * sr);
+          im += fma(d0, xs[G + g], 2.0 * si);
         }
         alm[o] = re;
         alm[o + 1] = im;
@@ -372,7 +417,7 @@ __global__ __launch_bounds__(256) void k_rotate_sets(const double* a_equ,
 
 // a_pol -> a_tmp and a_equ -> a_pol in l-major order; the rotation and
 // combine in place in a_pol; back to the healpy order into alm.
-template <int G>
+template <int G, int C>
 static int launch_rotate_sets(Plan& pl, int nsets, const int* flags, int flag_stride, double* alm,
                               int* status, hipStream_t s) {
   Workspace& ws = pl.ws;
@@ -382,12 +427,13 @@ static int launch_rotate_sets(Plan& pl, int nsets, const int* flags, int flag_st
   }
   const int nt = (pl.Nl + 31) / 32;
   const dim3 tg(nt, nt, nsets);
-  k_tri_transpose<true><<<tg, 256, 0, s>>>((const double2*)ws.a_pol, (double2*)ws.a_tmp, pl.Nl);
+  const dim3 tgg((pl.Nl + 15) / 16, (pl.Nl + 7) / 8, (nsets + G - 1) / G);
+  k_tri_group<G><<<tgg, 256, 0, s>>>((const double2*)ws.a_pol, ws.a_tmp, pl.Nl, nsets);
   DSHT_LAUNCH_CHECK();
   k_tri_transpose<true><<<tg, 256, 0, s>>>((const double2*)ws.a_equ, (double2*)ws.a_pol, pl.Nl);
   DSHT_LAUNCH_CHECK();
   dim3 grid(pl.Nl, (nsets + G - 1) / G);
-  k_rotate_sets<G><<<grid, 256, 0, s>>>(ws.a_pol, ws.a_tmp, ws.a_pol, pl.Nl, nsets, flags,
+  k_rotate_sets<G, C><<<grid, 256, 0, s>>>(ws.a_pol, ws.a_tmp, ws.a_pol, pl.Nl, nsets, flags,
                                         flag_stride, status, pl.rot_top, pl.rot_exp, pl.rot_AB,
                                         pl.rot_iB);
   DSHT_LAUNCH_CHECK();
@@ -418,11 +464,10 @@ int launch_rotate_combine(Plan& pl, int nsets, const int* flags, int flag_stride
     DSHT_LAUNCH_CHECK();
     return DSHT_OK;
   }
-  // at most 16 sets per workgroup: 32 would need two waves per SIMD (FP64
-  // accumulators) and runs slower than recomputing the d-columns per 16 sets
-  if (nsets > 8) return launch_rotate_sets<16>(pl, nsets, flags, flag_stride, alm, status, s);
-  if (nsets > 4) return launch_rotate_sets<8>(pl, nsets, flags, flag_stride, alm, status, s);
-  return launch_rotate_sets<4>(pl, nsets, flags, flag_stride, alm, status, s);
+  // G x C x 2 FP64 accumulators per thread: 16 sets x 2 chains, 8 x 4, 4 x 4
+  if (nsets > 8) return launch_rotate_sets<16, 2>(pl, nsets, flags, flag_stride, alm, status, s);
+  if (nsets > 4) return launch_rotate_sets<8, 4>(pl, nsets, flags, flag_stride, alm, status, s);
+  return launch_rotate_sets<4, 4>(pl, nsets, flags, flag_stride, alm, status, s);
 }
 
 }  // namespace dsht
