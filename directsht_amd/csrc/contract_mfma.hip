@@ -68,27 +68,29 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
   // row e%64, e = tid + 512 v) and this lane's B operands (K step t, node
   // pairs c0 + 4t + 2h + {0, 1})
   float4 ya[2], bq[4][2];
+  // every load is issued unconditionally at a clamped index; out-of-range
+  // entries are zeroed where the chunk is used, not where it is loaded, so
+  // that the compiler waits for the loads only a chunk later (a select right
+  // after a load waits for it at once; a conditional load of a stack zero
+  // becomes a flat load, which LDS waits also wait for)
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const int rA = tid & 63, npA = 2 * (tid >> 6);  // this thread's table entries
   auto load = [&](int c0) {
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
       // node pairs 2 (tid / 64) + v of the chunk: the two halves of one
       // A-operand element, so the staging writes whole 16-byte elements
-      const int np = c0 + 2 * (tid >> 6) + v, r = tid & 63;
-      if (np < Npair && r < gd.nrows) {  // read once: non-temporal
-        typedef float v4 __attribute__((ext_vector_type(4)));
-        const v4 t4 = __builtin_nontemporal_load(
-            reinterpret_cast<const v4*>(tab + gd.off + (int64_t)np * gd.nrows + r));
-        ya[v] = make_float4(t4.x, t4.y, t4.z, t4.w);
-      } else {
-        ya[v] = zero4;
-      }
+      const int npc = min(c0 + npA + v, Npair - 1), rc = min(rA, gd.nrows - 1);
+      const v4 t4 = __builtin_nontemporal_load(  // read once: non-temporal
+          reinterpret_cast<const v4*>(tab + gd.off + (int64_t)npc * gd.nrows + rc));
+      ya[v] = make_float4(t4.x, t4.y, t4.z, t4.w);
     }
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int np = c0 + 4 * t + 2 * h + u;
-        bq[t][u] = (colv && np < Npair) ? *reinterpret_cast<const float4*>(qb + np * qstride) : zero4;
+        bq[t][u] = *reinterpret_cast<const float4*>(qb + min(np, Npair - 1) * qstride);
       }
   };
   load(0);
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
       u32x4 hv, lv;
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
-        float4 y = ya[half];
+        float4 y = c0 + npA + half < Npair && rA < gd.nrows ? ya[half] : zero4;
         y.z *= dx;
         y.w *= dx;
         uint32_t h01, l01, h23, l23;
@@ -121,16 +123,16 @@ __global__ __launch_bounds__(kCThreads, 2) void k_contract_mfma(
     // B of this chunk, scaled; then the next chunk's loads go in flight
     float x[4][8];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      x[t][0] = bq[t][0].x * bsc;
-      x[t][1] = bq[t][0].y * bsc;
-      x[t][2] = bq[t][0].z * bscd;
-      x[t][3] = bq[t][0].w * bscd;
-      x[t][4] = bq[t][1].x * bsc;
-      x[t][5] = bq[t][1].y * bsc;
-      x[t][6] = bq[t][1].z * bscd;
-      x[t][7] = bq[t][1].w * bscd;
-    }
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const bool ok = colv && c0 + 4 * t + 2 * h + u < Npair;
+        const float s0 = ok ? bsc : 0.f, s1 = ok ? bscd : 0.f;
+        x[t][4 * u + 0] = bq[t][u].x * s0;
+        x[t][4 * u + 1] = bq[t][u].y * s0;
+        x[t][4 * u + 2] = bq[t][u].z * s1;
+        x[t][4 * u + 3] = bq[t][u].w * s1;
+      }
     if (c0 + kCK < Npair) load(c0 + kCK);
     // one barrier per chunk: every wave has finished the chunk that last
     // used buffer `buf` before anyone stages into it again

@@ -128,7 +128,7 @@ __device__ inline void pin(cplx& z) {
 // starts are seg_start, runs numbered from run0 (recorded in run_seg as
 // key_off + segment), m tile mt.
 template <int M1, int KB, bool kGather>
-__device__ inline void scatter_mfma_item(const PointRec* __restrict__ rec,
+__device__ __forceinline__ void scatter_mfma_item(const PointRec* __restrict__ rec,
                                          const int* __restrict__ seg_start, const int b,
                                          const int e, const int run0, const int key_off,
                                          const int Nl, const int mt,
