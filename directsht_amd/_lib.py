@@ -34,6 +34,8 @@ SIGNATURES = {
                                 _dp, _dp, _dp, _dp, _dp]),
     "dsht_plan_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_double, ct.c_int, ct.POINTER(_vp)]),
     "dsht_plan_destroy": (ct.c_int, [_vp]),
+    "dsht_plan_create_from_tables": (ct.c_int, [ct.c_int, ct.c_int, ct.c_double, ct.c_int, _vp,
+                                                ct.POINTER(_vp)]),
     "dsht_plan_clone": (ct.c_int, [_vp, ct.POINTER(_vp)]),
     "dsht_plan_reserve": (ct.c_int, [_vp, ct.c_int64, ct.c_int]),
     "dsht_plan_reserve_for": (ct.c_int, [_vp, ct.c_int64, ct.c_int, ct.c_int]),

@@ -929,6 +929,52 @@ int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** out
   return DSHT_OK;
 }
 
+int dsht_plan_create_from_tables(int Nl, int Nx, double xmax, int precision, const double* Yv,
+                                 dsht_plan** out) {
+  if (!out || !Yv) {
+    set_error("dsht_plan_create_from_tables: null argument");
+    return DSHT_EINVAL;
+  }
+  *out = nullptr;
+  dsht_plan* p = new dsht_plan();
+  Plan& pl = p->pl;
+  int rc = plan_init(pl, Nl, Nx, xmax, precision, true, false);
+  // the rows go through a device slab of at most 256 MiB (at least one row)
+  double* slab = nullptr;
+  const int64_t per = std::max<int64_t>(1, ((int64_t)256 << 20) / ((int64_t)Nx * 8));
+  if (rc == DSHT_OK) {
+    const size_t bytes = (size_t)std::min<int64_t>(per, pl.Nlm) * Nx * sizeof(double);
+    if (hipMalloc((void**)&slab, bytes) != hipSuccess) {
+      slab = nullptr;
+      set_error("dsht_plan_create_from_tables: out of device memory for the staging slab");
+      rc = DSHT_ENOMEM;
+    }
+  }
+  for (int64_t r0 = 0; rc == DSHT_OK && r0 < pl.Nlm; r0 += per) {
+    const int64_t n = std::min<int64_t>(per, pl.Nlm - r0);
+    hipError_t e = hipMemcpy(slab, Yv + r0 * Nx, (size_t)n * Nx * sizeof(double),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      set_error(std::string("dsht_plan_create_from_tables: ") + hipGetErrorString(e));
+      rc = DSHT_EHIP;
+      break;
+    }
+    rc = launch_relayout_y(pl, r0, n, slab, 0);
+    if (rc == DSHT_OK && hipDeviceSynchronize() != hipSuccess) {  // the slab is reused
+      set_error("dsht_plan_create_from_tables: relayout failed");
+      rc = DSHT_EHIP;
+    }
+  }
+  if (slab) (void)hipFree(slab);
+  if (rc) {
+    plan_free(pl);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return DSHT_OK;
+}
+
 int dsht_plan_destroy(dsht_plan* p) {
   if (!p) return DSHT_OK;
   DeviceGuard g(p->pl.device);

@@ -195,6 +195,10 @@ int64_t workspace_bytes(int Nl, int Nx, int precision, int64_t Np, int chunk);
 // (dst float*), 3 healpy rows (dst = Yv, hpYd = Yd)
 int launch_tables(const Plan& pl, int out, void* dst, double* hpYd, hipStream_t s);
 int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipStream_t s);
+// healpy rows row0 .. row0 + nrows - 1 of a Y table (`slab`, device, FP64)
+// into the plan's Y-only table
+int launch_relayout_y(const Plan& pl, int64_t row0, int64_t nrows, const double* slab,
+                      hipStream_t s);
 // binning + counting sort; wt NULL = unit weights; flags: the set's 4 status ints
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, int* flags, hipStream_t s);

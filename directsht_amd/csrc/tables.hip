@@ -109,6 +109,17 @@ __global__ __launch_bounds__(256) void k_tables(int Nl, int Nx, int Npair, doubl
   }
 }
 
+// (l, m) of healpy row `row`: inverts row = m(2Nl-1-m)/2 + l, as
+// sht/shared_utils.py:93-98 (the estimate corrected to the exact m)
+__device__ inline void hp_row_lm(int64_t row, int Nl, int& ell, int& m) {
+  const double b = 2.0 * Nl - 1.0;
+  m = (int)ceil((b - sqrt(b * b - 8.0 * ((double)row - (Nl - 1)))) / 2.0);
+  if (m < 0) m = 0;
+  while (m > 0 && hp_index(m, m, Nl) > row) --m;
+  while (m + 1 < Nl && hp_index(m + 1, m + 1, Nl) <= row) ++m;
+  ell = (int)(row - hp_index(0, m, Nl));
+}
+
 // Copy caller-provided healpy-row tables (csht's make_table layout) into the
 // contraction layout (FP64).  One block per healpy row.
 __global__ __launch_bounds__(256) void k_relayout(int Nl, int Nx, int Npair,
@@ -117,13 +128,8 @@ __global__ __launch_bounds__(256) void k_relayout(int Nl, int Nx, int Npair,
                                                   const double* __restrict__ hpYd,
                                                   double* __restrict__ tab64) {
   const int64_t row = blockIdx.x;
-  // invert row = m(2Nl-1-m)/2 + l (healpy index), as sht/shared_utils.py:93-98
-  const double b = 2.0 * Nl - 1.0;
-  int m = (int)ceil((b - sqrt(b * b - 8.0 * ((double)row - (Nl - 1)))) / 2.0);
-  if (m < 0) m = 0;
-  while (m > 0 && hp_index(m, m, Nl) > row) --m;
-  while (m + 1 < Nl && hp_index(m + 1, m + 1, Nl) <= row) ++m;
-  const int ell = (int)(row - hp_index(0, m, Nl));
+  int ell, m;
+  hp_row_lm(row, Nl, ell, m);
   const int k = ell - m, P = k & 1, i = k >> 1;
   const int g = i / kGroupRows, r = i % kGroupRows;
   const int nrows = min(kGroupRows, rows_mp(Nl, m, P) - g * kGroupRows);
@@ -135,6 +141,43 @@ __global__ __launch_bounds__(256) void k_relayout(int Nl, int Nx, int Npair,
     tab64[idx4 * 4 + (n & 1)] = y;
     tab64[idx4 * 4 + 2 + (n & 1)] = yd;
   }
+}
+
+// Healpy rows row0 .. row0 + gridDim.x - 1 of a Y table (`slab`: those rows,
+// [row][Nx], FP64) into the plan's Y-only table (FP64 or FP32), at the
+// positions k_tables writes: a plan built from a table saved earlier.  One
+// block per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_relayout_y(int Nl, int Nx, int Npair,
+                                                    const int64_t* __restrict__ mp_off,
+                                                    int64_t row0, const double* __restrict__ slab,
+                                                    T* __restrict__ tab) {
+  const int64_t row = row0 + blockIdx.x;
+  int ell, m;
+  hp_row_lm(row, Nl, ell, m);
+  const int k = ell - m, P = k & 1, i = k >> 1;
+  const int g = i / kGroupRows, r = i % kGroupRows;
+  const int nrows = min(kGroupRows, rows_mp(Nl, m, P) - g * kGroupRows);
+  const int64_t gbase = mp_off[2 * m + P] + (int64_t)g * kGroupRows * Npair;
+  const double* src = slab + (int64_t)blockIdx.x * Nx;
+  for (int n = threadIdx.x; n < 2 * Npair; n += blockDim.x) {
+    const int np = n >> 1;
+    const int64_t ey = gbase + (int64_t)(np >> 1) * 2 * nrows + 2 * r + (np & 1);
+    tab[ey * 2 + (n & 1)] = (T)(n < Nx ? src[n] : 0.0);  // the padding node of an odd Nx: 0
+  }
+}
+
+int launch_relayout_y(const Plan& pl, int64_t row0, int64_t nrows, const double* slab,
+                      hipStream_t s) {
+  if (nrows <= 0) return DSHT_OK;
+  if (pl.precision == DSHT_FP64)
+    k_relayout_y<double><<<(unsigned)nrows, 256, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.mp_off, row0,
+                                                         slab, pl.tab64y);
+  else
+    k_relayout_y<float><<<(unsigned)nrows, 256, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.mp_off, row0,
+                                                        slab, pl.tab32y);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
 }
 
 int launch_relayout(const Plan& pl, const double* hpYv, const double* hpYd, hipStream_t s) {

@@ -66,11 +66,13 @@ class TransformServer:
     submit(theta, phi, wt, Nl, Nx=None) -> concurrent.futures.Future of the
     healpy-ordered complex128 alm (numpy).  Inputs may be numpy arrays (copied
     to the device on the request's stream) or CUDA float64 tensors (used in
-    place).  Nx defaults to nx_factor * Nl.
+    place).  Nx defaults to nx_factor * Nl.  table_cache: a directory (or
+    tablecache.TableCache) whose saved Y tables the server's plans start from
+    (tables it builds are saved there).
     """
 
     def __init__(self, hbm_budget=None, max_inflight=4, dflt_type="float32", xmax=0.75,
-                 nx_factor=2, device=None):
+                 nx_factor=2, device=None, table_cache=None):
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = torch.device(device)
@@ -82,6 +84,7 @@ class TransformServer:
         self.xmax = float(xmax)
         self.nx_factor = int(nx_factor)
         self.max_inflight = int(max_inflight)
+        self.table_cache = table_cache  # directory of tables saved to disk (tablecache.py)
         self._tables = collections.OrderedDict()   # (Nl, Nx) -> resident DirectSHT (LRU order)
         self._table_bytes = {}                     # (Nl, Nx) -> bytes
         self._idle = collections.defaultdict(list)  # (Nl, Nx) -> idle clones
@@ -212,7 +215,8 @@ class TransformServer:
     # ---- scheduling -----------------------------------------------------
     def _acquire(self, key):
         if key not in self._tables:
-            base = DirectSHT(key[0], key[1], self.xmax, self.dflt_type, device=self.device)
+            base = DirectSHT(key[0], key[1], self.xmax, self.dflt_type, device=self.device,
+                             table_cache=self.table_cache)
             base._table_bytes_shared = base.footprint()  # no workspace yet: tables only
             self._tables[key] = base
             self._table_bytes[key] = base._table_bytes_shared

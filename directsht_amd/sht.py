@@ -39,8 +39,12 @@ class DirectSHT:
     """Brute-force spherical harmonic transforms of point sets (sht/sht.py:30)."""
 
     def __init__(self, Nell, Nx, xmax=0.75, dflt_type="float64", null_unphysical=True,
-                 device=None):
-        """Build the FP64 spline tables on the GPU (sht/sht.py:32-56)."""
+                 device=None, table_cache=None):
+        """Build the FP64 spline tables on the GPU (sht/sht.py:32-56).
+
+        ``table_cache`` (a directory or a ``tablecache.TableCache``; no
+        counterpart in the reference): take the Y table from the cache when it
+        holds this (Nl, Nx, xmax), else build it and save it there."""
         if dflt_type not in ("float64", "float32"):
             raise ValueError("dflt_type must be 'float64' or 'float32'")
         self.Nell, self.Nx, self.xmax = int(Nell), int(Nx), float(xmax)
@@ -56,12 +60,26 @@ class DirectSHT:
         self._precision = _lib.DSHT_FP64 if dflt_type == "float64" else _lib.DSHT_FP32
         lib = _lib.load()
         plan = ct.c_void_p()
+        cache = None
+        if table_cache is not None:
+            from .tablecache import TableCache
+            cache = table_cache if isinstance(table_cache, TableCache) else TableCache(table_cache)
+        cached = cache.load_ylm(self.Nell, self.Nx, self.xmax) if cache is not None else None
         with torch.cuda.device(self.device):
-            _lib.check(lib.dsht_plan_create(self.Nell, self.Nx, self.xmax, self._precision,
-                                            ct.byref(plan)), "dsht_plan_create")
+            if cached is not None:
+                Yv = np.ascontiguousarray(cached)
+                _lib.check(lib.dsht_plan_create_from_tables(
+                    self.Nell, self.Nx, self.xmax, self._precision, Yv.ctypes.data,
+                    ct.byref(plan)), "dsht_plan_create_from_tables")
+            else:
+                _lib.check(lib.dsht_plan_create(self.Nell, self.Nx, self.xmax, self._precision,
+                                                ct.byref(plan)), "dsht_plan_create")
         self._plan = plan
         self._status = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._Yv = self._Yd = None
+        self.from_cache = cached is not None
+        if cache is not None and cached is None:
+            cache.save_ylm(self.Nell, self.Nx, self.xmax, self.Yv)
 
     def __del__(self):
         plan = getattr(self, "_plan", None)

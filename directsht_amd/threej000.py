@@ -36,12 +36,53 @@ def threej000_table_device(N, device=None, stream=None):
 
 
 class Wigner3j:
-    """Table of (l1 l2 l3; 0 0 0) for l's < Nl (sht/threej000.py:77-116)."""
+    """Table of (l1 l2 l3; 0 0 0) for l's < Nl (sht/threej000.py:77-116).
 
-    def __init__(self, Nl, device=None):
+    ``table_cache`` (a directory or a ``tablecache.TableCache``): take the
+    table from the cache when it holds this Nl, else build it and save it
+    there -- the save/load that sht/threej000.py:80-81 describes."""
+
+    def __init__(self, Nl, device=None, table_cache=None):
         self.Nl = int(Nl)
-        self.store_device = threej000_table_device(self.Nl, device)
-        self._store = None
+        cache = None
+        if table_cache is not None:
+            from .tablecache import TableCache
+            cache = table_cache if isinstance(table_cache, TableCache) else TableCache(table_cache)
+        cached = cache.load_threej(self.Nl) if cache is not None else None
+        if cached is not None:
+            self._from_store(cached, device)
+        else:
+            self.store_device = threej000_table_device(self.Nl, device)
+            self._store = None
+            if cache is not None:
+                cache.save_threej(self.Nl, self.store)
+        self.from_cache = cached is not None
+
+    def save(self, table_cache):
+        """Write the table to a cache directory (or TableCache)."""
+        from .tablecache import TableCache
+        cache = table_cache if isinstance(table_cache, TableCache) else TableCache(table_cache)
+        return cache.save_threej(self.Nl, self.store)
+
+    def _from_store(self, store, device):
+        if device is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        self.store_device = torch.from_numpy(store).to(device)
+        self._store = store
+
+    @classmethod
+    def load(cls, Nl, table_cache, device=None):
+        """The table from a cache directory; KeyError when it holds no entry."""
+        from .tablecache import TableCache
+        cache = table_cache if isinstance(table_cache, TableCache) else TableCache(table_cache)
+        store = cache.load_threej(int(Nl))
+        if store is None:
+            raise KeyError(f"no cached 3j table for Nl={Nl} in {cache.root}")
+        obj = object.__new__(cls)
+        obj.Nl = int(Nl)
+        obj._from_store(store, device)
+        obj.from_cache = True
+        return obj
 
     @property
     def store(self):

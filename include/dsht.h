@@ -96,6 +96,16 @@ typedef struct dsht_plan dsht_plan;
 int dsht_plan_create(int Nl, int Nx, double xmax, int precision, dsht_plan** plan);
 int dsht_plan_destroy(dsht_plan* plan);
 
+/* dsht_plan_create with the Y table supplied instead of built: Yv is a host
+ * array in healpy row order [Nl(Nl+1)/2][Nx], FP64, as dsht_plan_get_tables
+ * writes it (the DirectSHT.Yv attribute, sht/sht.py:56).  No counterpart in
+ * the reference, which recomputes its tables per instance (sht/sht.py:50-56):
+ * it serves an on-disk table cache.  The plan's own table is Yv converted to
+ * the contraction layout (rounded to FP32 for DSHT_FP32), so a plan from the
+ * tables of an identical plan transforms bit for bit as that plan does. */
+int dsht_plan_create_from_tables(int Nl, int Nx, double xmax, int precision, const double* Yv,
+                                 dsht_plan** plan);
+
 /* A second plan over the same resident tables, with a workspace of its own,
  * so transforms at the same (Nl, Nx) can run concurrently on different
  * streams (the reference runs one transform at a time; this extends the plan
