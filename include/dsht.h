@@ -168,7 +168,7 @@ int dsht_transform(dsht_plan* plan, int64_t Np, const double* theta, const doubl
  * workspace is carved from a fixed 64 MiB device buffer inside libdsht.so
  * (one per device, held until the process ends) instead of the plan's
  * workspace, which is neither used, grown nor freed.  This covers Nl <= 256:
- * e.g. up to about 3.6e5 points at Nl=256, Nx=512 (FP32) or 9.7e5 at Nl=64,
+ * e.g. up to about 2.3e5 points at Nl=256, Nx=512 (FP32) or 8.4e5 at Nl=64,
  * Nx=128.  Larger transforms return DSHT_EINVAL with the bytes they would
  * need in dsht_last_error(), and leave alm untouched.  Every call on a device
  * shares that device's buffer: calls whose work overlaps on one device (other
