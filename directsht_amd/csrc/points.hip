@@ -441,12 +441,39 @@ __global__ __launch_bounds__(kColSeg * kColSplit) void k_col_offsets(int* __rest
 }
 
 // Each point's record at its tile's offset for its segment plus its rank.
-// Grid (blocks per tile, tiles): block (x, t) takes the points
-// t tile + 1024 x + 256 u + (0 .. 255), u < 4: each thread's four points'
-// loads are issued together, so more of the scattered record stores overlap
-// the reads and the FP64 classification.
+// Workgroup (tile t, x) takes the points t tile + 1024 x + 256 u + (0 .. 255),
+// u < 4: each thread's four points' loads are issued together, so more of
+// the scattered record stores overlap the reads and the FP64 classification.
 constexpr int kPlacePer = 4;
-__global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile,
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup L on XCD
+// L % 8) and each XCD has its own L2.  The placement's per-point lookups of
+// toff[tile][segment] are random within the tile's row (nseg + 1 ints), so
+// the workgroups of one tile are put on one XCD, which then reads the row
+// from HBM once for all of them: the 1-D grid is walked in groups of 8 tiles
+// x nbx workgroups, XCD = tile % 8.
+__device__ inline bool xcd_tile_block(int ntiles, int nbx, int& t, int& bx) {
+  const int L = blockIdx.x, per = 8 * nbx;
+  const int g = L / per, r = L - g * per;
+  t = 8 * g + (r & 7);
+  bx = r >> 3;
+  return t < ntiles;
+}
+inline unsigned xcd_tile_grid(int ntiles, int nbx) { return (unsigned)((ntiles + 7) / 8 * 8 * nbx); }
+
+__device__ inline void place_point(const Classified& c, int sg, int64_t i, int pos,
+                                   const PointParams& p, PointRec* __restrict__ rec) {
+  // the whole 64-byte line, in 16-byte stores (a partly written line costs a
+  // read-modify-write)
+  double2* d = reinterpret_cast<double2*>(rec + pos);
+  d[0] = make_double2(c.ere, c.eim);
+  d[1] = make_double2(c.h0, c.h1);
+  d[2] = make_double2(c.h2, c.h3);
+  const double uu = p.want_u ? c.u : 0.0;
+  reinterpret_cast<int4*>(d + 3)[0] = make_int4(sg, (int)i, __double2loint(uu), __double2hiint(uu));
+}
+
+__global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile, int ntiles, int nbx,
                                                       const double* __restrict__ theta,
                                                       const double* __restrict__ phi,
                                                       const double* __restrict__ wt, PointParams p,
@@ -456,9 +483,10 @@ __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile,
                                                       PointRec* __restrict__ rec,
                                                       unsigned* __restrict__ wmax) {
   const int nb = p.nseg + 1;
-  const int t = blockIdx.y;
+  int t, bx;
+  if (!xcd_tile_block(ntiles, nbx, t, bx)) return;
   const int64_t b = (int64_t)t * tile, e = min(Np, b + tile);
-  const int64_t i0 = b + (int64_t)blockIdx.x * (256 * kPlacePer) + threadIdx.x;
+  const int64_t i0 = b + (int64_t)bx * (256 * kPlacePer) + threadIdx.x;
   double th[kPlacePer], ph[kPlacePer], ww[kPlacePer];
   int sg[kPlacePer], rk[kPlacePer];
 #pragma unroll
@@ -477,16 +505,7 @@ __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile,
     if (i < e) {
       const Classified c = classify(th[u], ph[u], ww[u], p);
       vmax = fmaxf(vmax, abs_w(ww[u], sg[u], p.nseg));
-      const int pos = toff[(int64_t)t * nb + sg[u]] + rk[u];
-      // the whole 64-byte line, in 16-byte stores (a partly written line
-      // costs a read-modify-write)
-      double2* d = reinterpret_cast<double2*>(rec + pos);
-      d[0] = make_double2(c.ere, c.eim);
-      d[1] = make_double2(c.h0, c.h1);
-      d[2] = make_double2(c.h2, c.h3);
-      const double uu = p.want_u ? c.u : 0.0;
-      reinterpret_cast<int4*>(d + 3)[0] =
-          make_int4(sg[u], (int)i, __double2loint(uu), __double2hiint(uu));
+      place_point(c, sg[u], i, toff[(int64_t)t * nb + sg[u]] + rk[u], p, rec);
     }
   }
   block_wmax(vmax, wmax);
@@ -713,18 +732,20 @@ __global__ __launch_bounds__(1024) void k_scan_many(ManySets ms, const int* __re
 }
 
 // records at offset + rank: grid (blocks per tile, global tiles)
-__global__ __launch_bounds__(256) void k_place_many(ManySets ms, PointParams p,
+__global__ __launch_bounds__(256) void k_place_many(ManySets ms, int ntiles, int nbx,
+                                                    PointParams p,
                                                     const int* __restrict__ toff,
                                                     const int* __restrict__ seg_of_pt,
                                                     const int* __restrict__ rank_of_pt,
                                                     PointRec* __restrict__ rec,
                                                     unsigned* __restrict__ wmax) {
   const int nb = p.nseg + 1;
-  const int t = blockIdx.y;
+  int t, bx;
+  if (!xcd_tile_block(ntiles, nbx, t, bx)) return;
   const int s = many_set_of(ms.tile_off, ms.nsets, t);
   const int lt = t - ms.tile_off[s];
   const int b = lt * ms.tile[s], e = min(ms.n[s], b + ms.tile[s]);
-  const int i0 = b + blockIdx.x * (256 * kPlacePer) + threadIdx.x;
+  const int i0 = b + bx * (256 * kPlacePer) + threadIdx.x;
   const double* __restrict__ th = ms.th[s];
   const double* __restrict__ ph = ms.ph[s];
   const double* __restrict__ ww = ms.w[s];
@@ -813,9 +834,9 @@ int launch_points_many(Plan& pl, const ManySets& ms, int mode, int* flags, unsig
     k_col_offsets_many<<<dim3((nb + kColSeg - 1) / kColSeg, ms.nsets), kColSeg * kColSplit, 0, s>>>(
         ms, ws.thist, nb, stride, ws.seg_start);
     DSHT_LAUNCH_CHECK();
-    const int bx = (maxtile + 256 * kPlacePer - 1) / (256 * kPlacePer);
-    k_place_many<<<dim3(bx, ntiles), 256, 0, s>>>(ms, p, ws.thist, seg_of_pt, rank_of_pt, ws.rec,
-                                                  wmax);
+    const int nbx = (maxtile + 256 * kPlacePer - 1) / (256 * kPlacePer);
+    k_place_many<<<xcd_tile_grid(ntiles, nbx), 256, 0, s>>>(ms, ntiles, nbx, p, ws.thist,
+                                                             seg_of_pt, rank_of_pt, ws.rec, wmax);
     DSHT_LAUNCH_CHECK();
   }
   if (nchunks > 0) {
@@ -873,9 +894,10 @@ int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, 
           ws.thist, ntiles, nb, ws.seg_start);
       DSHT_LAUNCH_CHECK();
       const int* seg_of_pt = reinterpret_cast<const int*>(ws.segrank);
-      k_place_ranked<<<dim3((tile + 256 * kPlacePer - 1) / (256 * kPlacePer), ntiles), 256, 0, s>>>(
-          Np, tile, theta, phi, wt, p, ws.thist, seg_of_pt, seg_of_pt + ws.cap_points, ws.rec,
-          ws.wmax);
+      const int nbx = (tile + 256 * kPlacePer - 1) / (256 * kPlacePer);
+      k_place_ranked<<<xcd_tile_grid(ntiles, nbx), 256, 0, s>>>(
+          Np, tile, ntiles, nbx, theta, phi, wt, p, ws.thist, seg_of_pt, seg_of_pt + ws.cap_points,
+          ws.rec, ws.wmax);
     } else {
       k_place<<<blocks, threads, 0, s>>>(Np, theta, phi, wt, p, ws.seg_cursor, ws.rec, ws.wmax);
     }
