@@ -384,13 +384,13 @@ static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int
   }
 }
 
-// Spreading-scatter constants: n = 2^logn >= 4 Nl (at least 64), beta =
+// Spreading-scatter constants: n = 2^logn >= 4 Nl (at least 256), beta =
 // 2.3 W, the twiddles e^{-2 pi i k / n} and 1/psihat(m) for m < Nl, with
 // psihat(m) = (W/2) int_{-1}^{1} exp(beta (sqrt(1 - z^2) - 1)) cos(pi m W z / n) dz
 // by 128-point Gauss-Legendre quadrature (FP64).
 static void spread_constants(int Nl, int& logn, double& beta, std::vector<float2>& tw,
                              std::vector<float>& ipsi) {
-  logn = 6;
+  logn = 8;
   while ((1 << logn) < 4 * Nl) ++logn;
   const int n = 1 << logn;
   const double W = (double)kSpreadW;
@@ -690,7 +690,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     // one segment inside a work item) writes nsets partials that the fold
     // reads back, and there are about one run per work item plus one per
     // segment (fewer items: smaller capacities than the workspace holds)
-    pl.chunk = work_chunk(sets_scatter ? std::max(pl.chunk_max, kSetsChunk) : pl.chunk_max, Np);
+    pl.chunk = work_chunk(sets_scatter ? std::max(pl.chunk_max, kSetsChunk) : chunk_cap(pl), Np);
     ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
     if ((rc = launch_points(pl, Np, a.theta[0], a.phi[0], nullptr, mode, ws.flags, s))) return rc;
     const bool gathers = scatter_gathers(pl);
@@ -728,7 +728,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   } else {
     for (int set = 0; set < a.nsets; ++set) {
       const int64_t Np = a.Np[set];
-      pl.chunk = work_chunk(pl.chunk_max, Np);
+      pl.chunk = work_chunk(chunk_cap(pl), Np);
       ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
       if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
         return rc;
@@ -811,6 +811,7 @@ int dsht_plan_configure(dsht_plan* p, int option, int value) {
       pl.chunk_max = pl.chunk = value;
       pl.ws.cap_chunks = pl.ws.cap_runs = 0;  // force a workspace resize on next use
     }
+    pl.chunk_explicit = true;
     return DSHT_OK;
   }
   set_error("dsht_plan_configure: unknown option");
@@ -1005,6 +1006,7 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.precision = s.precision;
   d.chunk = s.chunk;
   d.chunk_max = s.chunk_max;
+  d.chunk_explicit = s.chunk_explicit;
   d.scatter = s.scatter;
   d.contract = s.contract;
   d.ngroups = s.ngroups;

@@ -138,6 +138,7 @@ struct Plan {
   int precision = DSHT_FP64;
   // scatter decomposition
   int chunk_max = 2048;  // configured points per scatter work item (the largest used)
+  bool chunk_explicit = false;  // chunk_max set by dsht_plan_configure
   int chunk = 2048;      // points per scatter work item of the transform being
                          // enqueued: work_chunk(chunk_max, Np)
   int scatter = DSHT_SCATTER_MATRIX;  // engine of FP32 plans
@@ -218,11 +219,24 @@ int launch_gather_weights(Plan& pl, int64_t Np, const double* w, hipStream_t s);
 int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
 // spreading scatter; wmode 0: the records' own weights, 1: w[sorted
 // position], 2: w[rec.idx] (a weight set in the caller's input order)
-constexpr int kSpreadW = 8;  // kernel taps (beta = 2.3 W)
+constexpr int kSpreadW = 8;        // kernel taps (beta = 2.3 W)
+constexpr int kSpreadMaxNl = 2048;  // grids of n <= 8192 (two 64-bit grids in LDS)
 int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
+// the spreading scatter serves FP32 plans up to kSpreadMaxNl; above it the
+// matrix engine takes its place
 inline bool uses_spread(const Plan& pl) {
-  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_SPREAD;
+  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_SPREAD && pl.Nl <= kSpreadMaxNl;
 }
+// Work items of the spreading scatter hold up to kSpreadChunk points unless a
+// chunk is configured: each run (a stretch of one segment inside an item)
+// costs one transform of its grids, so longer items mean fewer runs.
+constexpr int kSpreadChunk = 8192;
+inline int chunk_cap(const Plan& pl) {
+  return uses_spread(pl) && !pl.chunk_explicit ? std::max(pl.chunk_max, kSpreadChunk)
+                                               : pl.chunk_max;
+}
+// *ws.wmax = max |w[i]| over the finite entries of w[0..Np) (float bits)
+int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s);
 // gather: w holds a weight set in the caller's input order (read through the
 // records' idx); otherwise w is wsort as for launch_scatter
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s);

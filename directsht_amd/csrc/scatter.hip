@@ -274,9 +274,10 @@ __global__ __launch_bounds__(64) void k_scatter32(const PointRec* __restrict__ r
 int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
-  if (pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX)
-    return launch_scatter_mfma(pl, Np, wsort, false, s);
   if (uses_spread(pl)) return launch_scatter_spread(pl, Np, wsort, wsort ? 1 : 0, s);
+  if (pl.precision == DSHT_FP32 &&
+      (pl.scatter == DSHT_SCATTER_MATRIX || pl.scatter == DSHT_SCATTER_SPREAD))
+    return launch_scatter_mfma(pl, Np, wsort, false, s);
   if (pl.precision == DSHT_FP32) {
     constexpr int K = 8;
     dim3 grid(nchunks, (pl.Nl + 64 * K - 1) / (64 * K));

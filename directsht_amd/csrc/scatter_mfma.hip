@@ -580,18 +580,22 @@ __global__ void k_set_u32(unsigned* __restrict__ p, unsigned v) { *p = v; }
 // m tile of 256, 512 or 1024 orders: the narrowest that covers Nl, or 1024.
 // The operand scale comes from ws.wmax[0]: the points pass's max |w| for the
 // records' own weights; for a gathered weight set, its own max, reduced here.
+int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s) {
+  k_set_u32<<<1, 1, 0, s>>>(pl.ws.wmax, 0u);
+  DSHT_LAUNCH_CHECK();
+  const int blocks = (int)std::min<int64_t>((Np + 255) / 256, 2048);
+  k_absmax<<<blocks, 256, 0, s>>>(Np, w, pl.ws.wmax);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
   const unsigned* wmax = pl.ws.wmax;
-  if (gather || w) {
-    // records hold the basis alone (unit weights): the set's weights set the range
-    k_set_u32<<<1, 1, 0, s>>>(pl.ws.wmax, 0u);
-    DSHT_LAUNCH_CHECK();
-    const int blocks = (int)std::min<int64_t>((Np + 255) / 256, 2048);
-    k_absmax<<<blocks, 256, 0, s>>>(Np, w, pl.ws.wmax);
-    DSHT_LAUNCH_CHECK();
-  }
+  int rc;
+  // records hold the basis alone (unit weights): the set's weights set the range
+  if ((gather || w) && (rc = launch_weight_max(pl, Np, w, s))) return rc;
   if (pl.Nl <= 256) return launch_m1<8, 64>(pl, nchunks, w, gather, wmax, s);
   if (pl.Nl <= 512) return launch_m1<16, 64>(pl, nchunks, w, gather, wmax, s);
   return launch_m1<32, 128>(pl, nchunks, w, gather, wmax, s);

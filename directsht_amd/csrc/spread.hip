@@ -13,167 +13,320 @@
 // The modes are one-sided (0 <= m < Nl) and n >= 4 Nl, so every wanted
 // frequency is at most 1/4 per grid step and its aliases lie at 3/4 and
 // beyond, where psihat has decayed by e^-beta-ish: W = 8, beta = 2.3 W gives
-// ~2e-8 relative error, under the FP32 rounding of the grid.  The work per
-// point is 4 W grid updates instead of 4 Nl products.
+// ~2e-8 relative error, under the FP32 rounding of the transform.  The work
+// per point is 4 W grid updates instead of 4 Nl products.
 //
-// One workgroup (1024 threads) per chunk of sorted points; the chunk's runs
-// (stretches of one segment) are processed one after another in LDS.  The
-// four real grids travel as two complex ones, g0 + i g1 and g2 + i g3
-// ([2][n] float2), spread with LDS float atomics and transformed in place
-// (radix-4 decimation in frequency, one radix-2 stage first when log2 n is
-// odd; the transform is left digit-reversed and read where it lies), then
-// split into the four real-data transforms, scaled by 1/psihat and written
-// to part[run][k][m][2] (FP32), the layout the fold reads.
+// One workgroup per chunk of sorted points; the chunk's runs (stretches of
+// one segment) are processed one after another, each in LDS:
+//   * the grids are accumulated in 64-bit fixed point (units of 2^-40 of the
+//     weight scale) with LDS integer atomics.  Integer addition is
+//     associative, so the grid -- and everything after it -- is the same bit
+//     for bit whatever order the atomics land in; FP32 LDS atomics measured
+//     7x slower on this part, and their order (hence their rounding) would
+//     change from call to call;
+//   * the real grids g0..g3 travel as complex grids Z = g0 + i g1 and
+//     g2 + i g3 (or one of them per pass where n = 8192 leaves room for two
+//     grids only), transformed by radix-8 decimation-in-frequency passes
+//     (a radix-4 or radix-2 pass last when log2 n is not a multiple of 3),
+//     one butterfly per thread per radix-8 pass, the first pass reading the
+//     fixed-point grids and rounding them to FP32; the spectrum is left in
+//     digit-reversed order and read where it lies;
+//   * the four real-data transforms are split off Z(m) and Z(n - m), scaled
+//     by 1/psihat(m) and the weight scale and written to part[run][k][m][2]
+//     (FP32), the layout the fold reads.
 #include "common.h"
 #include "plan.h"
 
 namespace dsht {
 
-constexpr int kSpreadThreads = 1024;
+namespace {
 
-__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ inline float2 cmulf(float2 a, float2 b) {
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ __forceinline__ float2 c_mi(float2 a) { return make_float2(a.y, -a.x); }  // a * (-i)
 
-// In-place DIF transform of the two grids sg[0..n) and sg[n..2n); tw[k] =
-// e^{-2 pi i k / n}.  Frequency m ends up at position digit_rev(m).
-__device__ inline void fft_pair(float2* sg, int n, int logn, const float2* __restrict__ tw) {
-  const int tid = threadIdx.x;
-  int S = n;
-  if (logn & 1) {  // radix 2, span n
-    const int q = n >> 1;
-    for (int k = tid; k < n; k += kSpreadThreads) {  // n/2 butterflies per grid
-      const int g = k >= q;
-      const int j = k - g * q;
-      float2* x = sg + g * n + j;
-      const float2 a = x[0], b = x[q];
-      x[0] = cadd(a, b);
-      x[q] = cmulf(csub(a, b), tw[j]);
-    }
-    S = q;
-    __syncthreads();
-  }
-  for (; S >= 4; S >>= 2) {  // radix 4, span S
-    const int q = S >> 2, lq = __ffs(q) - 1;
-    const int ts = n / S;
-    const int nb = n >> 2;
-    for (int k = tid; k < 2 * nb; k += kSpreadThreads) {  // n/4 butterflies per grid
-      const int g = k >= nb;
-      const int bi = k - g * nb;
-      const int grp = bi >> lq, j = bi & (q - 1);
-      float2* x = sg + g * n + grp * S + j;
-      const float2 a = x[0], b = x[q], c = x[2 * q], d = x[3 * q];
-      const float2 s0 = cadd(a, c), d0 = csub(a, c), s1 = cadd(b, d), d1 = csub(b, d);
-      // y1 = (a - c) - i (b - d),  y3 = (a - c) + i (b - d)
-      const float2 y1 = make_float2(d0.x + d1.y, d0.y - d1.x);
-      const float2 y3 = make_float2(d0.x - d1.y, d0.y + d1.x);
-      x[0] = cadd(s0, s1);
-      x[q] = cmulf(y1, tw[j * ts]);
-      x[2 * q] = cmulf(csub(s0, s1), tw[2 * j * ts]);
-      x[3 * q] = cmulf(y3, tw[3 * j * ts]);
-    }
-    __syncthreads();
-  }
+// in-place DFT of size R (2, 4 or 8) of a[0..R): a[k] <- sum_t a[t] w_R^{kt},
+// w_R = e^{-2 pi i / R}
+template <int R>
+__device__ __forceinline__ void dft(float2 (&a)[8]);
+template <>
+__device__ __forceinline__ void dft<2>(float2 (&a)[8]) {
+  const float2 t = a[0];
+  a[0] = c_add(t, a[1]);
+  a[1] = c_sub(t, a[1]);
+}
+__device__ __forceinline__ void dft4(float2 x0, float2 x1, float2 x2, float2 x3, float2& X0,
+                                     float2& X1, float2& X2, float2& X3) {
+  const float2 s0 = c_add(x0, x2), d0 = c_sub(x0, x2);
+  const float2 s1 = c_add(x1, x3), d1 = c_mi(c_sub(x1, x3));
+  X0 = c_add(s0, s1);
+  X2 = c_sub(s0, s1);
+  X1 = c_add(d0, d1);
+  X3 = c_sub(d0, d1);
+}
+template <>
+__device__ __forceinline__ void dft<4>(float2 (&a)[8]) {
+  dft4(a[0], a[1], a[2], a[3], a[0], a[1], a[2], a[3]);
+}
+template <>
+__device__ __forceinline__ void dft<8>(float2 (&a)[8]) {
+  // even outputs: DFT4 of u_t = a_t + a_{t+4}; odd: DFT4 of v_t = (a_t - a_{t+4}) w8^t
+  constexpr float c = 0.70710678118654752f;
+  const float2 u0 = c_add(a[0], a[4]), u1 = c_add(a[1], a[5]);
+  const float2 u2 = c_add(a[2], a[6]), u3 = c_add(a[3], a[7]);
+  const float2 v0 = c_sub(a[0], a[4]);
+  const float2 e1 = c_sub(a[1], a[5]), e2 = c_sub(a[2], a[6]), e3 = c_sub(a[3], a[7]);
+  const float2 v1 = make_float2(c * (e1.x + e1.y), c * (e1.y - e1.x));   // * (1 - i)/sqrt2
+  const float2 v2 = c_mi(e2);                                            // * -i
+  const float2 v3 = make_float2(c * (e3.y - e3.x), -c * (e3.x + e3.y));  // * (-1 - i)/sqrt2
+  dft4(u0, u1, u2, u3, a[0], a[2], a[4], a[6]);
+  dft4(v0, v1, v2, v3, a[1], a[3], a[5], a[7]);
 }
 
-// position of frequency m after fft_pair: its digits (lowest first, radix 2
-// for the first stage when log2 n is odd, then radix 4) placed from the top
-__device__ inline int digit_rev(int m, int logn) {
-  int p = 0, span = 1 << logn;
-  if (logn & 1) {
-    span >>= 1;
-    p += (m & 1) * span;
-    m >>= 1;
+// FFT work arrays: one float2 of padding after every 32 and another after
+// every 64, which keeps every pass's and the spectrum read's LDS accesses
+// within 3 bank cycles per wavefront (2 is the floor for 8-byte accesses)
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 5) + (i >> 6); }
+
+// 64-bit fixed point: round(a b) for |a b| < 2^51, by one FMA onto the FP64
+// shift 1.5 * 2^52 (exact product, one rounding to an integer); the shift's
+// bits are 0x4338000000000000, whose low word is zero.
+__device__ __forceinline__ unsigned long long to_fixed(double a, double b) {
+  const double d = __builtin_fma(a, b, 0x1.8p52);
+  return (unsigned long long)(__double_as_longlong(d) - 0x4338000000000000LL);
+}
+
+}  // namespace
+
+// Grid geometry of the kernel for n = 2^LOGN: G real grids held per pass over
+// the run's points (4, or 2 where n = 8192), NCG = G / 2 complex grids, one
+// radix-8 butterfly per thread.
+template <int LOGN>
+struct SpreadGeom {
+  static constexpr int n = 1 << LOGN;
+  static constexpr int G = LOGN >= 13 ? 2 : 4;
+  static constexpr int NCG = G / 2;
+  static constexpr int kPasses8 = LOGN / 3;           // radix-8 passes
+  static constexpr int kRadixLast = 1 << (LOGN % 3);  // then one pass of 2 or 4 (1: none)
+  static constexpr int kThreads = NCG * n / 8;
+  static constexpr int kPad = n + n / 32 + n / 64;    // complex grid stride (float2)
+  // radix-8 passes with twiddles (span above 8), and how many of them keep
+  // each thread's twiddles in registers for the whole kernel (the butterfly
+  // a thread takes in a pass is the same in every run)
+  static constexpr int kTwPasses = kPasses8 - (kRadixLast == 1 ? 1 : 0);
+  static constexpr int kTwReg = kTwPasses < 3 ? kTwPasses : 3;
+  static_assert(kThreads >= 64 && kThreads <= 1024, "unsupported grid size");
+  static_assert((int64_t)G * n * 8 <= 128 * 1024, "grids exceed the LDS budget");
+  static_assert(NCG * kPad <= G * n, "the FFT's grids must fit in the spread grids' LDS");
+};
+
+// position of frequency f in the digit-reversed spectrum
+template <int LOGN>
+__device__ __forceinline__ int spec_pos(int f) {
+  using Gm = SpreadGeom<LOGN>;
+  int p = 0, span = Gm::n;
+#pragma unroll
+  for (int s = 0; s < Gm::kPasses8; ++s) {
+    span >>= 3;
+    p += (f & 7) * span;
+    f >>= 3;
   }
-  while (span > 1) {
-    span >>= 2;
-    p += (m & 3) * span;
-    m >>= 2;
-  }
+  if (Gm::kRadixLast > 1) p += f & (Gm::kRadixLast - 1);  // span 1
   return p;
 }
 
-// wmode: 0 the records' own weights, 1 w[sorted position], 2 w[rec.idx]
-template <int kW>
-__global__ __launch_bounds__(kSpreadThreads) void k_scatter_spread(
+// One in-place DIF pass of radix R and span S over the NCG complex grids
+// (butterflies bi = tid, tid + kThreads, ...).  The twiddles of radix-8 pass
+// P come from twr[P] (this thread's, loaded once) for P < kTwReg, else from
+// the table tw[k] = e^{-2 pi i k / n}.
+template <int LOGN, int R, int S, int P>
+__device__ __forceinline__ void fft_pass(float2* __restrict__ sz, const float2 (&twr)[3][7],
+                                         const float2* __restrict__ tw, int tid) {
+  using Gm = SpreadGeom<LOGN>;
+  constexpr int n = Gm::n, q = S / R, per = Gm::NCG * n / R / Gm::kThreads, ts = n / S;
+#pragma unroll
+  for (int k = 0; k < per; ++k) {
+    const int bi = tid + k * Gm::kThreads;
+    const int c = bi / (n / R), b = bi % (n / R);
+    const int g = b / q, j = b % q;
+    float2* base = sz + c * Gm::kPad;
+    const int i0 = g * S + j;
+    float2 a[8];
+#pragma unroll
+    for (int t = 0; t < R; ++t) a[t] = base[fpad(i0 + t * q)];
+    dft<R>(a);
+    if (S > R) {
+#pragma unroll
+      for (int t = 1; t < R; ++t)
+        a[t] = c_mul(a[t], P < Gm::kTwReg ? twr[P < 3 ? P : 0][t - 1] : tw[t * j * ts]);
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) base[fpad(i0 + t * q)] = a[t];
+  }
+}
+
+template <int LOGN, int P>
+__device__ __forceinline__ void fft_passes8(float2* sz, const float2 (&twr)[3][7],
+                                            const float2* tw, int tid) {
+  if constexpr (P < SpreadGeom<LOGN>::kPasses8) {
+    fft_pass<LOGN, 8, (SpreadGeom<LOGN>::n >> (3 * P)), P>(sz, twr, tw, tid);
+    __syncthreads();
+    fft_passes8<LOGN, P + 1>(sz, twr, tw, tid);
+  }
+}
+
+// wmode 0: the records' own weights, 1: w[sorted position], 2: w[rec.idx]
+template <int LOGN>
+__global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
     const PointRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg, int chunk,
-    int Nl, const int* __restrict__ run_base, const double* __restrict__ w, int wmode, int logn,
+    int Nl, const int* __restrict__ run_base, const double* __restrict__ w, int wmode,
     float beta, const float2* __restrict__ tw, const float* __restrict__ ipsi,
-    float* __restrict__ part, int* __restrict__ run_seg) {
-  extern __shared__ float2 sg[];  // [2][n]
-  const int n = 1 << logn;
+    float* __restrict__ part, int* __restrict__ run_seg, const unsigned* __restrict__ wmax) {
+  using Gm = SpreadGeom<LOGN>;
+  constexpr int n = Gm::n, G = Gm::G, NCG = Gm::NCG, kT = Gm::kThreads;
+  constexpr int kW = kSpreadW;
+  __shared__ __attribute__((aligned(16))) unsigned long long sgrid[G * n];
+  float2* sz = reinterpret_cast<float2*>(sgrid);  // the FFT's grids, after the spread
   const int tid = threadIdx.x;
   const int c = blockIdx.x;
   const int nvalid = seg_start[nseg];
   const int b = c * chunk, e = min(nvalid, b + chunk);
   if (b >= e) return;
+  // this thread's twiddles of the first kTwReg radix-8 passes: pass P has
+  // span S = n / 8^P and takes butterfly j = (tid mod n/8) mod S/8
+  float2 twr[3][7];
+#pragma unroll
+  for (int P = 0; P < 3; ++P) {
+    if (P < Gm::kTwReg) {
+      const int S = n >> (3 * P), j = (tid % (n / 8)) % (S / 8);
+#pragma unroll
+      for (int t = 1; t < 8; ++t) twr[P][t - 1] = tw[t * j * (n / S)];
+    }
+  }
+
+  // weight scale: 2^-ex brings max |w| below 1, so every fixed-point
+  // contribution stays below 2^40 and a cell's sum below 2^54; undone exactly
+  // at the output together with the 2^-40 of the fixed point
+  int ex = 0;
+  {
+    const float mx = __uint_as_float(wmax[0]);
+    if (mx > 0.f && mx <= 3.0e38f) (void)frexpf(mx, &ex);
+    ex = max(-120, min(120, ex));
+  }
+  const double w_fix = ldexp(1.0, 40 - ex);
+  const float w_out = ldexpf(1.0f, ex);
+
   int run = run_base[c];
   for (int p = b; p < e;) {  // uniform: one run per iteration
     const int cur = rec[p].seg;
     const int pe = min(e, seg_start[cur + 1]);
-    for (int i = tid; i < 2 * n; i += kSpreadThreads) sg[i] = make_float2(0.f, 0.f);
-    __syncthreads();
-    for (int q = p + tid; q < pe; q += kSpreadThreads) {
-      const PointRec& r = rec[q];
-      const double wt = wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[r.idx]);
-      const float h0 = (float)(wt * r.h0), h1 = (float)(wt * r.h1);
-      const float h2 = (float)(wt * r.h2), h3 = (float)(wt * r.h3);
-      const double t = r.u * (double)n;
-      const int j0 = (int)ceil(t - 0.5 * kW);
-      const float z0 = (float)((double)j0 - t);  // in [-W/2, -W/2 + 1)
+#pragma unroll 1
+    for (int pass = 0; pass < 4 / G; ++pass) {
+      for (int i = tid; i < G * n / 2; i += kT)
+        reinterpret_cast<ulonglong2*>(sgrid)[i] = make_ulonglong2(0ull, 0ull);
+      __syncthreads();
+      for (int q = p + tid; q < pe; q += kT) {
+        const PointRec& r = rec[q];
+        const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[r.idx])) * w_fix;
+        double a[G];
 #pragma unroll
-      for (int i = 0; i < kW; ++i) {
-        const float z = (z0 + (float)i) * (2.0f / kW);
-        const float psi = expf(beta * (sqrtf(fmaxf(0.f, 1.f - z * z)) - 1.f));
-        const int x = (j0 + i) & (n - 1);
-        atomicAdd(&sg[x].x, h0 * psi);
-        atomicAdd(&sg[x].y, h1 * psi);
-        atomicAdd(&sg[n + x].x, h2 * psi);
-        atomicAdd(&sg[n + x].y, h3 * psi);
-      }
-    }
-    __syncthreads();
-    fft_pair(sg, n, logn, tw);
-    // Z = G_a + i G_b for real grids a, b:  G_a(m) = (Z(m) + conj Z(n-m)) / 2,
-    // G_b(m) = (Z(m) - conj Z(n-m)) / 2i
-    for (int m = tid; m < Nl; m += kSpreadThreads) {
-      const int pa = digit_rev(m, logn), pb = digit_rev((n - m) & (n - 1), logn);
-      const float s = 0.5f * ipsi[m];
+        for (int g = 0; g < G; ++g) a[g] = wt * (&r.h0)[pass * G + g];
+        const double t = r.u * (double)n;
+        const double jf = ceil(t - 0.5 * kW);
+        const int j0 = (int)jf;
+        const float z0 = (float)(jf - t);  // in [-W/2, -W/2 + 1)
 #pragma unroll
-      for (int g = 0; g < 2; ++g) {
-        const float2 z = sg[g * n + pa], v = sg[g * n + pb];
-        const float2 ga = make_float2((z.x + v.x) * s, (z.y - v.y) * s);
-        const float2 gb = make_float2((z.y + v.y) * s, (v.x - z.x) * s);
-        *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + 2 * g) * Nl + m) * 2) = ga;
-        *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + 2 * g + 1) * Nl + m) * 2) = gb;
+        for (int i = 0; i < kW; ++i) {
+          const float z = (z0 + (float)i) * (2.0f / kW);
+          const double psi = (double)expf(beta * (sqrtf(fmaxf(0.f, 1.f - z * z)) - 1.f));
+          const int x = (j0 + i) & (n - 1);
+#pragma unroll
+          for (int g = 0; g < G; ++g) atomicAdd(&sgrid[g * n + x], to_fixed(a[g], psi));
+        }
       }
+      __syncthreads();
+
+      // first radix-8 pass straight from the fixed-point grids (span n)
+      {
+        constexpr int q = n / 8;
+        const int cg = tid / q, j = tid % q;
+        float2 v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int x = j + t * q;
+          const long long re = (long long)sgrid[(2 * cg) * n + x];
+          const long long im = (long long)sgrid[(2 * cg + 1) * n + x];
+          v[t] = make_float2((float)((double)re * 0x1p-40), (float)((double)im * 0x1p-40));
+        }
+        __syncthreads();  // every fixed-point value is read before sz overwrites it
+        dft<8>(v);
+#pragma unroll
+        for (int t = 1; t < 8; ++t) v[t] = c_mul(v[t], twr[0][t - 1]);
+        float2* base = sz + cg * Gm::kPad;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) base[fpad(j + t * q)] = v[t];
+      }
+      __syncthreads();
+      fft_passes8<LOGN, 1>(sz, twr, tw, tid);
+      if constexpr (Gm::kRadixLast > 1) {
+        fft_pass<LOGN, Gm::kRadixLast, Gm::kRadixLast, Gm::kPasses8>(sz, twr, tw, tid);
+        __syncthreads();
+      }
+
+      // Z = G_a + i G_b for real grids a, b:  G_a(m) = (Z(m) + conj Z(n-m)) / 2,
+      // G_b(m) = (Z(m) - conj Z(n-m)) / 2i
+      for (int m = tid; m < Nl; m += kT) {
+        const int pa = fpad(spec_pos<LOGN>(m)), pb = fpad(spec_pos<LOGN>((n - m) & (n - 1)));
+        const float s = 0.5f * ipsi[m] * w_out;
+#pragma unroll
+        for (int cg = 0; cg < NCG; ++cg) {
+          const float2 z = sz[cg * Gm::kPad + pa], v = sz[cg * Gm::kPad + pb];
+          const float2 ga = make_float2((z.x + v.x) * s, (z.y - v.y) * s);
+          const float2 gb = make_float2((z.y + v.y) * s, (v.x - z.x) * s);
+          const int k = pass * G + 2 * cg;
+          *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + k) * Nl + m) * 2) = ga;
+          *reinterpret_cast<float2*>(part + (((int64_t)run * 4 + k + 1) * Nl + m) * 2) = gb;
+        }
+      }
+      __syncthreads();  // the spectrum is read before the next grids are cleared
     }
     if (tid == 0) run_seg[run] = cur;
     ++run;
     p = pe;
-    __syncthreads();  // the grids are read before the next run clears them
   }
+}
+
+template <int LOGN>
+static int launch_spread_n(Plan& pl, int nchunks, const double* w, int wmode, const unsigned* wmax,
+                           hipStream_t s) {
+  using Gm = SpreadGeom<LOGN>;
+  k_scatter_spread<LOGN><<<nchunks, Gm::kThreads, 0, s>>>(
+      pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl, pl.ws.run_base, w, wmode,
+      (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.part, pl.ws.run_seg, wmax);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
 }
 
 int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
-  const int n = 1 << pl.spread_logn;
-  const size_t lds = (size_t)2 * n * sizeof(float2);
-  static bool attr = false;  // allow up to 128 KB of dynamic LDS (n = 8192)
-  if (!attr) {
-    DSHT_HIP(hipFuncSetAttribute((const void*)k_scatter_spread<kSpreadW>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 8192 * 8));
-    attr = true;
+  int rc;
+  if (wmode != 0 && (rc = launch_weight_max(pl, Np, w, s))) return rc;
+  const unsigned* wmax = pl.ws.wmax;
+  switch (pl.spread_logn) {
+    case 8: return launch_spread_n<8>(pl, nchunks, w, wmode, wmax, s);
+    case 9: return launch_spread_n<9>(pl, nchunks, w, wmode, wmax, s);
+    case 10: return launch_spread_n<10>(pl, nchunks, w, wmode, wmax, s);
+    case 11: return launch_spread_n<11>(pl, nchunks, w, wmode, wmax, s);
+    case 12: return launch_spread_n<12>(pl, nchunks, w, wmode, wmax, s);
+    case 13: return launch_spread_n<13>(pl, nchunks, w, wmode, wmax, s);
+    default:
+      set_error("dsht: the spreading scatter supports Nl <= " + std::to_string(kSpreadMaxNl));
+      return DSHT_EINVAL;
   }
-  k_scatter_spread<kSpreadW><<<nchunks, kSpreadThreads, lds, s>>>(
-      pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl, pl.ws.run_base, w, wmode,
-      pl.spread_logn, (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.part,
-      pl.ws.run_seg);
-  DSHT_LAUNCH_CHECK();
-  return DSHT_OK;
 }
 
 }  // namespace dsht
