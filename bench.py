@@ -573,7 +573,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--skip-batched", action="store_true", help="skip the batched workloads")
-    ap.add_argument("--scatter", choices=("matrix", "vector", "spread"), default=None,
+    ap.add_argument("--scatter", choices=("auto", "matrix", "vector", "spread"), default=None,
                     help="scatter engine of the plain workload (default: the plan's)")
     ap.add_argument("--chunk", type=int, default=None,
                     help="sorted points per scatter work item of the plain workload")
@@ -645,30 +645,56 @@ def main():
     stage_avg = {k: v / max(ntimed, 1) for k, v in stage_ms.items()}
     log(f"{ms:.3f} ms/step; stages {stage_avg}")
 
-    # Dominant kernel: the matrix-core scatter.  Algorithmic work per
-    # (point, order m): 22 flop -- the four complex multiply-adds
-    # w h_k(t) e^{-i m phi} into the accumulators (16) and the complex step of
-    # e^{-i m phi} (6) -- so 22 N Nl per transform, priced against the dense
-    # fp16 matrix peak the kernel runs on.  The kernel's instruction count is
-    # higher: the real GEMM [(k, m1, re|im) x point] . [point x (re|im, m2)]
-    # with m = 1024 mt + 32 m1 + m2 costs 32 flop per (point, m), three times
-    # for the fp16 hi*hi, hi*lo, lo*hi products = 96 MFMA flop, reported
-    # beside it.
-    scat_s = stage_avg["scatter"] * 1e-3
-    alg_tf = 22.0 * N * Nl / scat_s / 1e12
-    mfma_tf = 96.0 * N * Nl / scat_s / 1e12
-    # contraction stage (k_contract_y<float> + k_add_der): algorithmic bytes =
-    # the FP32 Y-only table, one float per (l, m, node) over the 2 ceil(Nx/4)
-    # node pairs of the layout
+    # Dominant kernel: the table contraction (k_contract_y<float> +
+    # k_add_der), HBM-bound: algorithmic bytes = the FP32 Y-only table, one
+    # float per (l, m, node) over the 2 ceil(Nx/4) node pairs of the layout,
+    # read once per transform.
     npair = 2 * ((Nx + 3) // 4)
     tab_bytes = 2.0 * Nlm * npair * 4
     con_gbs = tab_bytes / (stage_avg["contract"] * 1e-3) / 1e9
+    # The scatter (second largest stage).  The reference's algorithm does
+    # 22 flop per (point, order m) -- four complex multiply-adds
+    # w h_k(t) e^{-i m phi} into the accumulators (16) and the complex step
+    # of e^{-i m phi} (6) -- 22 N Nl per transform.  The matrix engine
+    # executes that form on the fp16 matrix cores (the real GEMM
+    # [(k, m1, re|im) x point] . [point x (re|im, m2)], 32 flop per (point,
+    # m), three times for the fp16 hi/lo split = 96 MFMA flop).  The
+    # spreading engine computes the same sums as a non-uniform FFT per run
+    # (32 fixed-point grid updates per point and one FFT of four grids per
+    # run), so 22 N Nl / t is only its direct-sum-equivalent rate; its own
+    # bound is LDS (integer atomics, FFT exchanges) and its HBM bytes are the
+    # 64-byte sorted records in and the run partials out.
+    scat_s = stage_avg["scatter"] * 1e-3
+    engine = sht.scatter_engine(N)
     default_workload = (N == 10_000_000)
     traffic = read_traffic() if default_workload else {}
 
     def tr(name):
         t = traffic.get(name)
         return None if t is None else t[0] + t[1]
+
+    alg_tf = 22.0 * N * Nl / scat_s / 1e12
+    if engine == "matrix":
+        mfma_tf = 96.0 * N * Nl / scat_s / 1e12
+        roof_scatter = {
+            "kernel": "k_scatter_mfma", "engine": "matrix (MFMA fp16 x3 split)", "bound": "mfma",
+            "achieved": alg_tf, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+            "frac": alg_tf / PEAK_FP16_TFLOPS, "traffic": tr("k_scatter_mfma"),
+            "algorithmic": "22 flop per (point, m); 22 N Nl per transform",
+            "mfma_instruction_tflops": mfma_tf, "mfma_instruction_frac": mfma_tf / PEAK_FP16_TFLOPS,
+            "mfma_instruction_flop": "96 per (point, m): 32-flop real GEMM form, x3 for the "
+                                     "fp16 hi/lo split"}
+    else:
+        runs_bytes = tr("k_scatter_spread")
+        rec_bytes = 64.0 * N
+        roof_scatter = {
+            "kernel": "k_scatter_spread", "engine": engine, "bound": "lds",
+            "direct_sum_equivalent_tflops": alg_tf,
+            "direct_sum_equivalent": "22 N Nl / t: the reference algorithm's flop count, "
+                                     "not flop executed (non-uniform FFT per run)",
+            "hbm_GBs_records_in": rec_bytes / scat_s / 1e9,
+            "hbm_frac_records_in": rec_bytes / scat_s / 1e9 / PEAK_HBM_GBS,
+            "traffic": runs_bytes}
 
     result = {
         "metric": METRIC,
@@ -689,21 +715,12 @@ def main():
                    "Nl": Nl, "Nx": Nx, "npoints": N, "parallelism": "single GPU"},
         "alm_per_s": Nlm / (ms * 1e-3),
         "stage_ms": stage_avg,
-        "roofline": {"kernel": "k_scatter_mfma", "bound": "mfma", "engine": "MFMA fp16 (x3 split)",
-                     "achieved": alg_tf, "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
-                     "frac": alg_tf / PEAK_FP16_TFLOPS, "traffic": tr("k_scatter_mfma"),
-                     "algorithmic": "22 flop per (point, m): 4 complex multiply-adds + the "
-                                    "complex step of e^{-i m phi}; 22 N Nl per transform",
-                     "mfma_instruction_tflops": mfma_tf,
-                     "mfma_instruction_frac": mfma_tf / PEAK_FP16_TFLOPS,
-                     "mfma_instruction_flop": "96 per (point, m): 32-flop real GEMM form, "
-                                              "x3 for the fp16 hi/lo split",
-                     "fp32_vector_frac": alg_tf / PEAK_FP32_TFLOPS},
-        "roofline_contract": {"kernel": "k_contract_y<float> + k_add_der", "bound": "hbm",
-                              "achieved": con_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                              "frac": con_gbs / PEAK_HBM_GBS, "traffic": tr("k_contract_y"),
-                              "algorithmic": "FP32 Y-only table Nlm*2*Npair*4 B "
-                                             "(Npair = 2 ceil(Nx/4))"},
+        "roofline": {"kernel": "k_contract_y<float> + k_add_der", "bound": "hbm",
+                     "achieved": con_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": con_gbs / PEAK_HBM_GBS, "traffic": tr("k_contract_y"),
+                     "algorithmic": "FP32 Y-only table Nlm*2*Npair*4 B (Npair = 2 ceil(Nx/4)), "
+                                    "read once per transform"},
+        "roofline_scatter": roof_scatter,
     }
 
     if args.copy_calibration:

@@ -21,7 +21,7 @@ DSHT_MODE_CALL, DSHT_MODE_BASIC = 0, 1
 DSHT_STATUS_EQU_RANGE, DSHT_STATUS_POLAR_RANGE = 1, 2
 STAGES = ("points", "scatter", "fold", "contract", "rotate")
 DSHT_OPT_SCATTER, DSHT_OPT_CHUNK, DSHT_OPT_CONTRACT = 1, 2, 3
-DSHT_SCATTER_MATRIX, DSHT_SCATTER_VECTOR, DSHT_SCATTER_SPREAD = 0, 1, 2
+DSHT_SCATTER_MATRIX, DSHT_SCATTER_VECTOR, DSHT_SCATTER_SPREAD, DSHT_SCATTER_AUTO = 0, 1, 2, 3
 DSHT_CONTRACT_MATRIX, DSHT_CONTRACT_VECTOR = 0, 1
 MAX_SETS = 64  # sets per batched call (dsht_transform_sets / dsht_transform_many)
 
@@ -51,6 +51,7 @@ SIGNATURES = {
     "dsht_transform_many": (ct.c_int, [_vp, ct.c_int, ct.POINTER(ct.c_int64), ct.POINTER(_vp),
                                        ct.POINTER(_vp), ct.POINTER(_vp), _vp, _vp, ct.c_int, _vp]),
     "dsht_plan_configure": (ct.c_int, [_vp, ct.c_int, ct.c_int]),
+    "dsht_plan_engine": (ct.c_int, [_vp, ct.c_int64, _vp]),
     "dsht_plan_set_timing": (ct.c_int, [_vp, ct.c_int]),
     "dsht_plan_get_timing": (ct.c_int, [_vp, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int64)]),
     "dsht_plan_debug_rhs": (ct.c_int, [_vp, _vp, ct.c_int64]),

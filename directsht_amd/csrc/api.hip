@@ -270,7 +270,8 @@ static int many_group_cap(const Plan& pl) {
 }
 
 static bool uses_many(const Plan& pl, int nsets, int mode) {
-  return use_mfma_contract(pl, nsets) && pl.scatter == DSHT_SCATTER_MATRIX &&
+  return use_mfma_contract(pl, nsets) &&
+         (pl.scatter == DSHT_SCATTER_MATRIX || pl.scatter == DSHT_SCATTER_AUTO) &&
          mode != DSHT_MODE_CSHT && sizeof(unsigned short) * (pl.nseg + 1) <= kManyLdsMax &&
          many_group_cap(pl) > 1;
 }
@@ -623,8 +624,12 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     }
     maxNp = std::max(maxNp, n);
   }
+  // the scatter engine: of the shared points for weight sets, the matrix
+  // engine for grouped point sets, per set otherwise (set again below)
+  pl.scatter_now = pick_scatter(pl, a.shared ? a.Np[0] : maxNp);
   const bool sets_scatter = a.shared && uses_sets_scatter(pl, a.nsets);
   const bool many = !a.shared && uses_many(pl, a.nsets, mode);
+  if (many) pl.scatter_now = DSHT_SCATTER_MATRIX;
   std::vector<ManySets> groups;
   int rc = DSHT_OK;
   if (many) {
@@ -728,6 +733,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   } else {
     for (int set = 0; set < a.nsets; ++set) {
       const int64_t Np = a.Np[set];
+      pl.scatter_now = pick_scatter(pl, Np);
       pl.chunk = work_chunk(chunk_cap(pl), Np);
       ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
       if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
@@ -777,6 +783,22 @@ extern "C" {
 
 const char* dsht_last_error(void) { return g_err.c_str(); }
 
+int dsht_plan_engine(const dsht_plan* p, int64_t Np, int* engine) {
+  if (!p || !engine || Np < 0) {
+    set_error("dsht_plan_engine: null argument or Np < 0");
+    return DSHT_EINVAL;
+  }
+  const Plan& pl = p->pl;
+  if (pl.precision != DSHT_FP32) {
+    *engine = DSHT_SCATTER_VECTOR;
+    return DSHT_OK;
+  }
+  int e = pick_scatter(pl, Np);
+  if (e == DSHT_SCATTER_SPREAD && pl.Nl > kSpreadMaxNl) e = DSHT_SCATTER_MATRIX;
+  *engine = e;
+  return DSHT_OK;
+}
+
 int dsht_plan_configure(dsht_plan* p, int option, int value) {
   if (!p) {
     set_error("dsht_plan_configure: null plan");
@@ -785,7 +807,7 @@ int dsht_plan_configure(dsht_plan* p, int option, int value) {
   Plan& pl = p->pl;
   if (option == DSHT_OPT_SCATTER) {
     if (value != DSHT_SCATTER_MATRIX && value != DSHT_SCATTER_VECTOR &&
-        value != DSHT_SCATTER_SPREAD) {
+        value != DSHT_SCATTER_SPREAD && value != DSHT_SCATTER_AUTO) {
       set_error("dsht_plan_configure: unknown scatter engine");
       return DSHT_EINVAL;
     }
@@ -1041,9 +1063,13 @@ int dsht_plan_reserve_for(dsht_plan* p, int64_t Np, int nsets, int kinds) {
   DeviceGuard g(p->pl.device);
   Plan& pl = p->pl;
   int rc = DSHT_OK;
-  if (kinds & DSHT_RESERVE_WEIGHT_SETS)
-    rc = ensure_workspace(pl, Np, nsets, nsets > 1 && !scatter_gathers(pl),
-                          uses_sets_scatter(pl, nsets) ? nsets : 1);
+  if (kinds & DSHT_RESERVE_WEIGHT_SETS) {
+    // room for either engine a transform of up to Np points may take
+    pl.scatter_now = pick_scatter(pl, 0);
+    const bool sets = uses_sets_scatter(pl, nsets);
+    const bool wsort = nsets > 1 && !scatter_gathers(pl);
+    rc = ensure_workspace(pl, Np, nsets, wsort, sets ? nsets : 1);
+  }
   if (rc == DSHT_OK && (kinds & DSHT_RESERVE_POINT_SETS)) {
     rc = ensure_workspace(pl, Np, nsets);
     // point-set batches (dsht_transform_many) sort groups of sets together

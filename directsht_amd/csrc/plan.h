@@ -141,7 +141,8 @@ struct Plan {
   bool chunk_explicit = false;  // chunk_max set by dsht_plan_configure
   int chunk = 2048;      // points per scatter work item of the transform being
                          // enqueued: work_chunk(chunk_max, Np)
-  int scatter = DSHT_SCATTER_MATRIX;  // engine of FP32 plans
+  int scatter = DSHT_SCATTER_AUTO;       // configured scatter engine of FP32 plans
+  int scatter_now = DSHT_SCATTER_MATRIX;  // engine of the transform being enqueued
   int contract = DSHT_CONTRACT_MATRIX;  // engine of batched FP32 contractions
   // contraction layout
   int ngroups = 0;
@@ -225,7 +226,19 @@ int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipS
 // the spreading scatter serves FP32 plans up to kSpreadMaxNl; above it the
 // matrix engine takes its place
 inline bool uses_spread(const Plan& pl) {
-  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_SPREAD && pl.Nl <= kSpreadMaxNl;
+  return pl.precision == DSHT_FP32 && pl.scatter_now == DSHT_SCATTER_SPREAD &&
+         pl.Nl <= kSpreadMaxNl;
+}
+// DSHT_SCATTER_AUTO: the spreading scatter costs one transform of four
+// grids per run, the matrix engine work proportional to points x orders, so
+// the spreading scatter is taken where a transform has at least
+// kSpreadMinPerSeg points per segment on average (measured break-even:
+// 170-430 points per segment for Nl 128-1024, uniform points).
+constexpr int kSpreadMinPerSeg = 400;
+inline int pick_scatter(const Plan& pl, int64_t Np) {
+  if (pl.scatter != DSHT_SCATTER_AUTO) return pl.scatter;
+  return pl.Nl <= kSpreadMaxNl && Np >= (int64_t)kSpreadMinPerSeg * pl.nseg ? DSHT_SCATTER_SPREAD
+                                                                            : DSHT_SCATTER_MATRIX;
 }
 // Work items of the spreading scatter hold up to kSpreadChunk points unless a
 // chunk is configured: each run (a stretch of one segment inside an item)
@@ -243,7 +256,7 @@ int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipS
 // the plan's scatter gathers weight sets itself (no launch_gather_weights)
 inline bool scatter_gathers(const Plan& pl) {
   return pl.precision == DSHT_FP32 &&
-         (pl.scatter == DSHT_SCATTER_MATRIX || pl.scatter == DSHT_SCATTER_SPREAD);
+         (pl.scatter_now == DSHT_SCATTER_MATRIX || pl.scatter_now == DSHT_SCATTER_SPREAD);
 }
 // Points per scatter work item of a transform of Np points: the configured
 // chunk, or less for a small transform, so that it still spreads over about
@@ -269,7 +282,8 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
 constexpr int kSetsScatterMin = 2;
 constexpr int kSetsChunk = 16384;  // largest work item of the weight-set scatter
 inline bool uses_sets_scatter(const Plan& pl, int nsets) {
-  return pl.precision == DSHT_FP32 && pl.scatter == DSHT_SCATTER_MATRIX && nsets >= kSetsScatterMin;
+  return pl.precision == DSHT_FP32 && pl.scatter_now == DSHT_SCATTER_MATRIX &&
+         nsets >= kSetsScatterMin;
 }
 inline int64_t part_set_stride(const Plan& pl) { return pl.ws.cap_set_runs * (int64_t)pl.Nl * 8; }
 // w: nsets rows of ldw weights in the caller's input order

@@ -276,7 +276,7 @@ int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s) {
   if (nchunks == 0) return DSHT_OK;
   if (uses_spread(pl)) return launch_scatter_spread(pl, Np, wsort, wsort ? 1 : 0, s);
   if (pl.precision == DSHT_FP32 &&
-      (pl.scatter == DSHT_SCATTER_MATRIX || pl.scatter == DSHT_SCATTER_SPREAD))
+      (pl.scatter_now == DSHT_SCATTER_MATRIX || pl.scatter_now == DSHT_SCATTER_SPREAD))
     return launch_scatter_mfma(pl, Np, wsort, false, s);
   if (pl.precision == DSHT_FP32) {
     constexpr int K = 8;

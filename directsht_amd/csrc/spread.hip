@@ -99,6 +99,15 @@ __device__ __forceinline__ unsigned long long to_fixed(double a, double b) {
   return (unsigned long long)(__double_as_longlong(d) - 0x4338000000000000LL);
 }
 
+// v 2^-40 in FP32 for a 64-bit fixed-point sum |v| < 2^54: the high word
+// (|hi| < 2^22) converts exactly, so the only roundings are the low word's
+// (below 2^-32 absolute) and the final FMA's
+__device__ __forceinline__ float from_fixed(unsigned long long v) {
+  const int hi = (int)(v >> 32);
+  const unsigned lo = (unsigned)v;
+  return __builtin_fmaf((float)hi, 0x1p-8f, (float)lo * 0x1p-40f);
+}
+
 }  // namespace
 
 // Grid geometry of the kernel for n = 2^LOGN: G real grids held per pass over
@@ -257,9 +266,8 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
           const int x = j + t * q;
-          const long long re = (long long)sgrid[(2 * cg) * n + x];
-          const long long im = (long long)sgrid[(2 * cg + 1) * n + x];
-          v[t] = make_float2((float)((double)re * 0x1p-40), (float)((double)im * 0x1p-40));
+          v[t] = make_float2(from_fixed(sgrid[(2 * cg) * n + x]),
+                             from_fixed(sgrid[(2 * cg + 1) * n + x]));
         }
         __syncthreads();  // every fixed-point value is read before sz overwrites it
         dft<8>(v);

@@ -214,8 +214,10 @@ class DirectSHT:
 
     # ---- tuning -----------------------------------------------------------
     def configure(self, scatter=None, chunk=None, contract=None):
-        """Select the scatter engine of FP32 plans ('matrix': fp16x3 MFMA
-        direct sums, 'vector': FP32 VALU direct sums, 'spread': per-run
+        """Select the scatter engine of FP32 plans ('auto', the default: per
+        transform, 'spread' where the points average at least 400 per
+        segment of the grid and Nl <= 2048, else 'matrix'; 'matrix': fp16x3
+        MFMA direct sums, 'vector': FP32 VALU direct sums, 'spread': per-run
         non-uniform FFT, ~2e-8 relative), the sorted points per scatter work
         item (the largest; small transforms use smaller items), and the
         table contraction of batched FP32 transforms ('matrix': fp16x3 MFMA
@@ -227,10 +229,19 @@ class DirectSHT:
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_CONTRACT, v), "configure")
         if scatter is not None:
             v = {"matrix": _lib.DSHT_SCATTER_MATRIX, "vector": _lib.DSHT_SCATTER_VECTOR,
-                 "spread": _lib.DSHT_SCATTER_SPREAD}[scatter]
+                 "spread": _lib.DSHT_SCATTER_SPREAD, "auto": _lib.DSHT_SCATTER_AUTO}[scatter]
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_SCATTER, v), "configure")
         if chunk is not None:
             _lib.check(lib.dsht_plan_configure(self._plan, _lib.DSHT_OPT_CHUNK, int(chunk)), "configure")
+
+    def scatter_engine(self, npoints):
+        """The scatter engine ('matrix', 'vector' or 'spread') a transform of
+        npoints points takes on this plan."""
+        e = ct.c_int()
+        _lib.check(_lib.load().dsht_plan_engine(self._plan, int(npoints), ct.byref(e)),
+                   "dsht_plan_engine")
+        return {_lib.DSHT_SCATTER_MATRIX: "matrix", _lib.DSHT_SCATTER_VECTOR: "vector",
+                _lib.DSHT_SCATTER_SPREAD: "spread"}[e.value]
 
     # ---- stage timing (HIP events on the transform's stream) -------------
     def set_timing(self, enable=True):

@@ -200,14 +200,19 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
 /* Tuning options (dsht_plan_configure).  Results do not depend on them beyond
  * rounding.
  *   DSHT_OPT_SCATTER  scatter engine of DSHT_FP32 plans:
- *                     DSHT_SCATTER_MATRIX (default) fp16x3 split-precision
- *                     MFMA GEMM, DSHT_SCATTER_VECTOR FP32 VALU recurrence,
- *                     or DSHT_SCATTER_SPREAD: each run's sums as a type-1
+ *                     DSHT_SCATTER_AUTO (default) picks per transform: the
+ *                     spreading engine where the points average at least 400
+ *                     per segment of the (Nx - 1)-bin grid (4 (Nx - 1)
+ *                     segments) and Nl <= 2048, else the matrix engine;
+ *                     DSHT_SCATTER_MATRIX fp16x3 split-precision MFMA GEMM;
+ *                     DSHT_SCATTER_VECTOR FP32 VALU recurrence;
+ *                     DSHT_SCATTER_SPREAD: each run's sums as a type-1
  *                     non-uniform FFT (kernel spreading onto a 4x oversampled
- *                     phi grid, FFT, deconvolution; ~2e-8 relative).
+ *                     phi grid in 64-bit fixed point, FFT, deconvolution;
+ *                     ~2e-8 relative; Nl <= 2048, the matrix engine above).
  *                     DSHT_FP64 plans always use the FP64 vector scatter.
  *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536; default
- *                     2048).  A transform of fewer than 1024 x chunk points
+ *                     2048, and 8192 for the spreading engine).  A transform of fewer than 1024 x chunk points
  *                     uses smaller items (about Np / 1024, at least 64) so
  *                     that it still spreads over the whole device.
  *   DSHT_OPT_CONTRACT table contraction of batched DSHT_FP32 transforms with
@@ -220,9 +225,17 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
 #define DSHT_SCATTER_MATRIX 0
 #define DSHT_SCATTER_VECTOR 1
 #define DSHT_SCATTER_SPREAD 2
+#define DSHT_SCATTER_AUTO 3
 #define DSHT_CONTRACT_MATRIX 0
 #define DSHT_CONTRACT_VECTOR 1
 int dsht_plan_configure(dsht_plan* plan, int option, int value);
+/* The scatter engine (DSHT_SCATTER_*) a single-set transform of Np points
+ * takes on this plan: the configured one, DSHT_SCATTER_AUTO resolved by the
+ * rule above, DSHT_SCATTER_MATRIX for DSHT_SCATTER_SPREAD above Nl = 2048,
+ * and DSHT_SCATTER_VECTOR for DSHT_FP64 plans.  No counterpart in the
+ * reference (its numpy path has one scatter); for callers that report or
+ * tune per engine. */
+int dsht_plan_engine(const dsht_plan* plan, int64_t Np, int* engine);
 
 /* Stage timing.  While enabled, every dsht_transform records HIP events on
  * its stream around its five stages: 0 point binning + counting sort,
