@@ -117,9 +117,23 @@ struct alignas(64) PointRec {
                            // the basis alone (w = 1) for weight-set transforms
   int seg;                 // segment id
   int idx;                 // the point's position in the caller's input arrays
-  double u;                // phi / 2pi in [0, 1), phi in the point's branch frame
-                           // (written for the spreading scatter only)
+  int pad[2];              // fills the line (written as zero)
 };
 static_assert(sizeof(PointRec) == 64, "PointRec must be one 64-byte line");
+
+// Per-point record of the spreading scatter, half a line: where the point's
+// kernel support starts on the plan's periodic grid of n cells over phi (the
+// n u - W/2 of the spreading, u = phi / 2pi, split into an integer cell and
+// an FP32 offset in FP64, so the scatter needs no FP64 position) and the
+// Hermite weights rounded to FP32.  Written in place of PointRec into the
+// same buffer when a transform uses the spreading scatter.
+struct alignas(32) SpreadRec {
+  float a0, a1, a2, a3;  // w * Hermite basis at t (the basis alone for weight sets)
+  int seg;               // segment id
+  int idx;               // the point's position in the caller's input arrays
+  int j0;                // first cell of the support: ceil(n u - W/2), in [-W/2, n - W/2]
+  float z0;              // j0 - n u, in [-W/2, -W/2 + 1)
+};
+static_assert(sizeof(SpreadRec) == 32, "SpreadRec must be 32 bytes");
 
 }  // namespace dsht

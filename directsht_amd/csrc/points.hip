@@ -35,7 +35,7 @@ namespace dsht {
 
 struct PointParams {
   int Nx, nseg, mode;
-  int want_u;  // also record u = phi / 2pi (spreading scatter)
+  int spread_n;  // grid cells of the spreading scatter (SpreadRec records), or 0 (PointRec)
   double xmax, dx, acos_xmax, acos_mxmax;
 };
 
@@ -44,7 +44,8 @@ struct Classified {
   int polar;     // 1 if the point went through the rotated branch
   int flag;      // DSHT_STATUS_* bits raised by this point
   double ere, eim, h0, h1, h2, h3;
-  double u;      // phi / 2pi in [0, 1) (when PointParams.want_u)
+  int j0;        // spreading support: first cell and offset (when PointParams.spread_n)
+  float z0;
 };
 
 __device__ inline Classified classify(double theta, double phi, double w, const PointParams& p) {
@@ -55,7 +56,8 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
   c.flag = 0;
   c.ere = 1.0; c.eim = 0.0;
   c.h0 = c.h1 = c.h2 = c.h3 = 0.0;
-  c.u = 0.0;
+  c.j0 = 0;
+  c.z0 = 0.f;
   double st, ct;
   double xu, sp, cp;
   int branch;
@@ -99,12 +101,17 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
     return c;  // NaN cos(theta): dropped
   }
   if (xu != xu) return c;  // NaN after rotation: dropped
-  if (p.want_u) {
+  if (p.spread_n) {
     // the azimuth from the branch's (cos phi, sin phi), so that it is exact
-    // whatever the caller's phi range: u = atan2(sin, cos) / 2pi mod 1
+    // whatever the caller's phi range: u = atan2(sin, cos) / 2pi mod 1, and
+    // the support of the W-tap kernel around n u
     double u = atan2(-c.eim, c.ere) * (0.5 / M_PI);
     if (u < 0.0) u += 1.0;
-    c.u = u < 1.0 ? u : 0.0;
+    if (!(u < 1.0)) u = 0.0;
+    const double t = u * (double)p.spread_n;
+    const double jf = ceil(t - 0.5 * kSpreadW);
+    c.j0 = (int)jf;
+    c.z0 = (float)(jf - t);
   }
   const int h = xu < 0.0 ? 1 : 0;
   const double ax = fabs(xu);
@@ -124,6 +131,25 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
   c.h3 = w * ((t * t) * (t - 1.0));
   c.seg = (branch * 2 + h) * (p.Nx - 1) + j;
   return c;
+}
+
+// The sorted record of a classified point at position pos: a SpreadRec for
+// the spreading scatter, else a PointRec.
+__device__ inline void place_point(const Classified& c, int sg, int64_t i, int pos,
+                                   const PointParams& p, PointRec* __restrict__ rec) {
+  if (p.spread_n) {  // a 32-byte SpreadRec, in two 16-byte stores
+    float4* d = reinterpret_cast<float4*>(reinterpret_cast<SpreadRec*>(rec) + pos);
+    d[0] = make_float4((float)c.h0, (float)c.h1, (float)c.h2, (float)c.h3);
+    reinterpret_cast<int4*>(d)[1] = make_int4(sg, (int)i, c.j0, __float_as_int(c.z0));
+    return;
+  }
+  // the whole 64-byte line, in 16-byte stores (a partly written line costs a
+  // read-modify-write)
+  double2* d = reinterpret_cast<double2*>(rec + pos);
+  d[0] = make_double2(c.ere, c.eim);
+  d[1] = make_double2(c.h0, c.h1);
+  d[2] = make_double2(c.h2, c.h3);
+  reinterpret_cast<int4*>(d + 3)[0] = make_int4(sg, (int)i, 0, 0);
 }
 
 // Fallback histogram for grids with more segments than k_rank_hist's LDS row
@@ -244,12 +270,7 @@ __global__ __launch_bounds__(256) void k_place(int64_t Np, const double* __restr
     Classified c = classify(theta[i], phi[i], wi, p);
     vmax = fmaxf(vmax, abs_w(wi, c.seg, p.nseg));
     const int pos = atomicAdd(&cursor[c.seg], 1);
-    double2* d = reinterpret_cast<double2*>(rec + pos);
-    d[0] = make_double2(c.ere, c.eim);
-    d[1] = make_double2(c.h0, c.h1);
-    d[2] = make_double2(c.h2, c.h3);
-    const double uu = p.want_u ? c.u : 0.0;
-    reinterpret_cast<int4*>(d + 3)[0] = make_int4(c.seg, (int)i, __double2loint(uu), __double2hiint(uu));
+    place_point(c, c.seg, i, pos, p, rec);
   }
   block_wmax(vmax, wmax);
 }
@@ -461,17 +482,6 @@ __device__ inline bool xcd_tile_block(int ntiles, int nbx, int& t, int& bx) {
 }
 inline unsigned xcd_tile_grid(int ntiles, int nbx) { return (unsigned)((ntiles + 7) / 8 * 8 * nbx); }
 
-__device__ inline void place_point(const Classified& c, int sg, int64_t i, int pos,
-                                   const PointParams& p, PointRec* __restrict__ rec) {
-  // the whole 64-byte line, in 16-byte stores (a partly written line costs a
-  // read-modify-write)
-  double2* d = reinterpret_cast<double2*>(rec + pos);
-  d[0] = make_double2(c.ere, c.eim);
-  d[1] = make_double2(c.h0, c.h1);
-  d[2] = make_double2(c.h2, c.h3);
-  const double uu = p.want_u ? c.u : 0.0;
-  reinterpret_cast<int4*>(d + 3)[0] = make_int4(sg, (int)i, __double2loint(uu), __double2hiint(uu));
-}
 
 __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile, int ntiles, int nbx,
                                                       const double* __restrict__ theta,
@@ -759,13 +769,7 @@ __global__ __launch_bounds__(256) void k_place_many(ManySets ms, int ntiles, int
       const int sg = seg_of_pt[o + i], rk = rank_of_pt[o + i];
       const Classified c = classify(th[i], ph[i], wi, p);
       vmax = fmaxf(vmax, abs_w(wi, sg, p.nseg));
-      const int pos = toff[(int64_t)t * nb + sg] + rk;
-      double2* d = reinterpret_cast<double2*>(rec + pos);
-      d[0] = make_double2(c.ere, c.eim);
-      d[1] = make_double2(c.h0, c.h1);
-      d[2] = make_double2(c.h2, c.h3);
-      const double uu = p.want_u ? c.u : 0.0;
-      reinterpret_cast<int4*>(d + 3)[0] = make_int4(sg, i, __double2loint(uu), __double2hiint(uu));
+      place_point(c, sg, i, toff[(int64_t)t * nb + sg] + rk, p, rec);
     }
   }
   block_wmax(vmax, wmax + s);
@@ -797,8 +801,8 @@ __global__ __launch_bounds__(256) void k_count_runs_many(ManySets ms,
 int launch_points_many(Plan& pl, const ManySets& ms, int mode, int* flags, unsigned* wmax,
                        hipStream_t s) {
   Workspace& ws = pl.ws;
-  PointParams p{pl.Nx, pl.nseg, mode, uses_spread(pl) ? 1 : 0, pl.xmax, pl.dx, pl.acos_xmax,
-                pl.acos_mxmax};
+  PointParams p{pl.Nx, pl.nseg, mode, uses_spread(pl) ? 1 << pl.spread_logn : 0, pl.xmax, pl.dx,
+                pl.acos_xmax, pl.acos_mxmax};
   const int nb = pl.nseg + 1, stride = pl.nseg + 2;
   const int ntiles = ms.tile_off[ms.nsets], nchunks = ms.chunk_off[ms.nsets];
   const int npts = ms.off[ms.nsets];
@@ -852,8 +856,8 @@ int launch_points_many(Plan& pl, const ManySets& ms, int mode, int* flags, unsig
 int launch_points(Plan& pl, int64_t Np, const double* theta, const double* phi, const double* wt,
                   int mode, int* flags, hipStream_t s) {
   Workspace& ws = pl.ws;
-  PointParams p{pl.Nx, pl.nseg, mode, uses_spread(pl) ? 1 : 0, pl.xmax, pl.dx, pl.acos_xmax,
-                pl.acos_mxmax};
+  PointParams p{pl.Nx, pl.nseg, mode, uses_spread(pl) ? 1 << pl.spread_logn : 0, pl.xmax, pl.dx,
+                pl.acos_xmax, pl.acos_mxmax};
   // counters are cleared by a kernel, not a memset node, so the transform can
   // be captured in a HIP graph and replayed
   k_clear2<<<(pl.nseg + 2 + 255) / 256, 256, 0, s>>>(ws.seg_count, pl.nseg + 2, flags, 4, ws.wmax);

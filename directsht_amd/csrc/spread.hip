@@ -190,7 +190,7 @@ __device__ __forceinline__ void fft_passes8(float2* sz, const float2 (&twr)[3][7
 // wmode 0: the records' own weights, 1: w[sorted position], 2: w[rec.idx]
 template <int LOGN>
 __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
-    const PointRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg, int chunk,
+    const SpreadRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg, int chunk,
     int Nl, const int* __restrict__ run_base, const double* __restrict__ w, int wmode,
     float beta, const float2* __restrict__ tw, const float* __restrict__ ipsi,
     float* __restrict__ part, int* __restrict__ run_seg, const unsigned* __restrict__ wmax) {
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 
   int run = run_base[c];
   for (int p = b; p < e;) {  // uniform: one run per iteration
-    const int cur = rec[p].seg;
+    const int cur = rec[p].seg;  // uniform
     const int pe = min(e, seg_start[cur + 1]);
 #pragma unroll 1
     for (int pass = 0; pass < 4 / G; ++pass) {
@@ -238,15 +238,15 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
         reinterpret_cast<ulonglong2*>(sgrid)[i] = make_ulonglong2(0ull, 0ull);
       __syncthreads();
       for (int q = p + tid; q < pe; q += kT) {
-        const PointRec& r = rec[q];
-        const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[r.idx])) * w_fix;
+        const float4 ah = reinterpret_cast<const float4*>(rec + q)[0];
+        const int4 ri = reinterpret_cast<const int4*>(rec + q)[1];  // seg, idx, j0, z0
+        const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri.y])) * w_fix;
+        const float hk[4] = {ah.x, ah.y, ah.z, ah.w};
         double a[G];
 #pragma unroll
-        for (int g = 0; g < G; ++g) a[g] = wt * (&r.h0)[pass * G + g];
-        const double t = r.u * (double)n;
-        const double jf = ceil(t - 0.5 * kW);
-        const int j0 = (int)jf;
-        const float z0 = (float)(jf - t);  // in [-W/2, -W/2 + 1)
+        for (int g = 0; g < G; ++g) a[g] = wt * (double)hk[pass * G + g];
+        const int j0 = ri.z;
+        const float z0 = __int_as_float(ri.w);  // in [-W/2, -W/2 + 1)
 #pragma unroll
         for (int i = 0; i < kW; ++i) {
           const float z = (z0 + (float)i) * (2.0f / kW);
@@ -312,7 +312,7 @@ static int launch_spread_n(Plan& pl, int nchunks, const double* w, int wmode, co
                            hipStream_t s) {
   using Gm = SpreadGeom<LOGN>;
   k_scatter_spread<LOGN><<<nchunks, Gm::kThreads, 0, s>>>(
-      pl.ws.rec, pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl, pl.ws.run_base, w, wmode,
+      reinterpret_cast<const SpreadRec*>(pl.ws.rec), pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl, pl.ws.run_base, w, wmode,
       (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.part, pl.ws.run_seg, wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
