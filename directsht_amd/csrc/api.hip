@@ -385,6 +385,35 @@ static void rotation_constants(int Nl, std::vector<double>& top, std::vector<int
   }
 }
 
+// Coefficients of the normalized Legendre recurrence the FP32 contraction
+// runs (k_contract_otf), per order m at otf_offset(m, Nl), pairs {A, B}:
+//   k = 0:  A = (-1)^m K_m N_m,  Y_mm(x) = A (1 - x^2)^(m/2),
+//           K_m = prod_{j<=m} sqrt(1 - 1/(2j))          (sht/legendre_py.py:28-30)
+//   k = 1:  A = sqrt(2m+1) N_{m+1} / N_m,  Y_{m+1,m}(x) = A x Y_mm(x)   (:32-34)
+//   k >= 2, l = m + k < Nl:  Y_l = A x Y_{l-1} - B Y_{l-2} with
+//           A = f1 (2l-1) / (l-m) N_l / N_{l-1},
+//           B = f1 f2 (l+m-1) / (l-m) N_l / N_{l-2}                 (:91-100)
+//   f1 = sqrt((l-m)/(l+m)), f2 = sqrt((l-m-1)/(l+m-1)), N_l = sqrt((2l+1)/4pi)
+//   (sht/legendre_py.py:77-81); zero for Nl <= l < Nl + 30 (rows past the band
+//   limit, which the kernel generates in its last step and discards).
+static void otf_coefficients(int Nl, std::vector<double>& rc) {
+  rc.assign(2 * (size_t)otf_offset(Nl, Nl), 0.0);
+  auto N = [](int l) { return std::sqrt((double)(2 * l + 1) / 4.0 / M_PI); };
+  double K = 1.0;
+  for (int m = 0; m < Nl; ++m) {
+    if (m > 0) K *= std::sqrt(1.0 - 1.0 / (double)(2 * m));
+    double* r = rc.data() + 2 * otf_offset(m, Nl);
+    r[0] = (m & 1 ? -K : K) * N(m);
+    r[2] = std::sqrt(2.0 * m + 1.0) * N(m + 1) / N(m);
+    for (int l = m + 2; l < Nl; ++l) {
+      const double f1 = std::sqrt((double)(l - m) / (double)(l + m));
+      const double f2 = std::sqrt(((double)(l - m) - 1.0) / ((double)(l + m) - 1.0));
+      r[2 * (l - m)] = f1 * (double)(2 * l - 1) / (double)(l - m) * N(l) / N(l - 1);
+      r[2 * (l - m) + 1] = f1 * f2 * (double)(l + m - 1) / (double)(l - m) * N(l) / N(l - 2);
+    }
+  }
+}
+
 // Spreading-scatter constants: n = 2^logn >= 4 Nl (at least 256), beta =
 // 2.3 W, the twiddles e^{-2 pi i k / n} and 1/psihat(m) for m < Nl, with
 // psihat(m) = (W/2) int_{-1}^{1} exp(beta (sqrt(1 - z^2) - 1)) cos(pi m W z / n) dz
@@ -478,8 +507,14 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   if (caller) DSHT_ALLOC(pl.tab64, pl.tab_vec4 * 4 * sizeof(double));
   if (alloc_tables && !caller && precision == DSHT_FP64)
     DSHT_ALLOC(pl.tab64y, pl.tab_vec4 * 2 * sizeof(double));
-  if (alloc_tables && !caller && precision == DSHT_FP32)
-    DSHT_ALLOC(pl.tab32y, pl.tab_vec4 * 2 * sizeof(float));
+  if (alloc_tables && !caller && precision == DSHT_FP32) {
+    // no Y table: the contraction regenerates Y from the recurrence's coefficients
+    std::vector<double> rcoef;
+    otf_coefficients(Nl, rcoef);
+    pl.otf = true;
+    DSHT_ALLOC(pl.otf_rc, rcoef.size() * sizeof(double));
+    DSHT_HIP(hipMemcpy(pl.otf_rc, rcoef.data(), rcoef.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   std::vector<double> top, A, iB;
   std::vector<int> ex;
   rotation_constants(Nl, top, ex, A, iB);
@@ -503,12 +538,11 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   pl.table_bytes = ws.bytes;
   ws.bytes = 0;
   pl.table_hold = std::make_shared<TableOwner>(
-      pl.device, std::vector<void*>{pl.groups, pl.mp_off, pl.tab64y, pl.tab32y, pl.tab64,
+      pl.device, std::vector<void*>{pl.groups, pl.mp_off, pl.tab64y, pl.tab32y, pl.otf_rc, pl.tab64,
                                     pl.rot_top, pl.rot_exp, pl.rot_AB, pl.rot_iB, pl.spread_tw,
                                     pl.spread_ipsi});
   if (alloc_tables && build && !caller) {
-    int rc = precision == DSHT_FP64 ? launch_tables(pl, 0, pl.tab64y, nullptr, 0)
-                                    : launch_tables(pl, 1, pl.tab32y, nullptr, 0);
+    int rc = precision == DSHT_FP64 ? launch_tables(pl, 0, pl.tab64y, nullptr, 0) : DSHT_OK;
     if (rc) return rc;
     DSHT_HIP(hipDeviceSynchronize());
   }
@@ -551,7 +585,7 @@ static void plan_free(Plan& pl) {
   if (pl.table_hold) {
     pl.table_hold.reset();  // frees the tables with the last plan sharing them
   } else {                  // plan_init failed before handing them to an owner
-    for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64y, (void*)pl.tab32y,
+    for (void* p : {(void*)pl.groups, (void*)pl.mp_off, (void*)pl.tab64y, (void*)pl.tab32y, (void*)pl.otf_rc,
                     (void*)pl.tab64, (void*)pl.rot_top, (void*)pl.rot_exp, (void*)pl.rot_AB,
                     (void*)pl.rot_iB, (void*)pl.spread_tw, (void*)pl.spread_ipsi})
       if (p) (void)hipFree(p);
@@ -973,7 +1007,8 @@ int dsht_plan_create_from_tables(int Nl, int Nx, double xmax, int precision, con
       rc = DSHT_ENOMEM;
     }
   }
-  for (int64_t r0 = 0; rc == DSHT_OK && r0 < pl.Nlm; r0 += per) {
+  // an FP32 plan holds no Y table (its contraction regenerates Y): nothing to load
+  for (int64_t r0 = 0; rc == DSHT_OK && !pl.otf && r0 < pl.Nlm; r0 += per) {
     const int64_t n = std::min<int64_t>(per, pl.Nlm - r0);
     hipError_t e = hipMemcpy(slab, Yv + r0 * Nx, (size_t)n * Nx * sizeof(double),
                              hipMemcpyHostToDevice);
@@ -1037,6 +1072,8 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.tab_vec4 = s.tab_vec4;
   d.tab64y = s.tab64y;
   d.tab32y = s.tab32y;
+  d.otf = s.otf;
+  d.otf_rc = s.otf_rc;
   d.tab64 = s.tab64;
   d.caller_tables = s.caller_tables;
   d.table_bytes = s.table_bytes;
@@ -1116,9 +1153,11 @@ int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table
   build_layout(Nl, Npair, groups, mp_off, vec4);
   groups = xcd_schedule(groups, kXcds);
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
+  // FP32 plans keep only the recurrence's coefficients (no Y table)
+  const int64_t tab = precision == DSHT_FP32 ? 16 * otf_offset(Nl, Nl) : vec4 * 2 * 8;
   if (table_bytes)
-    *table_bytes = vec4 * 2 * (precision == DSHT_FP32 ? 4 : 8) + Nlm * (3 * 8 + 4) +
-                   (int64_t)groups.size() * sizeof(GroupDesc) + (int64_t)mp_off.size() * 8;
+    *table_bytes = tab + Nlm * (3 * 8 + 4) + (int64_t)groups.size() * sizeof(GroupDesc) +
+                   (int64_t)mp_off.size() * 8;
   if (work_bytes) *work_bytes = workspace_bytes(Nl, Nx, precision, Np, 2048);
   return DSHT_OK;
 }

@@ -163,6 +163,161 @@ __global__ __launch_bounds__(64) void k_contract_y(const typename Couple<T>::typ
   }
 }
 
+// FP32 plans: the same sums without a table.  Y_lm(x_n) is regenerated by
+// the normalized Legendre recurrence in l (FP64, the coefficients of
+// otf_coefficients in api.hip, three FP64 operations per (l, m, node)), and
+// the products with the node coefficients run on the FP32 matrix cores:
+// per row tile (16 rows of one parity) and node quad,
+//   D[row][col] += sum_{k<4} Y(row, node 4q + k) B[k][col],
+// B = the 12 coefficients c, f, g x (er, ei, pr, pi) of the row's parity
+// (columns 12..15 unused), v_mfma_f32_16x16x4f32.  No table is read: the
+// kernel streams nothing but the coefficients (once per workgroup).
+//
+// Workgroup = (order m, range of 512 nodes), 4 wavefronts of 128 nodes,
+// each lane two chains (nodes 64 c + lane of its wavefront).  A step
+// generates 32 rows l = m + 32 s + t (t even: parity 0, odd: parity 1) into
+// the wavefront's LDS tiles, rounded to FP32, in the A-operand order (lane
+// k * 16 + i reads row i, nodes 4 q + k for four consecutive quads q as one
+// 16-byte read; rows padded to 132 floats and quads XOR-swizzled by k, so
+// neither the writes nor the reads conflict in LDS banks), and multiplies
+// them out: per wavefront FP32 sums over its 128 nodes, summed over the 4
+// wavefronts in FP64 and written as the partial sums of k_contract_y (the
+// node ranges are dealt to the gridDim.y workgroup columns, and a column
+// with several ranges adds the later ones to its sums: the same threads, in
+// order).  76 KB of LDS, so two workgroups share a CU; not synchronized with
+// each other, one's recurrence (vector unit) can run beside the other's
+// products (matrix cores).  (Double-buffered tiles with the next step's
+// recurrence interleaved between the MFMAs in each wavefront, at one
+// workgroup per CU, measured 1.5x slower.)
+constexpr int kOtfWaves = 4, kOtfChains = 2;
+constexpr int kOtfNodes = kOtfWaves * 64 * kOtfChains;  // nodes per workgroup
+constexpr int kOtfRow = 132;                            // floats per staged row
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kOtfWaves * 64) void k_contract_otf(
+    const double2* __restrict__ rc, const float* __restrict__ rhs, int Nl, int Nx, int Npair,
+    double dx, int64_t Nlm, double* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) float stage[kOtfWaves][2][16 * kOtfRow];
+  __shared__ float dsum[kOtfWaves][2][256];
+  const int m = blockIdx.x, split = blockIdx.y;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const double2* __restrict__ rcm = rc + otf_offset(m, Nl);
+  const int nranges = (2 * Npair + kOtfNodes - 1) / kOtfNodes;
+  const int kk = lane >> 4, col = lane & 15;
+  const int cofs = col < 12 ? 8 * (col >> 2) + 2 * (col & 3) : 0;
+  // where this lane writes node 64 c + lane (k, quad) and reads (row i, k)
+  const int wk = lane & 3, wq = lane >> 2;
+  const int ri = lane & 15, rk = lane >> 4;
+  const int nsteps = (Nl - m + 31) / 32;
+  for (int rg = split; rg < nranges; rg += gridDim.y) {
+    // chains: Y_{l-2}, Y_{l-1} (ya, yb) at x_n, from Y_mm and Y_{m+1,m}
+    double x[kOtfChains], ya[kOtfChains], yb[kOtfChains];
+#pragma unroll
+    for (int c = 0; c < kOtfChains; ++c) {
+      const int n = rg * kOtfNodes + w * 128 + c * 64 + lane;
+      const bool ok = n < Nx;  // padding nodes (odd Nx, past Nx) hold Y = 0
+      x[c] = ok ? (double)n * dx : 0.0;
+      const double ymm = ok ? rcm[0].x * pow(sqrt(1.0 - x[c] * x[c]), (double)m) : 0.0;
+      ya[c] = ymm;
+      yb[c] = rcm[1].x * x[c] * ymm;
+    }
+    // B operands: lane (k = lane / 16, col = lane % 16) holds column col of
+    // node 4 q + k of this wavefront, per parity and quad (loaded once per range)
+    float bop[2][32];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        const int n = rg * kOtfNodes + w * 128 + 4 * q + kk;
+        const int np = min(n >> 1, Npair - 1);  // past the nodes: Y = 0 there
+        bop[p][q] = rhs[((int64_t)(2 * m + p) * Npair + np) * 24 + cofs + (n & 1)];
+      }
+    float* tile = &stage[w][0][0];
+    // rows t0 .. t0+3 of step s into the wavefront's tiles
+    auto gen4 = [&](int s, int t0) {
+#pragma unroll
+      for (int dt = 0; dt < 4; ++dt) {
+        const int t = t0 + dt;
+        const bool start = s == 0 && t < 2;  // rows m, m+1: the start values
+        const double2 ab = start ? make_double2(0.0, 0.0) : rcm[32 * s + t];
+        float* row = tile + (t & 1) * 16 * kOtfRow + (t >> 1) * kOtfRow + wk * 32;
+#pragma unroll
+        for (int c = 0; c < kOtfChains; ++c) {
+          double y;
+          if (start) {
+            y = t == 0 ? ya[c] : yb[c];
+          } else {
+            y = fma(ab.x * x[c], yb[c], -(ab.y * ya[c]));
+            ya[c] = yb[c];
+            yb[c] = y;
+          }
+          row[(16 * c + wq) ^ (8 * wk)] = (float)y;
+        }
+      }
+    };
+    for (int s = 0; s < nsteps; ++s) {
+#pragma unroll
+      for (int t0 = 0; t0 < 32; t0 += 4) gen4(s, t0);
+      __syncthreads();  // the tiles are written (and the last step's sums read)
+      // eight independent accumulators (parity x quad mod 4), so consecutive
+      // MFMAs never wait for each other
+      f32x4_t acc[2][4];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[p][u] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q0 = 0; q0 < 32; q0 += 4) {
+        f32x4_t a4[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+          a4[p] = *reinterpret_cast<const f32x4_t*>(tile + p * 16 * kOtfRow + ri * kOtfRow +
+                                                    rk * 32 + (q0 ^ (8 * rk)));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int p = 0; p < 2; ++p)
+            acc[p][u] =
+                __builtin_amdgcn_mfma_f32_16x16x4f32(a4[p][u], bop[p][q0 + u], acc[p][u], 0, 0, 0);
+      }
+      // D of lane (k, col), register r: row 4 k + r, column col, kept at
+      // [r][k][col] (the 64 lanes of a store on 64 banks)
+      float* ds = &dsum[w][0][0];
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          ds[p * 256 + r * 64 + rk * 16 + ri] =
+              (acc[p][0][r] + acc[p][1][r]) + (acc[p][2][r] + acc[p][3][r]);
+      __syncthreads();  // the wavefronts' sums are in LDS
+      if (tid < 128) {  // (parity, row, part o): S_c - l S_f and S_g over the 4 wavefronts
+        const int p = tid >> 6, rr = (tid >> 2) & 15, o = tid & 3;
+        const int ell = m + 32 * s + 2 * rr + p;
+        if (ell < Nl) {
+          double sc = 0.0, sf = 0.0, sg = 0.0;
+          const int d0 = p * 256 + (rr & 3) * 64 + (rr >> 2) * 16;  // row rr, [r][k][col]
+#pragma unroll
+          for (int v = 0; v < kOtfWaves; ++v) {
+            const float* dv = &dsum[v][0][0];
+            sc += (double)dv[d0 + o];
+            sf += (double)dv[d0 + 4 + o];
+            sg += (double)dv[d0 + 8 + o];
+          }
+          double* out = part + ((int64_t)split * Nlm + hp_index(ell, m, Nl)) * 8;
+          if (rg == split) {
+            out[o] = sc - ell * sf;
+            out[4 + o] = sg;
+          } else {
+            out[o] += sc - ell * sf;
+            out[4 + o] += sg;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next range's tiles and sums overwrite this range's
+  }
+}
+
 // a_lm = sum over ranges of (S_c - l S_f) + G_l sum over ranges of S_g(l-1, m)
 // (l > m), G_l = sqrt((l^2 - m^2)(2l+1)/(2l-1)): the partial sums of
 // k_contract_y combined with the rest of the Y' term (thread per (l, m))
@@ -250,9 +405,14 @@ int launch_contract_set(Plan& pl, int nsets, int set, hipStream_t s) {
     return DSHT_OK;
   }
   const int64_t set_off = (int64_t)set * pl.Nlm;
-  const int nsplit = contract_splits(pl);
+  int nsplit = contract_splits(pl);
   dim3 cgrid(pl.ngroups, nsplit);
-  if (pl.precision == DSHT_FP32) {
+  if (pl.otf) {
+    nsplit = std::min(kContractSplits, (2 * pl.Npair + kOtfNodes - 1) / kOtfNodes);
+    k_contract_otf<<<dim3(pl.Nl, nsplit), kOtfWaves * 64, 0, s>>>(
+        reinterpret_cast<const double2*>(pl.otf_rc), static_cast<const float*>(ws.rhs), pl.Nl,
+        pl.Nx, pl.Npair, pl.dx, pl.Nlm, ws.a_der);
+  } else if (pl.precision == DSHT_FP32) {
     k_contract_y<float><<<cgrid, 64, 0, s>>>(reinterpret_cast<const float4*>(pl.tab32y),
                                              static_cast<const float*>(ws.rhs), pl.groups, pl.Npair,
                                              pl.Nl, pl.Nlm, ws.a_der);

@@ -154,6 +154,11 @@ struct Plan {
   // do_transform), contracted with the caller's Y' as given.
   double* tab64y = nullptr;
   float* tab32y = nullptr;
+  // FP32 plans without caller tables hold no Y table: the contraction
+  // regenerates Y by the Legendre recurrence (k_contract_otf), from otf_rc:
+  // per m at otf_offset(m, Nl), (Nl - m + 32) FP64 pairs {A, B}
+  bool otf = false;
+  double* otf_rc = nullptr;
   double* tab64 = nullptr;
   bool caller_tables = false;
   int64_t table_bytes = 0;
@@ -185,6 +190,11 @@ struct Plan {
   int64_t timed = 0;
 };
 
+// Offset of order m's recurrence coefficients in Plan::otf_rc (pairs), and
+// the table's length: entries k = 0 .. Nl - m + 31 for each m.
+__host__ __device__ inline int64_t otf_offset(int m, int Nl) {
+  return (int64_t)m * (Nl + 32) - (int64_t)m * (m - 1) / 2;
+}
 // Host-side layout computation shared by plan creation and estimates.
 void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups,
                   std::vector<int64_t>& mp_off, int64_t& tab_vec4);

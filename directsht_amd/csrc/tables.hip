@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_relayout_y(int Nl, int Nx, int Npair,
 
 int launch_relayout_y(const Plan& pl, int64_t row0, int64_t nrows, const double* slab,
                       hipStream_t s) {
-  if (nrows <= 0) return DSHT_OK;
+  if (nrows <= 0 || pl.otf) return DSHT_OK;  // FP32 plans hold no Y table
   if (pl.precision == DSHT_FP64)
     k_relayout_y<double><<<(unsigned)nrows, 256, 0, s>>>(pl.Nl, pl.Nx, pl.Npair, pl.mp_off, row0,
                                                          slab, pl.tab64y);

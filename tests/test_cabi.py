@@ -50,6 +50,12 @@ def test_estimate_without_gpu():
     lib = _lib.load()
     tb, wb = ct.c_int64(), ct.c_int64()
     assert lib.dsht_plan_estimate(1000, 2048, _lib.DSHT_FP32, 10_000_000, ct.byref(tb), ct.byref(wb)) == 0
-    # FP32 Y-only table: Nlm * Nx * 4 bytes (+ small metadata)
-    assert abs(tb.value - 500500 * 2048 * 4) / (500500 * 2048 * 4) < 0.01
+    # FP32 plans hold no Y table, only the recurrence's coefficients: 16 bytes
+    # per (m, k < Nl - m + 32), plus the rotation constants (28 B per (l, m))
+    # and the group metadata
+    coef = 16 * (1000 * 1032 - 1000 * 999 // 2)
+    assert coef + 500500 * 28 <= tb.value < coef + 500500 * 28 + 2_000_000
     assert wb.value > 10_000_000 * 52
+    # FP64 plans: the Y-only table, Nlm * Nx * 8 bytes (+ small metadata)
+    assert lib.dsht_plan_estimate(1000, 2048, _lib.DSHT_FP64, 10_000_000, ct.byref(tb), ct.byref(wb)) == 0
+    assert abs(tb.value - 500500 * 2048 * 8) / (500500 * 2048 * 8) < 0.01
