@@ -33,7 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "points·modes/s and alm/s at Nl=1000,1e7 pts; batched vs plain; p50/p99 latency"
-PEAK_FP32_TFLOPS = 157.3   # MI355X published dense FP32
+PEAK_FP32_TFLOPS = 157.3   # MI355X published dense FP32 (vector and matrix)
 PEAK_FP16_TFLOPS = 2500.0  # MI355X published dense FP16 matrix
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 # HBM bytes per launch of each kernel at the default workload, measured by
@@ -645,13 +645,19 @@ def main():
     stage_avg = {k: v / max(ntimed, 1) for k, v in stage_ms.items()}
     log(f"{ms:.3f} ms/step; stages {stage_avg}")
 
-    # Dominant kernel: the table contraction (k_contract_y<float> +
-    # k_add_der), HBM-bound: algorithmic bytes = the FP32 Y-only table, one
-    # float per (l, m, node) over the 2 ceil(Nx/4) node pairs of the layout,
-    # read once per transform.
+    # Dominant kernel: the contraction (k_contract_otf + k_add_der).  FP32
+    # plans hold no Y table: the kernel regenerates Y_lm at the nodes by the
+    # FP64 Legendre recurrence (4 flop per (l, m, node)) and multiplies it
+    # with the node coefficients on the FP32 matrix cores.  Algorithmic work
+    # = the contraction's flop count, 8 per (l, m, node, branch) -- the
+    # complex coefficients of Y and Y' -- over the 2 branches (equatorial,
+    # polar): 16 Nlm Nx per transform, priced against the FP32 matrix peak.
+    # Issued on the matrix cores: 16 columns (12 used: c, f, g x 4) x 2 flop
+    # per (l, m, node) over the 2 * 2 ceil(Nx/4) padded nodes.
     npair = 2 * ((Nx + 3) // 4)
-    tab_bytes = 2.0 * Nlm * npair * 4
-    con_gbs = tab_bytes / (stage_avg["contract"] * 1e-3) / 1e9
+    con_s = stage_avg["contract"] * 1e-3
+    con_tf = 16.0 * Nlm * Nx / con_s / 1e12
+    con_mfma_tf = 32.0 * Nlm * 2 * npair / con_s / 1e12
     # The scatter (second largest stage).  The reference's algorithm does
     # 22 flop per (point, order m) -- four complex multiply-adds
     # w h_k(t) e^{-i m phi} into the accumulators (16) and the complex step
@@ -686,12 +692,13 @@ def main():
                                      "fp16 hi/lo split"}
     else:
         runs_bytes = tr("k_scatter_spread")
-        rec_bytes = 64.0 * N
+        rec_bytes = 32.0 * N
         roof_scatter = {
             "kernel": "k_scatter_spread", "engine": engine, "bound": "lds",
             "direct_sum_equivalent_tflops": alg_tf,
             "direct_sum_equivalent": "22 N Nl / t: the reference algorithm's flop count, "
                                      "not flop executed (non-uniform FFT per run)",
+            "records": "32-byte sorted records in (N x 32 B)",
             "hbm_GBs_records_in": rec_bytes / scat_s / 1e9,
             "hbm_frac_records_in": rec_bytes / scat_s / 1e9 / PEAK_HBM_GBS,
             "traffic": runs_bytes}
@@ -715,11 +722,14 @@ def main():
                    "Nl": Nl, "Nx": Nx, "npoints": N, "parallelism": "single GPU"},
         "alm_per_s": Nlm / (ms * 1e-3),
         "stage_ms": stage_avg,
-        "roofline": {"kernel": "k_contract_y<float> + k_add_der", "bound": "hbm",
-                     "achieved": con_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": con_gbs / PEAK_HBM_GBS, "traffic": tr("k_contract_y"),
-                     "algorithmic": "FP32 Y-only table Nlm*2*Npair*4 B (Npair = 2 ceil(Nx/4)), "
-                                    "read once per transform"},
+        "roofline": {"kernel": "k_contract_otf + k_add_der", "bound": "mfma",
+                     "engine": "FP32 matrix cores (v_mfma_f32_16x16x4f32), Y by FP64 recurrence",
+                     "achieved": con_tf, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": con_tf / PEAK_FP32_TFLOPS, "traffic": tr("k_contract_otf"),
+                     "algorithmic": "8 flop per (l, m, node, branch), 2 branches: 16 Nlm Nx per "
+                                    "transform; no table is read",
+                     "mfma_instruction_tflops": con_mfma_tf,
+                     "mfma_instruction_frac": con_mfma_tf / PEAK_FP32_TFLOPS},
         "roofline_scatter": roof_scatter,
     }
 

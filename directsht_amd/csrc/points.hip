@@ -48,6 +48,9 @@ struct Classified {
   float z0;
 };
 
+// kPhase: also the point's azimuth (e^{-i phi} for PointRec, the spreading
+// support for SpreadRec); the segment passes need only the segment.
+template <bool kPhase = true>
 __device__ inline Classified classify(double theta, double phi, double w, const PointParams& p) {
 #pragma clang fp contract(off)
   Classified c;
@@ -92,22 +95,25 @@ __device__ inline Classified classify(double theta, double phi, double w, const 
     c.polar = 1;
   } else if (p.mode == DSHT_MODE_BASIC || xv < p.xmax) {
     if (!(theta >= 0.0 && theta > p.acos_xmax && theta < p.acos_mxmax)) c.flag |= DSHT_STATUS_EQU_RANGE;
-    sincos(phi, &sp, &cp);
+    if (kPhase && !p.spread_n) {
+      sincos(phi, &sp, &cp);
+      c.ere = cp;
+      c.eim = -sp;
+    }
     xu = ct;
-    c.ere = cp;
-    c.eim = -sp;
     branch = 0;
   } else {
     return c;  // NaN cos(theta): dropped
   }
   if (xu != xu) return c;  // NaN after rotation: dropped
-  if (p.spread_n) {
-    // the azimuth from the branch's (cos phi, sin phi), so that it is exact
-    // whatever the caller's phi range: u = atan2(sin, cos) / 2pi mod 1, and
-    // the support of the W-tap kernel around n u
-    double u = atan2(-c.eim, c.ere) * (0.5 / M_PI);
-    if (u < 0.0) u += 1.0;
-    if (!(u < 1.0)) u = 0.0;
+  if (kPhase && p.spread_n) {
+    // u = phi / 2pi mod 1 and the support of the W-tap kernel around n u:
+    // for equatorial points from phi itself (FP64, exact to rounding for
+    // any moderate phi), for rotated points from the rotated (cos, sin) by
+    // atan2
+    double u = branch ? atan2(-c.eim, c.ere) * (0.5 / M_PI) : phi * (0.5 / M_PI);
+    u -= floor(u);
+    if (!(u >= 0.0 && u < 1.0)) u = 0.0;
     const double t = u * (double)p.spread_n;
     const double jf = ceil(t - 0.5 * kSpreadW);
     c.j0 = (int)jf;
@@ -163,7 +169,7 @@ __global__ __launch_bounds__(256) void k_hist(int64_t Np, const double* __restri
   int flag = 0, npol = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
-    Classified c = classify(theta[i], phi[i], 0.0, p);  // weights do not move points
+    Classified c = classify<false>(theta[i], phi[i], 0.0, p);  // weights do not move points
     flag |= c.flag;
     npol += c.polar;
     atomicAdd(&seg_count[c.seg], 1);
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(256) void k_segments(int64_t Np, const double* __re
   int flag = 0, npol = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < Np;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const Classified c = classify(theta[i], phi[i], 0.0, p);  // weights do not move points
+    const Classified c = classify<false>(theta[i], phi[i], 0.0, p);  // weights do not move points
     seg_of_pt[i] = c.seg;
     flag |= c.flag;
     npol += c.polar;
@@ -573,7 +579,7 @@ __global__ __launch_bounds__(256) void k_segments_many(ManySets ms, PointParams 
   const double* __restrict__ ph = ms.ph[s];
   int flag = 0, npol = 0;
   for (int i = b + threadIdx.x; i < e; i += 256) {
-    const Classified c = classify(th[i], ph[i], 0.0, p);
+    const Classified c = classify<false>(th[i], ph[i], 0.0, p);
     seg_of_pt[ms.off[s] + i] = c.seg;
     flag |= c.flag;
     npol += c.polar;

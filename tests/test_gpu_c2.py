@@ -84,14 +84,28 @@ def test_c2_full_size_properties(torch_cuda):
     del s64, r64
     torch.cuda.empty_cache()
     s32 = DirectSHT(Nl, Nx, dflt_type="float32")
+    # the default engine at this size: the spreading scatter (fixed-point
+    # grids, so the same bits on every call), then the recurrence contraction
+    assert s32.scatter_engine(N) == "spread"
     a32, _ = s32.transform_device(th, ph, w)
     a32 = a32.clone()
+    again, _ = s32.transform_device(th, ph, w)
     r32, _ = s32.transform_device(math.pi - th, ph, w)
     r32 = r32.clone()
     torch.cuda.synchronize()
+    assert torch.equal(again, a32), "two calls differ"
     e, r = rel_max(a32, a64), rel_rms(a32, a64)
     print(f"C2 FP32 vs FP64: max {e:.2e} rms {r:.2e}")
     assert e < TOL["float32"] and r < TOL["float32"]
+    sm = DirectSHT(Nl, Nx, dflt_type="float32")
+    sm.configure(scatter="matrix")
+    am, _ = sm.transform_device(th, ph, w)
+    torch.cuda.synchronize()
+    e = rel_max(am, a64)
+    print(f"C2 FP32 matrix engine vs FP64: max {e:.2e}")
+    assert e < TOL["float32"]
+    del sm, am, again
+    torch.cuda.empty_cache()
     e = rel_max(r32, sgn * a64)
     print(f"C2 FP32 reflection: {e:.2e}")
     assert e < TOL["float32"]

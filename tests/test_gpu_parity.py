@@ -273,6 +273,33 @@ def test_high_l_second_m_tile(torch_cuda, Nl):
     assert rel_max(a, ref) < 1e-6 and rel_rms(a, ref) < 1e-6
 
 
+def test_auto_engine_and_repeat(torch_cuda):
+    """The default ('auto') engine takes the spreading scatter once a transform
+    averages 400 points per segment (4 (Nx-1) segments) and the matrix engine
+    below; both against the oracle at <= 1e-6, and the spreading path (64-bit
+    fixed-point grids) gives the same bits on every call."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    from oracle import refalg
+    Nl, Nx = 64, 128
+    nseg = 4 * (Nx - 1)
+    s = DirectSHT(Nl, Nx, dflt_type="float32")
+    assert s.scatter_engine(400 * nseg - 1) == "matrix"
+    assert s.scatter_engine(400 * nseg) == "spread"
+    tabs = oracle_tables(Nl, Nx)
+    for n in (50000, 400 * nseg + 1000):
+        rng = np.random.default_rng(n)
+        t, p, w = _uniform(rng, n)
+        ref = refalg.direct_sht(tabs, t, p, w)
+        td, pd, wd = (torch.from_numpy(a).cuda() for a in (t, p, w))
+        a1 = s(td, pd, wd).cpu().numpy()
+        a2 = s(td, pd, wd).cpu().numpy()
+        e = rel_max(a1, ref)
+        print(f"n={n} ({s.scatter_engine(n)}): max {e:.2e}")
+        assert e < 1e-6 and rel_rms(a1, ref) < 1e-6
+        assert np.array_equal(a1, a2)
+
+
 @pytest.mark.parametrize("n", [1000, 100000])
 def test_single_ring_accumulation(torch_cuda, n):
     """All points in one spline bin (a ring of constant theta): the longest
