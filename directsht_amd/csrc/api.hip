@@ -664,6 +664,13 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   const bool sets_scatter = a.shared && uses_sets_scatter(pl, a.nsets);
   const bool many = !a.shared && uses_many(pl, a.nsets, mode);
   if (many) pl.scatter_now = DSHT_SCATTER_MATRIX;
+  const bool mfma = use_mfma_contract(pl, a.nsets);
+  // weight sets on the spreading scatter (matrix-core contraction): each set
+  // spread into a slot of its own, the slots folded a group at a time, so the
+  // fold writes whole rows of 8 sets (one set at a time writes 64 of every
+  // 16 * 4 * nsets bytes of a coefficient row)
+  const int spread_group =
+      a.shared && a.nsets > 1 && mfma && uses_spread(pl) ? std::min(a.nsets, kSpreadFoldGroup) : 0;
   std::vector<ManySets> groups;
   int rc = DSHT_OK;
   if (many) {
@@ -703,10 +710,9 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     rc = ensure_workspace_z(pl, z, a.nsets, false, 1, maxg);
   } else {
     rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl),
-                          sets_scatter ? a.nsets : 1);
+                          sets_scatter ? a.nsets : std::max(1, spread_group));
   }
   if (rc) return rc;
-  const bool mfma = use_mfma_contract(pl, a.nsets);
   const float* full32 = nullptr;
   if (mfma && (rc = ensure_full32(pl, &full32))) return rc;
   Workspace& ws = pl.ws;
@@ -737,7 +743,17 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     // one slot per set: fold every set in one launch
     const bool fold_all = sets_scatter && mfma;
     if (fold_all && (rc = launch_fold_sets(pl, a.nsets, s, ws.part, part_set_stride(pl)))) return rc;
-    for (int set = 0; set < a.nsets && !fold_all; ++set) {
+    for (int g0 = 0; spread_group && g0 < a.nsets; g0 += spread_group) {
+      const int g = std::min(spread_group, a.nsets - g0);
+      for (int k = 0; k < g; ++k) {
+        const double* w = a.wsets + (g0 + k) * a.ldw;
+        float* part = (float*)ws.part + k * part_set_stride(pl);
+        if ((rc = launch_weight_max(pl, Np, w, s, ws.wmax + g0 + k))) return rc;
+        if ((rc = launch_scatter_spread(pl, Np, w, 2, s, part, ws.wmax + g0 + k))) return rc;
+      }
+      if ((rc = launch_fold_sets(pl, a.nsets, s, ws.part, part_set_stride(pl), g0, g))) return rc;
+    }
+    for (int set = 0; set < a.nsets && !fold_all && !spread_group; ++set) {
       const double* w = a.wsets + set * a.ldw;
       if (sets_scatter) {
         const float* part = (const float*)ws.part + set * part_set_stride(pl);

@@ -362,15 +362,18 @@ int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part) {
   return fold_impl(pl, nsets, set, 1, s, part, 0);
 }
 
-int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride) {
+int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride,
+                     int set0, int count) {
   Workspace& ws = pl.ws;
-  if (!use_mfma_contract(pl, nsets) || nsets > kMaxSets) {
+  if (count < 0) count = nsets;
+  if (!use_mfma_contract(pl, nsets) || nsets > kMaxSets || set0 < 0 || count < 1 ||
+      set0 + count > nsets) {
     set_error("dsht: the batched fold serves the matrix-core contraction (4..64 sets)");
     return DSHT_EINVAL;
   }
   const int* run_total = ws.run_base + ws.cur_chunks;  // scan total = number of runs
-  k_seg_clear<<<(std::max(pl.nseg, nsets) + 255) / 256, 256, 0, s>>>(pl.nseg, ws.seg_run_lo,
-                                                                      ws.seg_run_hi, ws.rmax, nsets);
+  k_seg_clear<<<(std::max(pl.nseg, count) + 255) / 256, 256, 0, s>>>(
+      pl.nseg, ws.seg_run_lo, ws.seg_run_hi, ws.rmax + set0, count);
   DSHT_LAUNCH_CHECK();
   const int64_t max_runs = (int64_t)ws.cur_chunks + pl.nseg + 1;
   k_seg_runs<<<(int)((max_runs + 255) / 256), 256, 0, s>>>(ws.run_seg, run_total, ws.seg_run_lo,
@@ -378,8 +381,8 @@ int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64
   DSHT_LAUNCH_CHECK();
   dim3 grid(pl.Npair, (pl.Nl + kFoldM - 1) / kFoldM);
   k_fold_sets<<<grid, 256, 0, s>>>((const float*)part, part_stride, ws.seg_run_lo,
-                                   ws.seg_run_hi, 0, pl.Nl, pl.Nx, pl.Npair, pl.dx, nsets, 0,
-                                   nsets, (float*)ws.rhs, ws.rmax);
+                                   ws.seg_run_hi, 0, pl.Nl, pl.Nx, pl.Npair, pl.dx, count, set0,
+                                   nsets, (float*)ws.rhs, ws.rmax + set0);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

@@ -232,7 +232,10 @@ int launch_scatter(Plan& pl, int64_t Np, const double* wsort, hipStream_t s);
 // position], 2: w[rec.idx] (a weight set in the caller's input order)
 constexpr int kSpreadW = 8;        // kernel taps (beta = 2.3 W)
 constexpr int kSpreadMaxNl = 2048;  // grids of n <= 8192 (two 64-bit grids in LDS)
-int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
+// part: where the run partials go (NULL: ws.part); wmax: max |w| of the
+// weights already reduced (NULL: reduced here into ws.wmax when wmode != 0)
+int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s,
+                          float* part = nullptr, const unsigned* wmax = nullptr);
 // the spreading scatter serves FP32 plans up to kSpreadMaxNl; above it the
 // matrix engine takes its place
 inline bool uses_spread(const Plan& pl) {
@@ -258,8 +261,10 @@ inline int chunk_cap(const Plan& pl) {
   return uses_spread(pl) && !pl.chunk_explicit ? std::max(pl.chunk_max, kSpreadChunk)
                                                : pl.chunk_max;
 }
-// *ws.wmax = max |w[i]| over the finite entries of w[0..Np) (float bits)
-int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s);
+// *out (NULL: ws.wmax) = max |w[i]| over the finite entries of w[0..Np)
+// (float bits)
+int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s,
+                      unsigned* out = nullptr);
 // gather: w holds a weight set in the caller's input order (read through the
 // records' idx); otherwise w is wsort as for launch_scatter
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s);
@@ -303,9 +308,14 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
 // slot): slot `set` of nsets for the matrix-core contraction and caller
 // tables, else the single plain slot
 int launch_fold(Plan& pl, int nsets, int set, hipStream_t s, const void* part = nullptr);
-// the fold of all nsets slots at once for the matrix-core contraction (FP32),
-// slot s's partials at part + s * part_stride
-int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride);
+// the fold of weight sets set0 .. set0 + count - 1 (count < 0: all nsets) of
+// an nsets batch at once for the matrix-core contraction (FP32), set
+// set0 + k's partials at part + k * part_stride
+int launch_fold_sets(Plan& pl, int nsets, hipStream_t s, const void* part, int64_t part_stride,
+                     int set0 = 0, int count = -1);
+// weight-set batches on the spreading scatter spread the sets one by one into
+// slots of their own and fold up to this many slots at once
+constexpr int kSpreadFoldGroup = 16;
 // angular power spectra of nsets alm arrays (dsht_alm2cl)
 int launch_alm2cl(int Nl, int nsets, const double* alm, int64_t ld, double* cl, int cross,
                   hipStream_t s);

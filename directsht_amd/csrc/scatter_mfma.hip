@@ -580,11 +580,12 @@ __global__ void k_set_u32(unsigned* __restrict__ p, unsigned v) { *p = v; }
 // m tile of 256, 512 or 1024 orders: the narrowest that covers Nl, or 1024.
 // The operand scale comes from ws.wmax[0]: the points pass's max |w| for the
 // records' own weights; for a gathered weight set, its own max, reduced here.
-int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s) {
-  k_set_u32<<<1, 1, 0, s>>>(pl.ws.wmax, 0u);
+int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s, unsigned* out) {
+  if (!out) out = pl.ws.wmax;
+  k_set_u32<<<1, 1, 0, s>>>(out, 0u);
   DSHT_LAUNCH_CHECK();
   const int blocks = (int)std::min<int64_t>((Np + 255) / 256, 2048);
-  k_absmax<<<blocks, 256, 0, s>>>(Np, w, pl.ws.wmax);
+  k_absmax<<<blocks, 256, 0, s>>>(Np, w, out);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

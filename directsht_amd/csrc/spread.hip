@@ -309,28 +309,32 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 
 template <int LOGN>
 static int launch_spread_n(Plan& pl, int nchunks, const double* w, int wmode, const unsigned* wmax,
-                           hipStream_t s) {
+                           float* part, hipStream_t s) {
   using Gm = SpreadGeom<LOGN>;
   k_scatter_spread<LOGN><<<nchunks, Gm::kThreads, 0, s>>>(
       reinterpret_cast<const SpreadRec*>(pl.ws.rec), pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl, pl.ws.run_base, w, wmode,
-      (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.part, pl.ws.run_seg, wmax);
+      (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, part, pl.ws.run_seg, wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
 
-int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s) {
+int launch_scatter_spread(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s,
+                          float* part, const unsigned* wmax) {
   const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
   if (nchunks == 0) return DSHT_OK;
   int rc;
-  if (wmode != 0 && (rc = launch_weight_max(pl, Np, w, s))) return rc;
-  const unsigned* wmax = pl.ws.wmax;
+  if (!wmax) {
+    if (wmode != 0 && (rc = launch_weight_max(pl, Np, w, s))) return rc;
+    wmax = pl.ws.wmax;
+  }
+  if (!part) part = (float*)pl.ws.part;
   switch (pl.spread_logn) {
-    case 8: return launch_spread_n<8>(pl, nchunks, w, wmode, wmax, s);
-    case 9: return launch_spread_n<9>(pl, nchunks, w, wmode, wmax, s);
-    case 10: return launch_spread_n<10>(pl, nchunks, w, wmode, wmax, s);
-    case 11: return launch_spread_n<11>(pl, nchunks, w, wmode, wmax, s);
-    case 12: return launch_spread_n<12>(pl, nchunks, w, wmode, wmax, s);
-    case 13: return launch_spread_n<13>(pl, nchunks, w, wmode, wmax, s);
+    case 8: return launch_spread_n<8>(pl, nchunks, w, wmode, wmax, part, s);
+    case 9: return launch_spread_n<9>(pl, nchunks, w, wmode, wmax, part, s);
+    case 10: return launch_spread_n<10>(pl, nchunks, w, wmode, wmax, part, s);
+    case 11: return launch_spread_n<11>(pl, nchunks, w, wmode, wmax, part, s);
+    case 12: return launch_spread_n<12>(pl, nchunks, w, wmode, wmax, part, s);
+    case 13: return launch_spread_n<13>(pl, nchunks, w, wmode, wmax, part, s);
     default:
       set_error("dsht: the spreading scatter supports Nl <= " + std::to_string(kSpreadMaxNl));
       return DSHT_EINVAL;

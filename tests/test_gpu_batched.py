@@ -37,9 +37,13 @@ def points(torch, seed, n):
     return th, ph
 
 
-@pytest.mark.parametrize("dflt_type,S", [("float64", 3), ("float64", 6), ("float32", 3),
-                                         ("float32", 9), ("float32", 70)])
-def test_weight_sets_match_single(torch_cuda, dflt_type, S):
+@pytest.mark.parametrize("dflt_type,S,engine", [
+    ("float64", 3, None), ("float64", 6, None), ("float32", 3, None), ("float32", 9, None),
+    ("float32", 70, None), ("float32", 3, "spread"), ("float32", 21, "spread")])
+def test_weight_sets_match_single(torch_cuda, dflt_type, S, engine):
+    """Every set of a batch against the single transform of its weights.  On
+    the spreading scatter, 21 sets are spread into slots of their own and
+    folded in groups of 16 and 5."""
     torch = torch_cuda
     from directsht_amd.sht import DirectSHT
     n = 40000
@@ -49,6 +53,8 @@ def test_weight_sets_match_single(torch_cuda, dflt_type, S):
     W[0] = 1.0
     W[1] = -W[1] * 1e3  # a set of another sign and scale
     sht = DirectSHT(80, 160, dflt_type=dflt_type)
+    if engine:
+        sht.configure(scatter=engine)
     alm, status = sht.transform_sets_device(th, ph, W)
     torch.cuda.synchronize()
     assert int(status.max()) == 0

@@ -70,14 +70,17 @@ def test_single_transform_weight_scales(torch_cuda, tables):
     assert e < TOL32
 
 
-def test_weight_set_paths_weight_scales(torch_cuda, tables):
-    """one set (the scatter gathers the weights itself) and 5 sets (the
-    weight-set scatter, matrix-core contraction)"""
+@pytest.mark.parametrize("engine", ["matrix", "spread"])
+def test_weight_set_paths_weight_scales(torch_cuda, tables, engine):
+    """one set (the scatter gathers the weights itself) and 5 sets (matrix
+    engine: the weight-set scatter; spreading engine: one slot per set, folded
+    together), matrix-core contraction"""
     torch = torch_cuda
     from directsht_amd.sht import DirectSHT
     from oracle import refalg
     t, p, w = _points(13, 20000)
     sht = DirectSHT(Nl, Nx, dflt_type="float32")
+    sht.configure(scatter=engine)
     th = torch.from_numpy(t).cuda()
     ph = torch.from_numpy(p).cuda()
     W = np.stack([w * s for s in SCALES] + [_mixed(14, len(t))])
