@@ -6,7 +6,8 @@ own Nl, Nx).  BatchTransform prepares everything once -- resident tables per
 of their largest set), workspaces reserved to size, output buffers -- and
 captures the whole batch as one HIP graph.  Sets that share tables go through
 one dsht_transform_many call (up to 64 sets): each set is binned, scattered and folded
-into its own slot, and the table contraction (a matrix-core GEMM over all
+into its own slot (sparse sets in groups on the matrix engine, dense ones
+alone on the spreading scatter when the call's engine is 'auto'), and the table contraction (a matrix-core GEMM over all
 slots) and the polar rotation run once for the call, so the tables are
 streamed from HBM once per call instead of once per set.  Calls are spread over several streams
 (longest first, onto the least loaded stream) so they overlap, and the graph
@@ -27,7 +28,11 @@ import torch
 from . import _lib
 from .sht import DirectSHT
 
-CANDIDATES = (("matrix", 1024), ("matrix", 2048), ("matrix", 4096), ("vector", 2048))
+# (scatter engine, points per work item; None: the engine's default).  On
+# 'auto' a dsht_transform_many call sends its dense sets one by one through
+# the spreading scatter and groups the others on the matrix engine.
+CANDIDATES = (("matrix", 1024), ("matrix", 2048), ("matrix", 4096), ("vector", 2048),
+              ("auto", None))
 
 
 def size_bucket(npoints):
@@ -70,7 +75,7 @@ class Autotuner:
             if engine == "vector" and base.dflt_type != "float32":
                 continue
             sht = base.clone()
-            sht.configure(scatter=engine, chunk=chunk)
+            sht.configure(scatter=engine, chunk=chunk)  # chunk None: the default
             sht.reserve(n)
             sht.transform_device(th, ph, w, out=out)  # warm-up
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -672,16 +672,31 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   const int spread_group =
       a.shared && a.nsets > 1 && mfma && uses_spread(pl) ? std::min(a.nsets, kSpreadFoldGroup) : 0;
   std::vector<ManySets> groups;
+  std::vector<int> group_first, dense_sets;
+  // point sets on the automatic engine: a set dense enough for the spreading
+  // scatter goes alone through it (its cost is per point and per run, the
+  // matrix engine's per point and order); the others are sorted, scattered
+  // (matrix engine) and folded in groups of consecutive sets
+  auto dense = [&](int k) {
+    return pl.scatter == DSHT_SCATTER_AUTO && pick_scatter(pl, a.Np[k]) == DSHT_SCATTER_SPREAD;
+  };
   int rc = DSHT_OK;
   if (many) {
     WsSizes z{1, 1, 1, 1};
     int maxg = 1;
+    int64_t max_dense = 0;
     const int cap = many_group_cap(pl);
     for (int g0 = 0; g0 < a.nsets;) {
+      if (dense(g0)) {
+        dense_sets.push_back(g0);
+        max_dense = std::max(max_dense, a.Np[g0++]);
+        continue;
+      }
       ManySets ms{};
       int64_t tot = a.Np[g0];
       int g = 1;
-      while (g0 + g < a.nsets && g < cap && tot + a.Np[g0 + g] <= kManyPoints) tot += a.Np[g0 + g++];
+      while (g0 + g < a.nsets && g < cap && !dense(g0 + g) && tot + a.Np[g0 + g] <= kManyPoints)
+        tot += a.Np[g0 + g++];
       ms.nsets = g;
       const int tile = sort_tile(tot);
       const int chunk = work_chunk(pl.chunk_max, tot);
@@ -705,9 +720,11 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       z.runs = std::max<int64_t>(z.runs, ms.chunk_off[g] + runs);
       maxg = std::max(maxg, g);
       groups.push_back(ms);
+      group_first.push_back(g0);
       g0 += g;
     }
-    rc = ensure_workspace_z(pl, z, a.nsets, false, 1, maxg);
+    if (!groups.empty()) rc = ensure_workspace_z(pl, z, a.nsets, false, 1, maxg);
+    if (rc == DSHT_OK && max_dense > 0) rc = ensure_workspace(pl, max_dense, a.nsets);
   } else {
     rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl),
                           sets_scatter ? a.nsets : std::max(1, spread_group));
@@ -773,12 +790,25 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
     }
   } else if (many) {
-    int g0 = 0;
-    for (const ManySets& ms : groups) {
+    pl.scatter_now = DSHT_SCATTER_MATRIX;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const ManySets& ms = groups[gi];
+      const int g0 = group_first[gi];
       if ((rc = launch_points_many(pl, ms, mode, ws.flags + 4 * g0, ws.wmax + g0, s))) return rc;
       if ((rc = launch_scatter_mfma_many(pl, ms, ws.wmax + g0, s))) return rc;
       if ((rc = launch_fold_many(pl, ms, g0, a.nsets, s))) return rc;
-      g0 += ms.nsets;
+    }
+    // the dense sets after every group: each uses the single transform's
+    // arrays (ws.wmax[0] among them), which the groups' launches have read
+    for (const int set : dense_sets) {
+      const int64_t Np = a.Np[set];
+      pl.scatter_now = DSHT_SCATTER_SPREAD;
+      pl.chunk = work_chunk(chunk_cap(pl), Np);
+      ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+      if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
+        return rc;
+      if ((rc = launch_scatter(pl, Np, nullptr, s))) return rc;
+      if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
     }
   } else {
     for (int set = 0; set < a.nsets; ++set) {

@@ -170,6 +170,44 @@ def test_many_point_sets_in_groups(torch_cuda):
             assert rel(alm[i], ref) < TOL["float32"], i
 
 
+def test_many_point_sets_dense_and_sparse(torch_cuda):
+    """A point-set batch on the automatic engine: the sets with at least 400
+    points per segment go one by one through the spreading scatter, the
+    others in groups of consecutive sparse sets through the matrix engine,
+    all into one matrix-core contraction.  Dense and sparse sets interleaved
+    (and an empty one), within a reservation: each set within 1e-6 of its
+    single transform and of the FP64 oracle, and nothing grows."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    from oracle import refalg
+    Nl, Nx = 48, 96
+    dense_n = 400 * 4 * (Nx - 1)
+    sizes = [dense_n + 5000, 700, 20000, 0, 3 * dense_n, 150, dense_n, 45000, 9000, 2 * dense_n - 1]
+    sets = []
+    for i, n in enumerate(sizes):
+        th, ph = points(torch, 900 + i, n)
+        sets.append((th, ph, 0.5 + torch.rand(n, device="cuda", dtype=torch.float64)))
+    sht = DirectSHT(Nl, Nx, dflt_type="float32")
+    assert [sht.scatter_engine(n) for n in sizes].count("spread") == 4
+    sht.reserve(max(sizes), len(sizes), weight_sets=False)
+    f0 = sht.footprint()
+    alm, status = sht.transform_many_device(sets)
+    again, _ = sht.transform_many_device(sets)
+    torch.cuda.synchronize()
+    assert int(status.max()) == 0
+    assert sht.footprint() == f0
+    assert torch.equal(alm, again)
+    tables = refalg.make_tables(Nl, Nx, 0.75)
+    for i, (th, ph, w) in enumerate(sets):
+        if th.numel() == 0:
+            assert float(alm[i].abs().max()) == 0.0, i
+            continue
+        assert rel(alm[i], sht(th, ph, w)) < TOL["float32"], i
+        if i in (0, 1, 4, 7):
+            ref = refalg.direct_sht(tables, *(a.cpu().numpy() for a in (th, ph, w)))
+            assert rel(alm[i], ref) < TOL["float32"], i
+
+
 def test_reserve_kinds_cover_their_transforms(torch_cuda):
     """A point-set reservation holds every dsht_transform_many group of its
     sizes, a weight-set reservation every dsht_transform_sets: the plan's
