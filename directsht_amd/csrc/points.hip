@@ -514,6 +514,11 @@ __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile, int 
     sg[u] = seg_of_pt[i];
     rk[u] = rank_of_pt[i];
   }
+  // the write offsets, as soon as the segments are known (unconditional:
+  // the FP64 classification below runs while they are in flight)
+  int off[kPlacePer];
+#pragma unroll
+  for (int u = 0; u < kPlacePer; ++u) off[u] = toff[(int64_t)t * nb + min(max(sg[u], 0), p.nseg)];
   float vmax = 0.f;
 #pragma unroll
   for (int u = 0; u < kPlacePer; ++u) {
@@ -521,7 +526,7 @@ __global__ __launch_bounds__(256) void k_place_ranked(int64_t Np, int tile, int 
     if (i < e) {
       const Classified c = classify(th[u], ph[u], ww[u], p);
       vmax = fmaxf(vmax, abs_w(ww[u], sg[u], p.nseg));
-      place_point(c, sg[u], i, toff[(int64_t)t * nb + sg[u]] + rk[u], p, rec);
+      place_point(c, sg[u], i, off[u] + rk[u], p, rec);
     }
   }
   block_wmax(vmax, wmax);
