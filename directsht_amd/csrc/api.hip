@@ -819,9 +819,14 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
         return rc;
       if ((rc = mark(1))) return rc;
-      if ((rc = launch_scatter(pl, Np, nullptr, s))) return rc;
-      if ((rc = mark(2))) return rc;
-      if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
+      if (uses_node_spread(pl, a.nsets)) {  // scatter and fold in one launch
+        if ((rc = launch_spread_nodes(pl, Np, nullptr, 0, s))) return rc;
+        if ((rc = mark(2))) return rc;
+      } else {
+        if ((rc = launch_scatter(pl, Np, nullptr, s))) return rc;
+        if ((rc = mark(2))) return rc;
+        if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
+      }
       if ((rc = mark(3))) return rc;
       if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
     }

@@ -291,6 +291,18 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
   return pl.precision == DSHT_FP32 && !pl.caller_tables && nsets >= kMfmaMinSets &&
          pl.contract == DSHT_CONTRACT_MATRIX;
 }
+// The node variant of the spreading scatter (scatter and fold in one
+// launch, one workgroup per node pair): transforms that write the plain
+// coefficient layout (no matrix-core contraction after them) on grids of at
+// least kNodeMinPairs node pairs (fewer leave the device short of work) and
+// at most 4096 cells (Nl <= 1024).  rhs: the plain layout [m][P][np][24];
+// w, wmode as launch_scatter_spread.
+constexpr int kNodeMinPairs = 512;
+inline bool uses_node_spread(const Plan& pl, int nsets) {
+  return uses_spread(pl) && !use_mfma_contract(pl, nsets) && pl.Npair >= kNodeMinPairs &&
+         pl.spread_logn <= 12;
+}
+int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
 // weight sets on shared points (DSHT_FP32, matrix scatter engine, from
 // kSetsScatterMin sets): one scatter of all sets, set s's run partials at
 // part + s * part_set_stride(pl)

@@ -149,9 +149,9 @@ __device__ __forceinline__ int spec_pos(int f) {
 
 // One in-place DIF pass of radix R and span S over the NCG complex grids
 // (butterflies bi = tid, tid + kThreads, ...).  The twiddles of radix-8 pass
-// P come from twr[P] (this thread's, loaded once) for P < kTwReg, else from
-// the table tw[k] = e^{-2 pi i k / n}.
-template <int LOGN, int R, int S, int P>
+// P come from twr[P] (this thread's, loaded once) for P < TWR (at most
+// kTwReg), else from the table tw[k] = e^{-2 pi i k / n}.
+template <int LOGN, int R, int S, int P, int TWR = SpreadGeom<LOGN>::kTwReg>
 __device__ __forceinline__ void fft_pass(float2* __restrict__ sz, const float2 (&twr)[3][7],
                                          const float2* __restrict__ tw, int tid) {
   using Gm = SpreadGeom<LOGN>;
@@ -170,20 +170,20 @@ __device__ __forceinline__ void fft_pass(float2* __restrict__ sz, const float2 (
     if (S > R) {
 #pragma unroll
       for (int t = 1; t < R; ++t)
-        a[t] = c_mul(a[t], P < Gm::kTwReg ? twr[P < 3 ? P : 0][t - 1] : tw[t * j * ts]);
+        a[t] = c_mul(a[t], P < TWR ? twr[P < 3 ? P : 0][t - 1] : tw[t * j * ts]);
     }
 #pragma unroll
     for (int t = 0; t < R; ++t) base[fpad(i0 + t * q)] = a[t];
   }
 }
 
-template <int LOGN, int P>
+template <int LOGN, int P, int TWR = SpreadGeom<LOGN>::kTwReg>
 __device__ __forceinline__ void fft_passes8(float2* sz, const float2 (&twr)[3][7],
                                             const float2* tw, int tid) {
   if constexpr (P < SpreadGeom<LOGN>::kPasses8) {
-    fft_pass<LOGN, 8, (SpreadGeom<LOGN>::n >> (3 * P)), P>(sz, twr, tw, tid);
+    fft_pass<LOGN, 8, (SpreadGeom<LOGN>::n >> (3 * P)), P, TWR>(sz, twr, tw, tid);
     __syncthreads();
-    fft_passes8<LOGN, P + 1>(sz, twr, tw, tid);
+    fft_passes8<LOGN, P + 1, TWR>(sz, twr, tw, tid);
   }
 }
 
@@ -304,6 +304,223 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
     if (tid == 0) run_seg[run] = cur;
     ++run;
     p = pe;
+  }
+}
+
+// Node variant (plain transforms): the spreading scatter and the node/parity
+// fold in one kernel, so the per-run partials never leave the workgroup.
+// Workgroup = node pair np; its four units (node n = 2 np + odd, branch br)
+// are done one after another.  Unit (n, br) spreads the points of bins n - 1
+// and n of segments (br, h, .) onto four real grids, for h = 0, 1:
+//   c^h = sum_{bin n} a0 psi + sum_{bin n-1} a2 psi,
+//   d^h = sum_{bin n} a1 psi + sum_{bin n-1} a3 psi,
+// whose deconvolved spectra are the fold's c and d of node n (the v0(n) +
+// v2(n-1) of sht/shared_interp_funcs.py:88-105 summed per node, sht/sht.py:
+// 266-277).  Each point is spread twice (once for each node it touches), two
+// of its four weights each time: as many grid updates as the run-based
+// kernel, but one FFT of four real grids per (node, branch) instead of one
+// per (segment) piece: half the FFTs, and no partials written or read.  The
+// thread of mode m keeps the first node's c^h, d^h at m in registers; at the
+// second it forms the parity sums c_P = c^0 +- c^1, d_P (times dx) and the
+// derivative coefficients f_P = x d_P / (1 - x^2), g = d_{1-P} / (1 - x^2)
+// of the pair's two nodes and writes the branch's half of rhs[m][P][np][24]
+// (the fold's layout: [c | f | g][equ re, equ im, pol re, pol im][node]).
+// (One workgroup per node pair and branch measured slower.)  Grids up to
+// n = 4096 (all four in LDS at once, one mode per thread).
+template <int LOGN>
+__global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
+    const SpreadRec* __restrict__ rec, const int* __restrict__ seg_start, int Nx, int Nl,
+    int Npair, double dx, const double* __restrict__ w, int wmode, float beta,
+    const float2* __restrict__ tw, const float* __restrict__ ipsi, float* __restrict__ rhs,
+    const unsigned* __restrict__ wmax) {
+  using Gm = SpreadGeom<LOGN>;
+  constexpr int n = Gm::n, G = Gm::G, NCG = Gm::NCG, kT = Gm::kThreads;
+  constexpr int kW = kSpreadW;
+  static_assert(G == 4 && kT >= n / 4, "one pass over four grids, one mode per thread");
+  __shared__ __attribute__((aligned(16))) unsigned long long sgrid[G * n];
+  float2* sz = reinterpret_cast<float2*>(sgrid);
+  const int tid = threadIdx.x;
+  const int np = blockIdx.x;
+  const int m = tid;  // this thread's mode
+  // twiddles of the first radix-8 pass in registers (the spectra held across
+  // the units leave no room for more), the others from the table
+  constexpr int kTwr = Gm::kTwReg < 1 ? Gm::kTwReg : 1;
+  float2 twr[3][7];
+  if (kTwr > 0) {
+    const int j = tid % (n / 8);
+#pragma unroll
+    for (int t = 1; t < 8; ++t) twr[0][t - 1] = tw[t * j];
+  }
+  int ex = 0;
+  {
+    const float mx = __uint_as_float(wmax[0]);
+    if (mx > 0.f && mx <= 3.0e38f) (void)frexpf(mx, &ex);
+    ex = max(-120, min(120, ex));
+  }
+  const double w_fix = ldexp(1.0, 40 - ex);
+  const float w_out = ldexpf(1.0f, ex);
+  double xs[2], inv[2];
+#pragma unroll
+  for (int odd = 0; odd < 2; ++odd) {
+    const int nd = 2 * np + odd;
+    xs[odd] = (double)nd * dx;
+    inv[odd] = nd < Nx ? 1.0 / (1.0 - xs[odd] * xs[odd]) : 0.0;
+  }
+
+  // the records of branch br at mode m from the pair's spectra
+  // a[odd][h][c re, c im, d re, d im], as the fold writes them: per P,
+  // [c | f | g] x (re, im) x (node 0, 1), three 16-byte pieces of 96 bytes
+  auto emit = [&](int br, const float (&a0)[2][4], const float (&a1)[2][4]) {
+#pragma unroll
+    for (int P = 0; P < 2; ++P) {
+      const double sg = P ? -1.0 : 1.0;
+      float c4[4], f4[4], g4[4];
+#pragma unroll
+      for (int ri = 0; ri < 2; ++ri)
+#pragma unroll
+        for (int odd = 0; odd < 2; ++odd) {
+          const float (&a)[2][4] = odd ? a1 : a0;
+          const double cv = (double)a[0][ri] + sg * (double)a[1][ri];
+          const double dv = dx * ((double)a[0][2 + ri] + sg * (double)a[1][2 + ri]);
+          const double du = dx * ((double)a[0][2 + ri] - sg * (double)a[1][2 + ri]);  // d_{1-P}
+          c4[2 * ri + odd] = (float)cv;
+          f4[2 * ri + odd] = (float)(xs[odd] * inv[odd] * dv);
+          g4[2 * ri + odd] = (float)(inv[odd] * du);
+        }
+      float* o = rhs + (((int64_t)m * 2 + P) * Npair + np) * 24 + 4 * br;
+      *reinterpret_cast<float4*>(o) = make_float4(c4[0], c4[1], c4[2], c4[3]);
+      *reinterpret_cast<float4*>(o + 8) = make_float4(f4[0], f4[1], f4[2], f4[3]);
+      *reinterpret_cast<float4*>(o + 16) = make_float4(g4[0], g4[1], g4[2], g4[3]);
+    }
+  };
+
+#pragma unroll 1
+  for (int br = 0; br < 2; ++br) {
+    float a0[2][4];  // the pair's first node at mode m: [h][c re, c im, d re, d im]
+#pragma unroll
+    for (int odd = 0; odd < 2; ++odd) {
+      const int node = 2 * np + odd;
+      // hemisphere h: points [lo, hi) of bins node - 1 (below mid) and node
+      int lo[2], mid[2], hi[2];
+      bool any = false;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int base = (br * 2 + h) * (Nx - 1);
+        const int jlo = max(node - 1, 0), jhi = min(node, Nx - 2);
+        if (jlo <= jhi) {
+          lo[h] = seg_start[base + jlo];
+          hi[h] = seg_start[base + jhi + 1];
+          mid[h] = node <= Nx - 2 ? seg_start[base + node] : hi[h];
+        } else {
+          lo[h] = mid[h] = hi[h] = 0;
+        }
+        any |= lo[h] < hi[h];
+      }
+      float a[2][4] = {};
+      if (any) {  // uniform; no points at this node: zero spectra, no transform
+        for (int i = tid; i < G * n / 2; i += kT)
+          reinterpret_cast<ulonglong2*>(sgrid)[i] = make_ulonglong2(0ull, 0ull);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          unsigned long long* gc = sgrid + (2 * h) * n;  // c^h, then d^h
+          unsigned long long* gd = gc + n;
+          for (int q = lo[h] + tid; q < hi[h]; q += kT) {
+            const float4 ah = reinterpret_cast<const float4*>(rec + q)[0];
+            const int4 ri = reinterpret_cast<const int4*>(rec + q)[1];  // seg, idx, j0, z0
+            const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri.y])) * w_fix;
+            const bool upper = q >= mid[h];  // bin node: weights a0, a1; bin node - 1: a2, a3
+            const double ac = wt * (double)(upper ? ah.x : ah.z);
+            const double ad = wt * (double)(upper ? ah.y : ah.w);
+            const int j0 = ri.z;
+            const float z0 = __int_as_float(ri.w);
+#pragma unroll
+            for (int i = 0; i < kW; ++i) {
+              const float z = (z0 + (float)i) * (2.0f / kW);
+              const double psi = (double)expf(beta * (sqrtf(fmaxf(0.f, 1.f - z * z)) - 1.f));
+              const int x = (j0 + i) & (n - 1);
+              atomicAdd(&gc[x], to_fixed(ac, psi));
+              atomicAdd(&gd[x], to_fixed(ad, psi));
+            }
+          }
+        }
+        __syncthreads();
+        {
+          constexpr int q = n / 8;
+          const int cg = tid / q, j = tid % q;
+          float2 v[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const int x = j + t * q;
+            v[t] = make_float2(from_fixed(sgrid[(2 * cg) * n + x]),
+                               from_fixed(sgrid[(2 * cg + 1) * n + x]));
+          }
+          __syncthreads();
+          dft<8>(v);
+#pragma unroll
+          for (int t = 1; t < 8; ++t) v[t] = c_mul(v[t], kTwr > 0 ? twr[0][t - 1] : tw[t * j]);
+          float2* base = sz + cg * Gm::kPad;
+#pragma unroll
+          for (int t = 0; t < 8; ++t) base[fpad(j + t * q)] = v[t];
+        }
+        __syncthreads();
+        fft_passes8<LOGN, 1, kTwr>(sz, twr, tw, tid);
+        if constexpr (Gm::kRadixLast > 1) {
+          fft_pass<LOGN, Gm::kRadixLast, Gm::kRadixLast, Gm::kPasses8, kTwr>(sz, twr, tw, tid);
+          __syncthreads();
+        }
+        // complex grid h = c^h + i d^h: split and deconvolve
+        if (m < Nl) {
+          const int pa = fpad(spec_pos<LOGN>(m)), pb = fpad(spec_pos<LOGN>((n - m) & (n - 1)));
+          const float s = 0.5f * ipsi[m] * w_out;
+#pragma unroll
+          for (int h = 0; h < NCG; ++h) {
+            const float2 z = sz[h * Gm::kPad + pa], v = sz[h * Gm::kPad + pb];
+            a[h][0] = (z.x + v.x) * s;
+            a[h][1] = (z.y - v.y) * s;
+            a[h][2] = (z.y + v.y) * s;
+            a[h][3] = (v.x - z.x) * s;
+          }
+        }
+        __syncthreads();  // the spectrum is read before the next grids are cleared
+      }
+      if (odd == 0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) a0[h][q] = a[h][q];
+      } else if (m < Nl) {
+        emit(br, a0, a);
+      }
+    }
+  }
+}
+
+template <int LOGN>
+static int launch_nodes_n(Plan& pl, const double* w, int wmode, const unsigned* wmax,
+                          hipStream_t s) {
+  using Gm = SpreadGeom<LOGN>;
+  k_spread_nodes<LOGN><<<pl.Npair, Gm::kThreads, 0, s>>>(
+      reinterpret_cast<const SpreadRec*>(pl.ws.rec), pl.ws.seg_start, pl.Nx, pl.Nl, pl.Npair,
+      pl.dx, w, wmode, (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.rhs,
+      wmax);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s) {
+  int rc;
+  if (wmode != 0 && Np > 0 && (rc = launch_weight_max(pl, Np, w, s))) return rc;
+  const unsigned* wmax = pl.ws.wmax;
+  switch (pl.spread_logn) {
+    case 8: return launch_nodes_n<8>(pl, w, wmode, wmax, s);
+    case 9: return launch_nodes_n<9>(pl, w, wmode, wmax, s);
+    case 10: return launch_nodes_n<10>(pl, w, wmode, wmax, s);
+    case 11: return launch_nodes_n<11>(pl, w, wmode, wmax, s);
+    case 12: return launch_nodes_n<12>(pl, w, wmode, wmax, s);
+    default:
+      set_error("dsht: the node spreading scatter supports grids up to 4096 cells");
+      return DSHT_EINVAL;
   }
 }
 
