@@ -418,8 +418,8 @@ static void otf_coefficients(int Nl, std::vector<double>& rc) {
 // 2.3 W, the twiddles e^{-2 pi i k / n} and 1/psihat(m) for m < Nl, with
 // psihat(m) = (W/2) int_{-1}^{1} exp(beta (sqrt(1 - z^2) - 1)) cos(pi m W z / n) dz
 // by 128-point Gauss-Legendre quadrature (FP64).
-static void spread_constants(int Nl, int& logn, double& beta, std::vector<float2>& tw,
-                             std::vector<float>& ipsi) {
+static int spread_constants(int Nl, int& logn, double& beta, std::vector<float2>& tw,
+                            std::vector<float>& ipsi) {
   logn = 8;
   while ((1 << logn) < 4 * Nl) ++logn;
   const int n = 1 << logn;
@@ -449,14 +449,19 @@ static void spread_constants(int Nl, int& logn, double& beta, std::vector<float2
     zq[i] = z;
     wq[i] = 2.0 / ((1.0 - z * z) * dp * dp);
   }
+  // the transform of the kernel the device evaluates (its sqrt and exp are
+  // the hardware's, not correctly rounded): its values at the nodes, taken
+  // at the nodes rounded to FP32 as the scatters' offsets are
+  float zf[kq], psi[kq];
+  for (int i = 0; i < kq; ++i) zf[i] = (float)zq[i];
+  if (int rc = spread_kernel_values(zf, kq, beta, psi)) return rc;
   ipsi.resize(Nl);
   for (int m = 0; m < Nl; ++m) {
     double sum = 0.0;
-    for (int i = 0; i < kq; ++i)
-      sum += wq[i] * std::exp(beta * (std::sqrt(1.0 - zq[i] * zq[i]) - 1.0)) *
-             std::cos(M_PI * m * W * zq[i] / n);
+    for (int i = 0; i < kq; ++i) sum += wq[i] * (double)psi[i] * std::cos(M_PI * m * W * zf[i] / n);
     ipsi[m] = (float)(1.0 / (0.5 * W * sum));
   }
+  return DSHT_OK;
 }
 
 // alloc_tables: the plan's Y-only table (FP64 or FP32 by precision), built
@@ -529,7 +534,7 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   if (precision == DSHT_FP32) {
     std::vector<float2> tw;
     std::vector<float> ipsi;
-    spread_constants(Nl, pl.spread_logn, pl.spread_beta, tw, ipsi);
+    if (int rc = spread_constants(Nl, pl.spread_logn, pl.spread_beta, tw, ipsi)) return rc;
     DSHT_ALLOC(pl.spread_tw, tw.size() * sizeof(float2));
     DSHT_ALLOC(pl.spread_ipsi, ipsi.size() * sizeof(float));
     DSHT_HIP(hipMemcpy(pl.spread_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));

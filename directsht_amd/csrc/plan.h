@@ -303,6 +303,9 @@ inline bool uses_node_spread(const Plan& pl, int nsets) {
          pl.spread_logn <= 12;
 }
 int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
+// the spreading kernel's values at the nq points z (host arrays), as the
+// device evaluates them in the scatters (synchronous; at plan creation)
+int spread_kernel_values(const float* z, int nq, double beta, float* out);
 // weight sets on shared points (DSHT_FP32, matrix scatter engine, from
 // kSetsScatterMin sets): one scatter of all sets, set s's run partials at
 // part + s * part_set_stride(pl)

@@ -108,6 +108,20 @@ __device__ __forceinline__ float from_fixed(unsigned long long v) {
   return __builtin_fmaf((float)hi, 0x1p-8f, (float)lo * 0x1p-40f);
 }
 
+// The kernel at z = (x - t) 2 / W, z in [-1, 1]:
+//   psi = exp(beta (sqrt(1 - z^2) - 1)) = 2^(beta log2(e) (sqrt(1 - z^2) - 1)),
+// with the hardware square root and base-2 exponential (about 1 ulp each; the
+// exponent lies in [-beta log2(e), 0], so neither needs range handling):
+// six instructions per tap, where the correctly rounded sqrtf and expf
+// expand to about thirty.  Their rounding is not centred (it scaled the
+// transform by ~2e-7), so the deconvolution divides by the transform of
+// this very function: its values at the quadrature nodes come from the
+// device (spread_kernel_values).
+__device__ __forceinline__ float es_kernel(float z, float beta_log2e) {
+  const float s = __builtin_amdgcn_sqrtf(fmaxf(0.f, __builtin_fmaf(-z, z, 1.f)));
+  return __builtin_amdgcn_exp2f(__builtin_fmaf(beta_log2e, s, -beta_log2e));
+}
+
 }  // namespace
 
 // Grid geometry of the kernel for n = 2^LOGN: G real grids held per pass over
@@ -192,7 +206,7 @@ template <int LOGN>
 __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
     const SpreadRec* __restrict__ rec, const int* __restrict__ seg_start, int nseg, int chunk,
     int Nl, const int* __restrict__ run_base, const double* __restrict__ w, int wmode,
-    float beta, const float2* __restrict__ tw, const float* __restrict__ ipsi,
+    float beta_log2e, const float2* __restrict__ tw, const float* __restrict__ ipsi,
     float* __restrict__ part, int* __restrict__ run_seg, const unsigned* __restrict__ wmax) {
   using Gm = SpreadGeom<LOGN>;
   constexpr int n = Gm::n, G = Gm::G, NCG = Gm::NCG, kT = Gm::kThreads;
@@ -246,11 +260,11 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 #pragma unroll
         for (int g = 0; g < G; ++g) a[g] = wt * (double)hk[pass * G + g];
         const int j0 = ri.z;
-        const float z0 = __int_as_float(ri.w);  // in [-W/2, -W/2 + 1)
+        const float zs = __int_as_float(ri.w) * (2.0f / kW);  // z0 in [-W/2, -W/2 + 1), scaled
 #pragma unroll
         for (int i = 0; i < kW; ++i) {
-          const float z = (z0 + (float)i) * (2.0f / kW);
-          const double psi = (double)expf(beta * (sqrtf(fmaxf(0.f, 1.f - z * z)) - 1.f));
+          const float z = zs + (float)i * (2.0f / kW);
+          const double psi = (double)es_kernel(z, beta_log2e);
           const int x = (j0 + i) & (n - 1);
 #pragma unroll
           for (int g = 0; g < G; ++g) atomicAdd(&sgrid[g * n + x], to_fixed(a[g], psi));
@@ -330,7 +344,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 template <int LOGN>
 __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
     const SpreadRec* __restrict__ rec, const int* __restrict__ seg_start, int Nx, int Nl,
-    int Npair, double dx, const double* __restrict__ w, int wmode, float beta,
+    int Npair, double dx, const double* __restrict__ w, int wmode, float beta_log2e,
     const float2* __restrict__ tw, const float* __restrict__ ipsi, float* __restrict__ rhs,
     const unsigned* __restrict__ wmax) {
   using Gm = SpreadGeom<LOGN>;
@@ -433,11 +447,11 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
             const double ac = wt * (double)(upper ? ah.x : ah.z);
             const double ad = wt * (double)(upper ? ah.y : ah.w);
             const int j0 = ri.z;
-            const float z0 = __int_as_float(ri.w);
+            const float zs = __int_as_float(ri.w) * (2.0f / kW);  // z0, scaled
 #pragma unroll
             for (int i = 0; i < kW; ++i) {
-              const float z = (z0 + (float)i) * (2.0f / kW);
-              const double psi = (double)expf(beta * (sqrtf(fmaxf(0.f, 1.f - z * z)) - 1.f));
+              const float z = zs + (float)i * (2.0f / kW);
+              const double psi = (double)es_kernel(z, beta_log2e);
               const int x = (j0 + i) & (n - 1);
               atomicAdd(&gc[x], to_fixed(ac, psi));
               atomicAdd(&gd[x], to_fixed(ad, psi));
@@ -496,13 +510,36 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
   }
 }
 
+__global__ void k_es_values(const float* __restrict__ z, int nq, float beta_log2e,
+                            float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nq) out[i] = es_kernel(z[i], beta_log2e);
+}
+
+int spread_kernel_values(const float* z, int nq, double beta, float* out) {
+  float* d = nullptr;
+  DSHT_HIP(hipMalloc(&d, 2 * (size_t)nq * sizeof(float)));
+  hipError_t e = hipMemcpy(d, z, nq * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    k_es_values<<<(nq + 255) / 256, 256>>>(d, nq, (float)(beta * M_LOG2E), d + nq);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d + nq, nq * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) {
+    set_error(std::string("spread_kernel_values: ") + hipGetErrorString(e));
+    return DSHT_EHIP;
+  }
+  return DSHT_OK;
+}
+
 template <int LOGN>
 static int launch_nodes_n(Plan& pl, const double* w, int wmode, const unsigned* wmax,
                           hipStream_t s) {
   using Gm = SpreadGeom<LOGN>;
   k_spread_nodes<LOGN><<<pl.Npair, Gm::kThreads, 0, s>>>(
       reinterpret_cast<const SpreadRec*>(pl.ws.rec), pl.ws.seg_start, pl.Nx, pl.Nl, pl.Npair,
-      pl.dx, w, wmode, (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.rhs,
+      pl.dx, w, wmode, (float)(pl.spread_beta * M_LOG2E), pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.rhs,
       wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
@@ -530,7 +567,7 @@ static int launch_spread_n(Plan& pl, int nchunks, const double* w, int wmode, co
   using Gm = SpreadGeom<LOGN>;
   k_scatter_spread<LOGN><<<nchunks, Gm::kThreads, 0, s>>>(
       reinterpret_cast<const SpreadRec*>(pl.ws.rec), pl.ws.seg_start, pl.nseg, pl.chunk, pl.Nl, pl.ws.run_base, w, wmode,
-      (float)pl.spread_beta, pl.spread_tw, pl.spread_ipsi, part, pl.ws.run_seg, wmax);
+      (float)(pl.spread_beta * M_LOG2E), pl.spread_tw, pl.spread_ipsi, part, pl.ws.run_seg, wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }
