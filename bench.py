@@ -665,11 +665,14 @@ def main():
     # executes that form on the fp16 matrix cores (the real GEMM
     # [(k, m1, re|im) x point] . [point x (re|im, m2)], 32 flop per (point,
     # m), three times for the fp16 hi/lo split = 96 MFMA flop).  The
-    # spreading engine computes the same sums as a non-uniform FFT per run
-    # (32 fixed-point grid updates per point and one FFT of four grids per
-    # run), so 22 N Nl / t is only its direct-sum-equivalent rate; its own
-    # bound is LDS (integer atomics, FFT exchanges) and its HBM bytes are the
-    # 64-byte sorted records in and the run partials out.
+    # spreading engine computes the same sums as non-uniform FFTs (32
+    # fixed-point grid updates per point; on grids up to 4096 cells with at
+    # least 512 node pairs the node variant k_spread_nodes, one FFT of four
+    # grids per (node, branch), the fold included; otherwise one FFT per run
+    # and the fold after it), so 22 N Nl / t is only its
+    # direct-sum-equivalent rate; its own bound is LDS (integer atomics, FFT
+    # exchanges) and its HBM bytes are the 32-byte sorted records in and the
+    # node coefficients (or run partials) out.
     scat_s = stage_avg["scatter"] * 1e-3
     engine = sht.scatter_engine(N)
     default_workload = (N == 10_000_000)
@@ -691,17 +694,19 @@ def main():
             "mfma_instruction_flop": "96 per (point, m): 32-flop real GEMM form, x3 for the "
                                      "fp16 hi/lo split"}
     else:
-        runs_bytes = tr("k_scatter_spread")
+        node = 2 * ((Nx + 3) // 4) >= 512 and Nl <= 1024
+        kname = "k_spread_nodes" if node else "k_scatter_spread"
         rec_bytes = 32.0 * N
         roof_scatter = {
-            "kernel": "k_scatter_spread", "engine": engine, "bound": "lds",
+            "kernel": kname, "engine": engine + (" (node variant, fold included)" if node else ""),
+            "bound": "lds",
             "direct_sum_equivalent_tflops": alg_tf,
             "direct_sum_equivalent": "22 N Nl / t: the reference algorithm's flop count, "
                                      "not flop executed (non-uniform FFT per run)",
             "records": "32-byte sorted records in (N x 32 B)",
             "hbm_GBs_records_in": rec_bytes / scat_s / 1e9,
             "hbm_frac_records_in": rec_bytes / scat_s / 1e9 / PEAK_HBM_GBS,
-            "traffic": runs_bytes}
+            "traffic": tr(kname)}
 
     result = {
         "metric": METRIC,

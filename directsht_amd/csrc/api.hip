@@ -665,7 +665,10 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   }
   // the scatter engine: of the shared points for weight sets, the matrix
   // engine for grouped point sets, per set otherwise (set again below)
-  pl.scatter_now = pick_scatter(pl, a.shared ? a.Np[0] : maxNp);
+  pl.scatter_now = a.shared ? pick_scatter(pl, a.Np[0], a.nsets >= kSetsScatterMin
+                                                            ? kPickWeightSets
+                                                            : kPickPointSet)
+                           : pick_scatter(pl, maxNp);
   const bool sets_scatter = a.shared && uses_sets_scatter(pl, a.nsets);
   const bool many = !a.shared && uses_many(pl, a.nsets, mode);
   if (many) pl.scatter_now = DSHT_SCATTER_MATRIX;
@@ -818,7 +821,7 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   } else {
     for (int set = 0; set < a.nsets; ++set) {
       const int64_t Np = a.Np[set];
-      pl.scatter_now = pick_scatter(pl, Np);
+      pl.scatter_now = pick_scatter(pl, Np, mfma ? kPickPointSet : kPickPlain);
       pl.chunk = work_chunk(chunk_cap(pl), Np);
       ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
       if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
@@ -883,7 +886,7 @@ int dsht_plan_engine(const dsht_plan* p, int64_t Np, int* engine) {
     *engine = DSHT_SCATTER_VECTOR;
     return DSHT_OK;
   }
-  int e = pick_scatter(pl, Np);
+  int e = pick_scatter(pl, Np, kPickPlain);
   if (e == DSHT_SCATTER_SPREAD && pl.Nl > kSpreadMaxNl) e = DSHT_SCATTER_MATRIX;
   *engine = e;
   return DSHT_OK;

@@ -242,16 +242,37 @@ inline bool uses_spread(const Plan& pl) {
   return pl.precision == DSHT_FP32 && pl.scatter_now == DSHT_SCATTER_SPREAD &&
          pl.Nl <= kSpreadMaxNl;
 }
+// The node variant of the spreading scatter (scatter and fold in one
+// launch, one workgroup per node pair) serves grids of at least
+// kNodeMinPairs node pairs (fewer leave the device short of work) and at
+// most 4096 cells (Nl <= 1024).
+constexpr int kNodeMinPairs = 512;
+inline bool node_spread_fits(const Plan& pl) {
+  return pl.Npair >= kNodeMinPairs && pl.spread_logn <= 12;
+}
 // DSHT_SCATTER_AUTO: the spreading scatter costs one transform of four
-// grids per run, the matrix engine work proportional to points x orders, so
-// the spreading scatter is taken where a transform has at least
-// kSpreadMinPerSeg points per segment on average (measured break-even:
-// 170-430 points per segment for Nl 128-1024, uniform points).
-constexpr int kSpreadMinPerSeg = 400;
-inline int pick_scatter(const Plan& pl, int64_t Np) {
+// grids per run (per node and branch in the node variant) and a fixed
+// number of grid updates per point, the matrix engine work proportional to
+// points x orders, so the spreading scatter is taken where a transform has
+// at least this many points per segment on average.  Measured break-even
+// (whole transforms, uniform points, Nx = 2 Nl): the node variant 25-50
+// points per segment for Nl 512-1024, the run-based one (and the fold
+// after it) about 250 at Nl 256.  Weight sets on shared points are
+// scattered together on the matrix engine (its operands built once for all
+// sets) but spread one by one: 0.14 against ~0.25 ms a set at 2e6 points,
+// Nl = 768 (326 points per segment), 0.78 against 0.44 ms at 1e7 points,
+// Nl = 1000 (1221): break-even near 600.
+constexpr int kSpreadMinPerSeg = 256;
+constexpr int kNodeSpreadMinPerSeg = 64;
+constexpr int kSetsSpreadMinPerSeg = 640;
+enum PickFor { kPickPlain, kPickPointSet, kPickWeightSets };
+inline int pick_scatter(const Plan& pl, int64_t Np, PickFor use = kPickPointSet) {
   if (pl.scatter != DSHT_SCATTER_AUTO) return pl.scatter;
-  return pl.Nl <= kSpreadMaxNl && Np >= (int64_t)kSpreadMinPerSeg * pl.nseg ? DSHT_SCATTER_SPREAD
-                                                                            : DSHT_SCATTER_MATRIX;
+  const int per = use == kPickWeightSets                        ? kSetsSpreadMinPerSeg
+                  : use == kPickPlain && node_spread_fits(pl) ? kNodeSpreadMinPerSeg
+                                                                : kSpreadMinPerSeg;
+  return pl.Nl <= kSpreadMaxNl && Np >= (int64_t)per * pl.nseg ? DSHT_SCATTER_SPREAD
+                                                               : DSHT_SCATTER_MATRIX;
 }
 // Work items of the spreading scatter hold up to kSpreadChunk points unless a
 // chunk is configured: each run (a stretch of one segment inside an item)
@@ -291,16 +312,11 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
   return pl.precision == DSHT_FP32 && !pl.caller_tables && nsets >= kMfmaMinSets &&
          pl.contract == DSHT_CONTRACT_MATRIX;
 }
-// The node variant of the spreading scatter (scatter and fold in one
-// launch, one workgroup per node pair): transforms that write the plain
-// coefficient layout (no matrix-core contraction after them) on grids of at
-// least kNodeMinPairs node pairs (fewer leave the device short of work) and
-// at most 4096 cells (Nl <= 1024).  rhs: the plain layout [m][P][np][24];
-// w, wmode as launch_scatter_spread.
-constexpr int kNodeMinPairs = 512;
+// The node variant for transforms that write the plain coefficient layout
+// (no matrix-core contraction after them).  rhs: the plain layout
+// [m][P][np][24]; w, wmode as launch_scatter_spread.
 inline bool uses_node_spread(const Plan& pl, int nsets) {
-  return uses_spread(pl) && !use_mfma_contract(pl, nsets) && pl.Npair >= kNodeMinPairs &&
-         pl.spread_logn <= 12;
+  return uses_spread(pl) && !use_mfma_contract(pl, nsets) && node_spread_fits(pl);
 }
 int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
 // the spreading kernel's values at the nq points z (host arrays), as the

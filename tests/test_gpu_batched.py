@@ -171,7 +171,7 @@ def test_many_point_sets_in_groups(torch_cuda):
 
 
 def test_many_point_sets_dense_and_sparse(torch_cuda):
-    """A point-set batch on the automatic engine: the sets with at least 400
+    """A point-set batch on the automatic engine: the sets with at least 256
     points per segment go one by one through the spreading scatter, the
     others in groups of consecutive sparse sets through the matrix engine,
     all into one matrix-core contraction.  Dense and sparse sets interleaved
@@ -181,7 +181,7 @@ def test_many_point_sets_dense_and_sparse(torch_cuda):
     from directsht_amd.sht import DirectSHT
     from oracle import refalg
     Nl, Nx = 48, 96
-    dense_n = 400 * 4 * (Nx - 1)
+    dense_n = 256 * 4 * (Nx - 1)
     sizes = [dense_n + 5000, 700, 20000, 0, 3 * dense_n, 150, dense_n, 45000, 9000, 2 * dense_n - 1]
     sets = []
     for i, n in enumerate(sizes):

@@ -275,19 +275,20 @@ def test_high_l_second_m_tile(torch_cuda, Nl):
 
 def test_auto_engine_and_repeat(torch_cuda):
     """The default ('auto') engine takes the spreading scatter once a transform
-    averages 400 points per segment (4 (Nx-1) segments) and the matrix engine
-    below; both against the oracle at <= 1e-6, and the spreading path (64-bit
-    fixed-point grids) gives the same bits on every call."""
+    averages 256 points per segment (4 (Nx-1) segments) on grids too small for
+    its node variant, and the matrix engine below; both against the oracle at
+    <= 1e-6, and the spreading path (64-bit fixed-point grids) gives the same
+    bits on every call."""
     torch = torch_cuda
     from directsht_amd.sht import DirectSHT
     from oracle import refalg
     Nl, Nx = 64, 128
     nseg = 4 * (Nx - 1)
     s = DirectSHT(Nl, Nx, dflt_type="float32")
-    assert s.scatter_engine(400 * nseg - 1) == "matrix"
-    assert s.scatter_engine(400 * nseg) == "spread"
+    assert s.scatter_engine(256 * nseg - 1) == "matrix"
+    assert s.scatter_engine(256 * nseg) == "spread"
     tabs = oracle_tables(Nl, Nx)
-    for n in (50000, 400 * nseg + 1000):
+    for n in (50000, 256 * nseg + 1000):
         rng = np.random.default_rng(n)
         t, p, w = _uniform(rng, n)
         ref = refalg.direct_sht(tabs, t, p, w)
@@ -298,6 +299,35 @@ def test_auto_engine_and_repeat(torch_cuda):
         print(f"n={n} ({s.scatter_engine(n)}): max {e:.2e}")
         assert e < 1e-6 and rel_rms(a1, ref) < 1e-6
         assert np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("Nx", [1024, 1023])
+def test_node_spread_variant(torch_cuda, Nx):
+    """The node variant of the spreading scatter (grids of 512 or more node
+    pairs, Nl <= 1024: scatter and fold in one kernel) is taken from 64 points
+    per segment; against the FP64 path at <= 1e-6 and the same bits on a
+    repeat, with an odd Nx (the last node of the last pair is padding), points
+    on both branches and both hemispheres, and points exactly at the grid's
+    first and last nodes."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    Nl = 512
+    nseg = 4 * (Nx - 1)
+    s = DirectSHT(Nl, Nx, dflt_type="float32")
+    assert s.scatter_engine(64 * nseg - 1) == "matrix"
+    assert s.scatter_engine(64 * nseg) == "spread"
+    rng = np.random.default_rng(Nx)
+    t, p, w = _uniform(rng, 64 * nseg + 5000)
+    xg = np.arange(Nx) * 0.75 / (Nx - 1)
+    t[:4] = np.arccos([xg[0], -xg[0], xg[-1], -xg[-1]])  # at the first and last nodes
+    td, pd, wd = (torch.from_numpy(a).cuda() for a in (t, p, w))
+    ref = DirectSHT(Nl, Nx, dflt_type="float64")(td, pd, wd).cpu().numpy()
+    a1 = s(td, pd, wd).cpu().numpy()
+    a2 = s(td, pd, wd).cpu().numpy()
+    e, r = rel_max(a1, ref), rel_rms(a1, ref)
+    print(f"node variant Nx={Nx}: max {e:.2e} rms {r:.2e}")
+    assert e < 1e-6 and r < 1e-6
+    assert np.array_equal(a1, a2)
 
 
 @pytest.mark.parametrize("n", [1000, 100000])
