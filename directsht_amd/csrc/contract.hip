@@ -87,15 +87,15 @@ __global__ __launch_bounds__(64) void k_contract_y(const typename Couple<T>::typ
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
   const T4* t = tab + gd.off / 2 + (valid ? r : 0);  // gd.off is even (Npair is)
-  // [c | f | g][er, ei, pr, pi][node] of node pair np at q[24 np]
-  const T* q = rhs + (int64_t)(gd.m * 2 + gd.P) * Npair * 24;
+  // [c | f | g][er, ei, pr, pi][node] of node pair np at q[np * 2 Nl * 24]
+  const T* q = rhs + (int64_t)(gd.m * 2 + gd.P) * 24;
   const int64_t stride = gd.nrows;
   T4 ring[kRing];
 #pragma unroll
   for (int k = 0; k < kRing; ++k) ring[k] = stream_load(t + (int64_t)min(lo + k, hi - 1) * stride);
   double S[3][4] = {};
   auto coef = [&](int np, T (&c)[24]) {
-    const T* qq = q + (int64_t)np * 24;
+    const T* qq = q + (int64_t)np * (2 * Nl * 24);
 #pragma unroll
     for (int i = 0; i < 24; ++i) c[i] = qq[i];
   };
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kOtfWaves * 64) void k_contract_otf(
       for (int q = 0; q < 32; ++q) {
         const int n = rg * kOtfNodes + w * 128 + 4 * q + kk;
         const int np = min(n >> 1, Npair - 1);  // past the nodes: Y = 0 there
-        bop[p][q] = rhs[((int64_t)(2 * m + p) * Npair + np) * 24 + cofs + (n & 1)];
+        bop[p][q] = rhs[(((int64_t)np * Nl + m) * 2 + p) * 24 + cofs + (n & 1)];
       }
     float* tile = &stage[w][0][0];
     // rows t0 .. t0+3 of step s into the wavefront's tiles

@@ -31,7 +31,9 @@
 // with f_P(n) = x_n d_P(n)/(1 - x_n^2) and g(n) = d_P(n)/(1 - x_n^2).  Row l-1
 // has the other parity, so the rows of parity P are contracted with c_P, f_P
 // and g = d_{1-P}/(1 - x^2) (for the row above them).  That fold writes one
-// slot rhs[m][P][np][24]: [c | f | g][er, ei, pr, pi][node 2np, 2np+1].
+// slot rhs[np][m][P][24]: [c | f | g][er, ei, pr, pi][node 2np, 2np+1] (node
+// pair major: a pair's records, all orders and both parities, are one block,
+// which the spreading scatter's node variant writes from one workgroup).
 #include <type_traits>
 
 #include "common.h"
@@ -179,7 +181,7 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   if (live) {
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
-      T* o = kY ? rhs + (((int64_t)m * 2 + P) * Npair + np) * 24
+      T* o = kY ? rhs + (((int64_t)np * Nl + m) * 2 + P) * 24
                 : rhs + ((((int64_t)m * 2 + P) * Npair + np) * nsets + set) * 16;
 #pragma unroll
       for (int q = 0; q < kOut; q += 2)

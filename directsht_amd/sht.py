@@ -264,10 +264,10 @@ class DirectSHT:
         kinds c, f, g (see dsht_plan_debug_rhs in include/dsht.h)."""
         npair = 2 * ((self.Nx + 3) // 4)
         dt = np.float32 if self._precision == _lib.DSHT_FP32 else np.float64
-        out = np.empty((self.Nell, 2, npair, 24), dtype=dt)
-        _lib.check(_lib.load().dsht_plan_debug_rhs(self._plan, out.ctypes.data, out.nbytes),
+        raw = np.empty((npair, self.Nell, 2, 24), dtype=dt)  # the device layout
+        _lib.check(_lib.load().dsht_plan_debug_rhs(self._plan, raw.ctypes.data, raw.nbytes),
                    "dsht_plan_debug_rhs")
-        return out
+        return np.ascontiguousarray(raw.transpose(1, 2, 0, 3))
 
     # ---- transforms -----------------------------------------------------
     def transform_device(self, theta, phi, wt, out=None, mode=_lib.DSHT_MODE_CALL, stream=None):
