@@ -67,6 +67,15 @@ __host__ __device__ inline int64_t hp_index(int ell, int m, int Nl) {
   return ((int64_t)m * (2 * Nl - 1 - m)) / 2 + ell;
 }
 
+// The plain node coefficients (one set at a time): for node pair np, branch
+// br (0 equatorial, 1 polar), order m and parity P, 12 values
+// [c | f | g][re | im][node 2np, 2np + 1] at plain_rec(np, br, m, P, Nl).  A
+// branch's records of one pair (all orders, both parities) are one block, so
+// the kernels that produce them write whole lines.
+__host__ __device__ inline int64_t plain_rec(int np, int br, int m, int P, int Nl) {
+  return ((((int64_t)np * 2 + br) * Nl + m) * 2 + P) * 12;
+}
+
 // ---------------------------------------------------------------------------
 // Segments of the sorted point array
 // ---------------------------------------------------------------------------

@@ -87,17 +87,20 @@ __global__ __launch_bounds__(64) void k_contract_y(const typename Couple<T>::typ
   const int r = threadIdx.x;
   const bool valid = r < gd.nrows;
   const T4* t = tab + gd.off / 2 + (valid ? r : 0);  // gd.off is even (Npair is)
-  // [c | f | g][er, ei, pr, pi][node] of node pair np at q[np * 2 Nl * 24]
-  const T* q = rhs + (int64_t)(gd.m * 2 + gd.P) * 24;
+  // c[8 kind + 2 (2 br + ri) + node]: [c | f | g][er, ei, pr, pi][node] of
+  // node pair np, from its two branch records
   const int64_t stride = gd.nrows;
   T4 ring[kRing];
 #pragma unroll
   for (int k = 0; k < kRing; ++k) ring[k] = stream_load(t + (int64_t)min(lo + k, hi - 1) * stride);
   double S[3][4] = {};
   auto coef = [&](int np, T (&c)[24]) {
-    const T* qq = q + (int64_t)np * (2 * Nl * 24);
 #pragma unroll
-    for (int i = 0; i < 24; ++i) c[i] = qq[i];
+    for (int br = 0; br < 2; ++br) {
+      const T* qq = rhs + plain_rec(np, br, gd.m, gd.P, Nl);
+#pragma unroll
+      for (int i = 0; i < 12; ++i) c[8 * (i >> 2) + 4 * br + (i & 3)] = qq[i];
+    }
   };
   auto mul = [&](T (&f)[3][4], T y0, T y1, const T (&c)[24]) {
 #pragma unroll
@@ -204,7 +207,10 @@ __global__ __launch_bounds__(kOtfWaves * 64) void k_contract_otf(
   const double2* __restrict__ rcm = rc + otf_offset(m, Nl);
   const int nranges = (2 * Npair + kOtfNodes - 1) / kOtfNodes;
   const int kk = lane >> 4, col = lane & 15;
-  const int cofs = col < 12 ? 8 * (col >> 2) + 2 * (col & 3) : 0;
+  // column col = 4 kind + 2 br + ri of the B operand: branch br's record,
+  // value 4 kind + 2 ri + node (columns 12..15 load a valid, unused value)
+  const int cbr = col < 12 ? (col & 3) >> 1 : 0;
+  const int cofs = col < 12 ? 4 * (col >> 2) + 2 * (col & 1) : 0;
   // where this lane writes node 64 c + lane (k, quad) and reads (row i, k)
   const int wk = lane & 3, wq = lane >> 2;
   const int ri = lane & 15, rk = lane >> 4;
@@ -230,7 +236,7 @@ __global__ __launch_bounds__(kOtfWaves * 64) void k_contract_otf(
       for (int q = 0; q < 32; ++q) {
         const int n = rg * kOtfNodes + w * 128 + 4 * q + kk;
         const int np = min(n >> 1, Npair - 1);  // past the nodes: Y = 0 there
-        bop[p][q] = rhs[(((int64_t)np * Nl + m) * 2 + p) * 24 + cofs + (n & 1)];
+        bop[p][q] = rhs[plain_rec(np, cbr, m, p, Nl) + cofs + (n & 1)];
       }
     float* tile = &stage[w][0][0];
     // rows t0 .. t0+3 of step s into the wavefront's tiles

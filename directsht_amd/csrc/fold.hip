@@ -30,10 +30,10 @@
 // so that  sum_n Y'_l(x_n) d_P(n) = G_l sum_n Y_{l-1}(x_n) g(n) - l sum_n Y_l(x_n) f_P(n)
 // with f_P(n) = x_n d_P(n)/(1 - x_n^2) and g(n) = d_P(n)/(1 - x_n^2).  Row l-1
 // has the other parity, so the rows of parity P are contracted with c_P, f_P
-// and g = d_{1-P}/(1 - x^2) (for the row above them).  That fold writes one
-// slot rhs[np][m][P][24]: [c | f | g][er, ei, pr, pi][node 2np, 2np+1] (node
-// pair major: a pair's records, all orders and both parities, are one block,
-// which the spreading scatter's node variant writes from one workgroup).
+// and g = d_{1-P}/(1 - x^2) (for the row above them).  That fold writes the
+// plain records: per (node pair, branch, m, P) [c | f | g][re, im][node 2np,
+// 2np+1] at plain_rec (a pair's branch is one block, which the spreading
+// scatter's node variant writes from one workgroup).
 #include <type_traits>
 
 #include "common.h"
@@ -181,11 +181,24 @@ __global__ __launch_bounds__(kFoldM * kFoldNp) void k_fold(const TP* __restrict_
   if (live) {
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
-      T* o = kY ? rhs + (((int64_t)np * Nl + m) * 2 + P) * 24
-                : rhs + ((((int64_t)m * 2 + P) * Npair + np) * nsets + set) * 16;
+      if constexpr (kY) {
+        // out[P][8 kind + 2 (2 br + ri) + node] -> branch br's record
 #pragma unroll
-      for (int q = 0; q < kOut; q += 2)
-        *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
+        for (int br = 0; br < 2; ++br) {
+          T* o = rhs + plain_rec(np, br, m, P, Nl);
+#pragma unroll
+          for (int q = 0; q < 12; q += 2) {
+            const int kind = q >> 2, ri = (q >> 1) & 1;
+            const int src = 8 * kind + 2 * (2 * br + ri);
+            *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][src], out[P][src + 1]};
+          }
+        }
+      } else {
+        T* o = rhs + ((((int64_t)m * 2 + P) * Npair + np) * nsets + set) * 16;
+#pragma unroll
+        for (int q = 0; q < kOut; q += 2)
+          *reinterpret_cast<typename Vec2<T>::type*>(o + q) = {out[P][q], out[P][q + 1]};
+      }
     }
   }
   // per-set magnitude bound for the matrix-core contraction's scaling (only

@@ -82,7 +82,7 @@ struct Workspace {
   int2* segrank = nullptr;    // [2][cap_points] ints: each input point's segment, then its
                               // rank among its tile's points of that segment (input order)
   void* rhs = nullptr;        // node coefficients (float or double): plain contraction
-                              // [Npair][Nl][2][24] (one set at a time); matrix-core
+                              // plain_rec (one set at a time); matrix-core
                               // contraction and caller tables [Nl][2][Npair][cap_sets][16]
   int64_t rhs_elems = 0;      // capacity of rhs in elements
   unsigned* rmax = nullptr;   // [cap_sets]: max(|c|, |d/dx|) per set (float bits)
@@ -314,7 +314,7 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
 }
 // The node variant for transforms that write the plain coefficient layout
 // (no matrix-core contraction after them).  rhs: the plain layout
-// [np][m][P][24]; w, wmode as launch_scatter_spread.
+// records (plain_rec); w, wmode as launch_scatter_spread.
 inline bool uses_node_spread(const Plan& pl, int nsets) {
   return uses_spread(pl) && !use_mfma_contract(pl, nsets) && node_spread_fits(pl);
 }

@@ -337,9 +337,9 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 // thread of mode m keeps the first node's c^h, d^h at m in registers; at the
 // second it forms the parity sums c_P = c^0 +- c^1, d_P (times dx) and the
 // derivative coefficients f_P = x d_P / (1 - x^2), g = d_{1-P} / (1 - x^2)
-// of the pair's two nodes and writes the branch's half of rhs[np][m][P][24]
-// (the fold's layout: [c | f | g][equ re, equ im, pol re, pol im][node]; the
-// pair's records are one contiguous block, whole lines from one workgroup).
+// of the pair's two nodes and writes the branch's records (the fold's
+// layout, plain_rec: [c | f | g][re, im][node], one contiguous block per
+// pair and branch, whole lines from one workgroup).
 // (One workgroup per node pair and branch measured slower.)  Grids up to
 // n = 4096 (all four in LDS at once, one mode per thread).
 template <int LOGN>
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
 
   // the records of branch br at mode m from the pair's spectra
   // a[odd][h][c re, c im, d re, d im], as the fold writes them: per P,
-  // [c | f | g] x (re, im) x (node 0, 1), three 16-byte pieces of 96 bytes
+  // [c | f | g] x (re, im) x (node 0, 1), 48 contiguous bytes
   auto emit = [&](int br, const float (&a0)[2][4], const float (&a1)[2][4]) {
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
@@ -402,10 +402,10 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
           f4[2 * ri + odd] = (float)(xs[odd] * inv[odd] * dv);
           g4[2 * ri + odd] = (float)(inv[odd] * du);
         }
-      float* o = rhs + (((int64_t)np * Nl + m) * 2 + P) * 24 + 4 * br;
+      float* o = rhs + plain_rec(np, br, m, P, Nl);
       *reinterpret_cast<float4*>(o) = make_float4(c4[0], c4[1], c4[2], c4[3]);
-      *reinterpret_cast<float4*>(o + 8) = make_float4(f4[0], f4[1], f4[2], f4[3]);
-      *reinterpret_cast<float4*>(o + 16) = make_float4(g4[0], g4[1], g4[2], g4[3]);
+      *reinterpret_cast<float4*>(o + 4) = make_float4(f4[0], f4[1], f4[2], f4[3]);
+      *reinterpret_cast<float4*>(o + 8) = make_float4(g4[0], g4[1], g4[2], g4[3]);
     }
   };
 

@@ -264,10 +264,13 @@ class DirectSHT:
         kinds c, f, g (see dsht_plan_debug_rhs in include/dsht.h)."""
         npair = 2 * ((self.Nx + 3) // 4)
         dt = np.float32 if self._precision == _lib.DSHT_FP32 else np.float64
-        raw = np.empty((npair, self.Nell, 2, 24), dtype=dt)  # the device layout
+        # the device layout [pair][branch][m][P][kind][re|im][node]
+        raw = np.empty((npair, 2, self.Nell, 2, 3, 2, 2), dtype=dt)
         _lib.check(_lib.load().dsht_plan_debug_rhs(self._plan, raw.ctypes.data, raw.nbytes),
                    "dsht_plan_debug_rhs")
-        return np.ascontiguousarray(raw.transpose(1, 2, 0, 3))
+        # -> [m][P][pair][kind][branch][re|im][node]
+        return np.ascontiguousarray(raw.transpose(2, 3, 0, 4, 1, 5, 6)).reshape(
+            self.Nell, 2, npair, 24)
 
     # ---- transforms -----------------------------------------------------
     def transform_device(self, theta, phi, wt, out=None, mode=_lib.DSHT_MODE_CALL, stream=None):

@@ -251,10 +251,11 @@ int dsht_plan_get_timing(dsht_plan* plan, double stage_ms[DSHT_NSTAGES], int64_t
  * transform (the right-hand side of the contraction, after the scatter and
  * fold) to `out`: Nl * 2 * Npair * 24 values, Npair = 2 ceil(Nx/4) node pairs
  * (nodes past Nx are zero padding), float for DSHT_FP32 plans,
- * double for DSHT_FP64, layout [node pair][m][parity P][kind*8 + q*2 + node]
- * with kind c (the Y coefficients c_P), f (x d_P / (1 - x^2)) and
- * g (d_{1-P} / (1 - x^2)), d the Y' coefficients, q = equatorial re, im,
- * polar re, im, node = even, odd node of the pair.  Waits for the device. */
+ * double for DSHT_FP64, layout [node pair][branch][m][parity P]
+ * [kind*4 + part*2 + node] with branch 0 equatorial, 1 polar, kind c (the Y
+ * coefficients c_P), f (x d_P / (1 - x^2)) and g (d_{1-P} / (1 - x^2)), d the
+ * Y' coefficients, part re, im, node = even, odd node of the pair.  Waits
+ * for the device. */
 int dsht_plan_debug_rhs(dsht_plan* plan, void* out, int64_t bytes);
 
 /* Angular power spectra of nsets alm arrays on the device (healpy order,
