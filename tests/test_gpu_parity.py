@@ -330,6 +330,40 @@ def test_node_spread_variant(torch_cuda, Nx):
     assert np.array_equal(a1, a2)
 
 
+def test_node_spread_variant_inputs(torch_cuda):
+    """The node variant on the inputs the small-grid tests give the other
+    engines: polar-only, equatorial-only and one-hemisphere point sets (units
+    with no points skip their transform), NaN points (dropped), weights far
+    from 1 and of both signs (the fixed-point scale follows max |w|), each
+    against the FP64 path at <= 1e-6."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    Nl, Nx = 512, 1024
+    n = 64 * 4 * (Nx - 1) + 2000
+    s32 = DirectSHT(Nl, Nx, dflt_type="float32")
+    s64 = DirectSHT(Nl, Nx, dflt_type="float64")
+    assert s32.scatter_engine(n) == "spread"
+    rng = np.random.default_rng(77)
+    cases = []
+    for lo, hi in ((0.8, 1.0), (-1.0, -0.76), (-0.7, 0.7), (0.0, 0.74), (-0.74, -0.01)):
+        cases.append(((lo, hi), _uniform(rng, n, lo, hi)))
+    t, p, w = _uniform(rng, n)
+    t[::97] = np.nan
+    cases.append(("nan", (t, p, w)))
+    t, p, w = _uniform(rng, n)
+    for scale in (1e-6, 1e12):
+        cases.append((f"w x {scale:g}", (t, p, w * scale)))
+    ws = w * np.exp(rng.uniform(np.log(1e-6), np.log(1e6), n)) * rng.choice([-1.0, 1.0], n)
+    cases.append(("mixed weights", (t, p, ws)))
+    for name, (t, p, w) in cases:
+        td, pd, wd = (torch.from_numpy(a).cuda() for a in (t, p, w))
+        ref = s64(td, pd, wd).cpu().numpy()
+        a = s32(td, pd, wd).cpu().numpy()
+        e = rel_max(a, ref)
+        print(f"node variant, {name}: max {e:.2e}")
+        assert e < 1e-6, name
+
+
 @pytest.mark.parametrize("n", [1000, 100000])
 def test_single_ring_accumulation(torch_cuda, n):
     """All points in one spline bin (a ring of constant theta): the longest
