@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "points·modes/s and alm/s at Nl=1000,1e7 pts; batched vs plain; p50/p99 latency"
 PEAK_FP32_TFLOPS = 157.3   # MI355X published dense FP32 (vector and matrix)
 PEAK_FP16_TFLOPS = 2500.0  # MI355X published dense FP16 matrix
+PEAK_FP64_TFLOPS = 78.6    # MI355X published dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 # HBM bytes per launch of each kernel at the default workload, measured by
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel of
@@ -370,6 +371,16 @@ def bench_downstream(reps=20):
         nbytes = S * Nlm * 16
         out[name] = {"ms": ms, "alm_bytes": nbytes, "hbm_GBs": nbytes / ms / 1e6,
                      "hbm_frac": nbytes / ms / 1e6 / PEAK_HBM_GBS}
+        if cross:
+            # the Gram products on the FP64 matrix cores: 2 flop per (entry of
+            # a 16 x 16 tile pair i <= j, l, term), 2 (l + 1) terms (m, re|im)
+            # per l: 2 npairs 256 Nl (Nl + 1)
+            nt = (S + 15) // 16
+            mf = 2.0 * (nt * (nt + 1) // 2) * 256 * Nl * (Nl + 1)
+            out[name].update({"fp64_mfma_tflops": mf / ms / 1e9,
+                              "fp64_mfma_frac": mf / ms / 1e9 / PEAK_FP64_TFLOPS,
+                              "bound": "max(HBM read, FP64 matrix products): both about 0.1 ms "
+                                       "at peak for 64 sets at Nl=1000"})
         log(f"downstream {name}: {out[name]}")
     # CPU: the reference's alm2cl (sht/sht.py:95-109, restated in numpy by the
     # oracle) on 8 of the 64 sets, scaled to 64
@@ -398,7 +409,7 @@ def bench_downstream(reps=20):
         flop = 5.0 * terms
         out[f"mode_coupling_Nl{nl}"] = {"ms": ms, "terms": terms,
                                         "fp64_TFLOPs": flop / ms / 1e9,
-                                        "fp64_frac": flop / ms / 1e9 / 78.6}
+                                        "fp64_frac": flop / ms / 1e9 / PEAK_FP64_TFLOPS}
         log(f"downstream mode_coupling Nl={nl}: {out[f'mode_coupling_Nl{nl}']}")
         del M
     N = 1127
