@@ -433,6 +433,13 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
       }
       float a[2][4] = {};
       if (any) {  // uniform; no points at this node: zero spectra, no transform
+        // a grid here takes all the node's points of one hemisphere (the
+        // run-based kernel at most a work item's): 2^-sh more fixed-point
+        // scale for more than 2^14 of them keeps every cell sum below 2^54
+        const int cnt = max(hi[0] - lo[0], hi[1] - lo[1]);
+        const int sh = max(0, 32 - __builtin_clz((unsigned)cnt) - 14);
+        const double wf = ldexp(w_fix, -sh);
+        const float wo = ldexpf(w_out, sh);
         for (int i = tid; i < G * n / 2; i += kT)
           reinterpret_cast<ulonglong2*>(sgrid)[i] = make_ulonglong2(0ull, 0ull);
         __syncthreads();
@@ -443,7 +450,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
           for (int q = lo[h] + tid; q < hi[h]; q += kT) {
             const float4 ah = reinterpret_cast<const float4*>(rec + q)[0];
             const int4 ri = reinterpret_cast<const int4*>(rec + q)[1];  // seg, idx, j0, z0
-            const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri.y])) * w_fix;
+            const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri.y])) * wf;
             const bool upper = q >= mid[h];  // bin node: weights a0, a1; bin node - 1: a2, a3
             const double ac = wt * (double)(upper ? ah.x : ah.z);
             const double ad = wt * (double)(upper ? ah.y : ah.w);
@@ -487,7 +494,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
         // complex grid h = c^h + i d^h: split and deconvolve
         if (m < Nl) {
           const int pa = fpad(spec_pos<LOGN>(m)), pb = fpad(spec_pos<LOGN>((n - m) & (n - 1)));
-          const float s = 0.5f * ipsi[m] * w_out;
+          const float s = 0.5f * ipsi[m] * wo;
 #pragma unroll
           for (int h = 0; h < NCG; ++h) {
             const float2 z = sz[h * Gm::kPad + pa], v = sz[h * Gm::kPad + pb];

@@ -355,6 +355,11 @@ def test_node_spread_variant_inputs(torch_cuda):
         cases.append((f"w x {scale:g}", (t, p, w * scale)))
     ws = w * np.exp(rng.uniform(np.log(1e-6), np.log(1e6), n)) * rng.choice([-1.0, 1.0], n)
     cases.append(("mixed weights", (t, p, ws)))
+    # 250000 points at one position: every point of a node on the same 8 grid
+    # cells (the per-node fixed-point scale keeps the cell sums in range)
+    t, p, w = _uniform(rng, n)
+    t[:250000], p[:250000] = 1.1, 2.0
+    cases.append(("coincident points", (t, p, w)))
     for name, (t, p, w) in cases:
         td, pd, wd = (torch.from_numpy(a).cuda() for a in (t, p, w))
         ref = s64(td, pd, wd).cpu().numpy()
