@@ -16,8 +16,17 @@
 // ~2e-8 relative error, under the FP32 rounding of the transform.  The work
 // per point is 4 W grid updates instead of 4 Nl products.
 //
-// One workgroup per chunk of sorted points; the chunk's runs (stretches of
-// one segment) are processed one after another, each in LDS:
+// Two kernels: k_spread_nodes (single transforms on grids of up to 4096
+// cells with 512 or more node pairs) spreads per node and includes the
+// node/parity fold (below, before its launcher); k_scatter_spread (the
+// rest) spreads per run and leaves the fold to fold.hip.  Common to both:
+//   * the kernel values 2^(beta log2(e) (sqrt(1 - z^2) - 1)) come from the
+//     hardware sqrt and exp2 (es_kernel), and 1/psihat is the transform of
+//     that very function (spread_kernel_values at plan creation).
+//
+// k_scatter_spread: one workgroup per chunk of sorted points; the chunk's
+// runs (stretches of one segment) are processed one after another, each in
+// LDS:
 //   * the grids are accumulated in 64-bit fixed point (units of 2^-40 of the
 //     weight scale) with LDS integer atomics.  Integer addition is
 //     associative, so the grid -- and everything after it -- is the same bit

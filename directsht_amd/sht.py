@@ -215,14 +215,16 @@ class DirectSHT:
     # ---- tuning -----------------------------------------------------------
     def configure(self, scatter=None, chunk=None, contract=None):
         """Select the scatter engine of FP32 plans ('auto', the default: per
-        transform, 'spread' where the points average at least 400 per
-        segment of the grid and Nl <= 2048, else 'matrix'; 'matrix': fp16x3
-        MFMA direct sums, 'vector': FP32 VALU direct sums, 'spread': per-run
-        non-uniform FFT, ~2e-8 relative), the sorted points per scatter work
-        item (the largest; small transforms use smaller items), and the
-        table contraction of batched FP32 transforms ('matrix': fp16x3 MFMA
-        GEMM over all sets, 'vector': FP32 GEMV per set).  Results do not
-        depend on these beyond rounding (and the spreading kernel's ~2e-8)."""
+        transform, 'spread' where Nl <= 2048 and the points average at least
+        64 (single transforms on grids of 512 or more node pairs, Nl <= 1024),
+        256 (other transforms) or 640 (weight-set batches) per segment of
+        the grid, else 'matrix'; 'matrix': fp16x3 MFMA direct sums,
+        'vector': FP32 VALU direct sums, 'spread': non-uniform FFTs,
+        ~1e-7 relative), the sorted points per scatter work item (the
+        largest; small transforms use smaller items), and the table
+        contraction of batched FP32 transforms ('matrix': fp16x3 MFMA GEMM
+        over all sets, 'vector': FP32 GEMV per set).  Results do not depend
+        on these beyond rounding (and the spreading kernel's ~1e-7)."""
         lib = _lib.load()
         if contract is not None:
             v = {"matrix": _lib.DSHT_CONTRACT_MATRIX, "vector": _lib.DSHT_CONTRACT_VECTOR}[contract]

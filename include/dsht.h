@@ -201,15 +201,20 @@ int dsht_transform_many(dsht_plan* plan, int nsets, const int64_t* Np, const dou
  * rounding.
  *   DSHT_OPT_SCATTER  scatter engine of DSHT_FP32 plans:
  *                     DSHT_SCATTER_AUTO (default) picks per transform: the
- *                     spreading engine where the points average at least 400
- *                     per segment of the (Nx - 1)-bin grid (4 (Nx - 1)
- *                     segments) and Nl <= 2048, else the matrix engine;
+ *                     spreading engine where Nl <= 2048 and the points
+ *                     average at least k per segment of the (Nx - 1)-bin grid
+ *                     (4 (Nx - 1) segments): k = 64 for a single transform
+ *                     with 512 or more node pairs and Nl <= 1024, 256 for
+ *                     other transforms and the dense sets of a
+ *                     dsht_transform_many batch, 640 for dsht_transform_sets
+ *                     batches; else the matrix engine;
  *                     DSHT_SCATTER_MATRIX fp16x3 split-precision MFMA GEMM;
  *                     DSHT_SCATTER_VECTOR FP32 VALU recurrence;
- *                     DSHT_SCATTER_SPREAD: each run's sums as a type-1
- *                     non-uniform FFT (kernel spreading onto a 4x oversampled
- *                     phi grid in 64-bit fixed point, FFT, deconvolution;
- *                     ~2e-8 relative; Nl <= 2048, the matrix engine above).
+ *                     DSHT_SCATTER_SPREAD: the sums as type-1 non-uniform
+ *                     FFTs (kernel spreading onto a 4x oversampled phi grid
+ *                     in 64-bit fixed point, FFT, deconvolution; per node
+ *                     with the fold included, or per run; ~1e-7 relative;
+ *                     Nl <= 2048, the matrix engine above).
  *                     DSHT_FP64 plans always use the FP64 vector scatter.
  *   DSHT_OPT_CHUNK    sorted points per scatter work item (64..65536; default
  *                     2048, and 8192 for the spreading engine).  A transform of fewer than 1024 x chunk points
