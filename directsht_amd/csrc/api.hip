@@ -148,12 +148,15 @@ static int64_t rhs_per_pair(const Plan& pl, int nsets) {
 // Lay out every workspace array of a transform with capacities `want`
 // through alloc(ptr, bytes) (device allocation, or carving from a fixed
 // buffer), and record the capacities.  psets > 1: weight-set slots of
-// slot.points points and slot.runs runs each.
+// slot.points points and slot.runs runs each; sorted weight rows (wsf) for
+// max(psets, wrows) sets of slot.points points if that is more than 1.
 template <class Alloc>
 static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, int sets,
                             int psets, bool with_w, int64_t rhs_elems, Alloc&& alloc,
-                            int segsets = 1, WsSizes slot = WsSizes{0, 0, 0, 0}) {
-  if (psets <= 1) slot = WsSizes{0, 0, 0, 0};
+                            int segsets = 1, WsSizes slot = WsSizes{0, 0, 0, 0}, int wrows = 0) {
+  const int wr = std::max(psets > 1 ? psets : 0, wrows);
+  if (psets <= 1) slot.runs = 0;
+  if (wr <= 1) slot = WsSizes{0, 0, 0, 0};
   const int fsz = pl.precision == DSHT_FP32 ? 4 : 8;
   int rc = DSHT_OK;
   auto get = [&](auto*& ptr, int64_t nbytes) {
@@ -177,11 +180,11 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
   get(ws.rhs, rhs_elems * fsz);
   get(ws.rmax, sets * (int64_t)sizeof(unsigned));
   get(ws.wmax, sets * (int64_t)sizeof(unsigned));
-  if (psets > 1) {
+  if (wr > 1) {
     ws.ldws = (slot.points + 63) / 64 * 64 + 64;
     get(ws.sidx, want.points * (int64_t)sizeof(int));
-    // sorted rows [psets][ldws], then the point-major copy [slot.points][psets]
-    get(ws.wsf, 2 * psets * ws.ldws * (int64_t)sizeof(float));
+    // sorted rows [wr][ldws], then the point-major copy [slot.points][wr]
+    get(ws.wsf, 2 * wr * ws.ldws * (int64_t)sizeof(float));
   }
   if (with_w) get(ws.wsort, want.points * (int64_t)sizeof(double));
   get(ws.a_equ, pl.Nlm * 2 * sets * (int64_t)sizeof(double));
@@ -197,6 +200,7 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
   ws.cap_tiles = want.tiles;
   ws.cap_sets = sets;
   ws.cap_part_sets = psets;
+  ws.cap_wsf_sets = wr > 1 ? wr : 0;
   ws.cap_set_points = slot.points;
   ws.cap_set_runs = slot.runs;
   ws.has_wsort = with_w;
@@ -206,13 +210,15 @@ static int layout_workspace(const Plan& pl, Workspace& ws, const WsSizes& want, 
 }
 
 // Grow the workspace to capacities z (and the other counts) at least;
-// part_sets > 1: as many weight-set slots, each for a transform within z.
+// part_sets > 1: as many weight-set slots, each for a transform within z;
+// wrows > 1: as many sorted weight rows of z.points points.
 static int ensure_workspace_z(Plan& pl, const WsSizes& z, int nsets, bool wsort, int part_sets,
-                              int segsets) {
+                              int segsets, int wrows = 0) {
   Workspace& ws = pl.ws;
   const int64_t rhs_need = (int64_t)pl.Nl * 2 * pl.Npair * rhs_per_pair(pl, nsets);
   const bool slots_ok =
-      part_sets <= 1 || (z.points <= ws.cap_set_points && z.runs <= ws.cap_set_runs);
+      (part_sets <= 1 || (z.points <= ws.cap_set_points && z.runs <= ws.cap_set_runs)) &&
+      (wrows <= 1 || (z.points <= ws.cap_set_points && wrows <= ws.cap_wsf_sets));
   if (ws.flags && z.points <= ws.cap_points && z.chunks <= ws.cap_chunks && z.runs <= ws.cap_runs &&
       z.tiles <= ws.cap_tiles && nsets <= ws.cap_sets && (!wsort || ws.has_wsort) &&
       rhs_need <= ws.rhs_elems && part_sets <= ws.cap_part_sets && segsets <= ws.cap_segsets &&
@@ -231,10 +237,9 @@ static int ensure_workspace_z(Plan& pl, const WsSizes& z, int nsets, bool wsort,
   const int64_t rhs_elems = std::max(rhs_need, ws.rhs_elems);
   const int ssets = std::max(segsets, ws.cap_segsets);
   WsSizes slot{ws.cap_set_points, 0, ws.cap_set_runs, 0};
-  if (part_sets > 1) {
-    slot.points = std::max(slot.points, z.points);
-    slot.runs = std::max(slot.runs, z.runs);
-  }
+  if (part_sets > 1 || wrows > 1) slot.points = std::max(slot.points, z.points);
+  if (part_sets > 1) slot.runs = std::max(slot.runs, z.runs);
+  const int wr = std::max(wrows, ws.cap_wsf_sets);
   free_workspace(ws);
   auto dev_alloc = [&](void** q, int64_t nbytes) -> int {
     const size_t n = (size_t)std::max<int64_t>(nbytes, 16);
@@ -247,12 +252,14 @@ static int ensure_workspace_z(Plan& pl, const WsSizes& z, int nsets, bool wsort,
     ws.bytes += nbytes;
     return DSHT_OK;
   };
-  return layout_workspace(pl, ws, want, sets, psets, with_w, rhs_elems, dev_alloc, ssets, slot);
+  return layout_workspace(pl, ws, want, sets, psets, with_w, rhs_elems, dev_alloc, ssets, slot,
+                          wr);
 }
 
 static int ensure_workspace(Plan& pl, int64_t Np, int nsets = 1, bool wsort = false,
-                            int part_sets = 1) {
-  return ensure_workspace_z(pl, ws_sizes(pl.Nx, Np, pl.chunk_max), nsets, wsort, part_sets, 1);
+                            int part_sets = 1, int wrows = 0) {
+  return ensure_workspace_z(pl, ws_sizes(pl.Nx, Np, pl.chunk_max), nsets, wsort, part_sets, 1,
+                            wrows);
 }
 
 // Point-set batches (dsht_transform_many) on the matrix-core engines run in
@@ -673,12 +680,16 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   const bool many = !a.shared && uses_many(pl, a.nsets, mode);
   if (many) pl.scatter_now = DSHT_SCATTER_MATRIX;
   const bool mfma = use_mfma_contract(pl, a.nsets);
-  // weight sets on the spreading scatter (matrix-core contraction): each set
-  // spread into a slot of its own, the slots folded a group at a time, so the
-  // fold writes whole rows of 8 sets (one set at a time writes 64 of every
-  // 16 * 4 * nsets bytes of a coefficient row)
-  const int spread_group =
-      a.shared && a.nsets > 1 && mfma && uses_spread(pl) ? std::min(a.nsets, kSpreadFoldGroup) : 0;
+  // weight sets on the spreading scatter (matrix-core contraction): on the
+  // node variant each set is spread and folded straight into its slot;
+  // otherwise each set is spread into a partial slot of its own and the
+  // slots are folded a group at a time, so the fold writes whole rows of 8
+  // sets (one set at a time writes 64 of every 16 * 4 * nsets bytes of a
+  // coefficient row)
+  const bool node_sets = a.shared && mfma && uses_spread(pl) && node_spread_fits(pl);
+  const int spread_group = a.shared && a.nsets > 1 && mfma && uses_spread(pl) && !node_sets
+                               ? std::min(a.nsets, kSpreadFoldGroup)
+                               : 0;
   std::vector<ManySets> groups;
   std::vector<int> group_first, dense_sets;
   // point sets on the automatic engine: a set dense enough for the spreading
@@ -735,7 +746,8 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     if (rc == DSHT_OK && max_dense > 0) rc = ensure_workspace(pl, max_dense, a.nsets);
   } else {
     rc = ensure_workspace(pl, maxNp, a.nsets, a.shared && !scatter_gathers(pl),
-                          sets_scatter ? a.nsets : std::max(1, spread_group));
+                          sets_scatter ? a.nsets : std::max(1, spread_group),
+                          node_sets ? a.nsets : 0);
   }
   if (rc) return rc;
   const float* full32 = nullptr;
@@ -768,6 +780,10 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     // one slot per set: fold every set in one launch
     const bool fold_all = sets_scatter && mfma;
     if (fold_all && (rc = launch_fold_sets(pl, a.nsets, s, ws.part, part_set_stride(pl)))) return rc;
+    if (node_sets) {  // sorted FP32 weight rows, then every set in one launch
+      if ((rc = launch_sort_weight_sets(pl, Np, a.wsets, a.ldw, a.nsets, s))) return rc;
+      if ((rc = launch_spread_nodes_sets(pl, 0, a.nsets, a.nsets, true, s))) return rc;
+    }
     for (int g0 = 0; spread_group && g0 < a.nsets; g0 += spread_group) {
       const int g = std::min(spread_group, a.nsets - g0);
       for (int k = 0; k < g; ++k) {
@@ -778,11 +794,16 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       }
       if ((rc = launch_fold_sets(pl, a.nsets, s, ws.part, part_set_stride(pl), g0, g))) return rc;
     }
-    for (int set = 0; set < a.nsets && !fold_all && !spread_group; ++set) {
+    for (int set = 0; set < a.nsets && !fold_all && !spread_group && !node_sets; ++set) {
       const double* w = a.wsets + set * a.ldw;
       if (sets_scatter) {
         const float* part = (const float*)ws.part + set * part_set_stride(pl);
         if ((rc = launch_fold(pl, a.nsets, set, s, part))) return rc;
+        if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
+        continue;
+      }
+      if (gathers && uses_node_spread(pl, a.nsets)) {  // scatter and fold in one launch
+        if ((rc = launch_spread_nodes(pl, Np, w, 2, s))) return rc;
         if (!mfma && (rc = launch_contract_set(pl, a.nsets, set, s))) return rc;
         continue;
       }
@@ -815,6 +836,10 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
       ws.cur_chunks = (int)((Np + pl.chunk - 1) / pl.chunk);
       if ((rc = launch_points(pl, Np, a.theta[set], a.phi[set], a.wt[set], mode, ws.flags + 4 * set, s)))
         return rc;
+      if (node_spread_fits(pl)) {  // scatter and fold into the set's slot in one launch
+        if ((rc = launch_spread_nodes_sets(pl, set, 1, a.nsets, false, s))) return rc;
+        continue;
+      }
       if ((rc = launch_scatter(pl, Np, nullptr, s))) return rc;
       if ((rc = launch_fold(pl, a.nsets, set, s))) return rc;
     }
@@ -1161,10 +1186,13 @@ int dsht_plan_reserve_for(dsht_plan* p, int64_t Np, int nsets, int kinds) {
   int rc = DSHT_OK;
   if (kinds & DSHT_RESERVE_WEIGHT_SETS) {
     // room for either engine a transform of up to Np points may take
+    // (the node spreading of weight sets: their sorted rows)
+    pl.scatter_now = pick_scatter(pl, Np, kPickWeightSets);
+    const bool rows = use_mfma_contract(pl, nsets) && uses_spread(pl) && node_spread_fits(pl);
     pl.scatter_now = pick_scatter(pl, 0);
     const bool sets = uses_sets_scatter(pl, nsets);
     const bool wsort = nsets > 1 && !scatter_gathers(pl);
-    rc = ensure_workspace(pl, Np, nsets, wsort, sets ? nsets : 1);
+    rc = ensure_workspace(pl, Np, nsets, wsort, sets ? nsets : 1, rows ? nsets : 0);
   }
   if (rc == DSHT_OK && (kinds & DSHT_RESERVE_POINT_SETS)) {
     rc = ensure_workspace(pl, Np, nsets);

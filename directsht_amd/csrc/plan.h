@@ -70,6 +70,7 @@ struct Workspace {
                               // [cap_part_sets][cap_set_runs] (one slot per set),
                               // whichever is larger
   int cap_part_sets = 0;      // slots of part
+  int cap_wsf_sets = 0;       // rows of wsf
   int64_t cap_set_points = 0; // points of a weight-set transform (wsf rows)
   int64_t cap_set_runs = 0;   // runs of a part slot of a weight-set transform
   int* seg_count = nullptr;   // [nseg + 2]
@@ -89,9 +90,10 @@ struct Workspace {
   unsigned* wmax = nullptr;   // [cap_sets]: max |w| per weight set (float bits)
   int* sidx = nullptr;        // [cap_points]: input index of each sorted point
                               // (weight-set scatter)
-  float* wsf = nullptr;       // [cap_part_sets][ldws]: weight sets in sorted order,
-                              // FP32, zero past the valid points (weight-set scatter);
-                              // followed by the same size of scratch for the sort
+  float* wsf = nullptr;       // [cap_wsf_sets][ldws]: weight sets in sorted order,
+                              // FP32, zero past the valid points (weight-set scatter,
+                              // node spreading of weight sets); followed by the same
+                              // size of scratch for the sort
   int64_t ldws = 0;           // row stride of wsf: cap_set_points rounded up to 64, + 64
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
@@ -286,6 +288,11 @@ inline int chunk_cap(const Plan& pl) {
 // (float bits)
 int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s,
                       unsigned* out = nullptr);
+// out[k] = the same of weight set k, w + k ldw, for k < nsets (one launch)
+int launch_weight_max_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                           unsigned* out, hipStream_t s);
+// p[0 .. n) = 0 and, if q, q[0 .. n) = 0 (a kernel: capturable)
+int launch_clear_u32(unsigned* p, unsigned* q, int n, hipStream_t s);
 // gather: w holds a weight set in the caller's input order (read through the
 // records' idx); otherwise w is wsort as for launch_scatter
 int launch_scatter_mfma(Plan& pl, int64_t Np, const double* w, bool gather, hipStream_t s);
@@ -318,7 +325,20 @@ inline bool use_mfma_contract(const Plan& pl, int nsets) {
 inline bool uses_node_spread(const Plan& pl, int nsets) {
   return uses_spread(pl) && !use_mfma_contract(pl, nsets) && node_spread_fits(pl);
 }
+// The node variant (scatter and fold in one launch) of a single transform:
+// the plain records; w, wmode as launch_scatter_spread.
 int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
+// The node variant for sets set0 .. set0 + count - 1 of a batch of ntot sets
+// (matrix-core contraction), one launch, into their 16-value row slots, with
+// each set's bound into ws.rmax (cleared here): sorted_w, the weight sets of
+// launch_sort_weight_sets (count of them, maxima in ws.wmax[0 .. count));
+// else one set of the records' own weights (count 1, ws.wmax[0]).
+int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorted_w,
+                             hipStream_t s);
+// The nsets weight sets w + k ldw (k < nsets) of the sorted points in sorted
+// order, FP32, into the rows of ws.wsf, and max |w| per set into ws.wmax.
+int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                            hipStream_t s);
 // the spreading kernel's values at the nq points z (host arrays), as the
 // device evaluates them in the scatters (synchronous; at plan creation)
 int spread_kernel_values(const float* z, int nq, double beta, float* out);

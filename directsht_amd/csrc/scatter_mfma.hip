@@ -556,36 +556,72 @@ int launch_scatter_mfma_many(Plan& pl, const ManySets& ms, const unsigned* wmax,
   return launch_many_m1<32, 128>(pl, ms, wmax, s);
 }
 
-// *wmax = max |w[i]| over the finite entries of one weight set (float bits)
+// wmax[y] = max |w[y ldw + i]| over the finite entries of weight set y
+// (float bits; blockIdx.y = set).  Four independent loads per thread and step.
 __global__ __launch_bounds__(256) void k_absmax(int64_t n, const double* __restrict__ w,
-                                                unsigned* __restrict__ wmax) {
+                                                int64_t ldw, unsigned* __restrict__ wmax) {
   __shared__ float red[4];
-  float v = 0.f;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const float a = (float)fabs(w[i]);
-    if (a <= 3.0e38f) v = fmaxf(v, a);
+  w += blockIdx.y * ldw;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  float v[4] = {0.f, 0.f, 0.f, 0.f};
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = w[i + k * st];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float f = (float)fabs(a[k]);
+      if (f <= 3.0e38f) v[k] = fmaxf(v[k], f);
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  for (; i < n; i += st) {
+    const float f = (float)fabs(w[i]);
+    if (f <= 3.0e38f) v[0] = fmaxf(v[0], f);
+  }
+  float m = fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     const float b = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    atomic_max_pos(wmax, b);
+    atomic_max_pos(wmax + blockIdx.y, b);
   }
 }
 
-__global__ void k_set_u32(unsigned* __restrict__ p, unsigned v) { *p = v; }
+// p[0 .. n) = 0 and, if q, q[0 .. n) = 0
+__global__ void k_clear_u32(unsigned* __restrict__ p, unsigned* __restrict__ q, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p[i] = 0u;
+    if (q) q[i] = 0u;
+  }
+}
+
+int launch_clear_u32(unsigned* p, unsigned* q, int n, hipStream_t s) {
+  if (n <= 0) return DSHT_OK;
+  k_clear_u32<<<(n + 255) / 256, 256, 0, s>>>(p, q, n);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
 
 // m tile of 256, 512 or 1024 orders: the narrowest that covers Nl, or 1024.
 // The operand scale comes from ws.wmax[0]: the points pass's max |w| for the
 // records' own weights; for a gathered weight set, its own max, reduced here.
 int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s, unsigned* out) {
-  if (!out) out = pl.ws.wmax;
-  k_set_u32<<<1, 1, 0, s>>>(out, 0u);
-  DSHT_LAUNCH_CHECK();
-  const int blocks = (int)std::min<int64_t>((Np + 255) / 256, 2048);
-  k_absmax<<<blocks, 256, 0, s>>>(Np, w, out);
+  return launch_weight_max_sets(pl, Np, w, 0, 1, out ? out : pl.ws.wmax, s);
+}
+
+int launch_weight_max_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                           unsigned* out, hipStream_t s) {
+  (void)pl;
+  int rc;
+  if ((rc = launch_clear_u32(out, nullptr, nsets, s))) return rc;
+  if (Np <= 0) return DSHT_OK;
+  // about 2048 blocks in all, four loads in flight per thread
+  const int per = std::max(16, 2048 / nsets);
+  const int blocks = (int)std::min<int64_t>((Np + 1023) / 1024, per);
+  k_absmax<<<dim3(blocks, nsets), 256, 0, s>>>(Np, w, ldw, out);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

@@ -69,12 +69,15 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
   if (i < n) p[i] = v;
 }
 
-// sidx[pos] = input index of sorted point pos (valid points only)
+// sidx[pos] = input index of sorted point pos (valid points only), from the
+// matrix engine's records or (spread) the spreading scatter's
 __global__ __launch_bounds__(256) void k_sorted_idx(int64_t Np, const int* __restrict__ nvalid,
-                                                    const PointRec* __restrict__ rec,
+                                                    const void* __restrict__ rec, int spread,
                                                     int* __restrict__ sidx) {
   const int64_t pos = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (pos < Np && pos < *nvalid) sidx[pos] = rec[pos].idx;
+  if (pos < Np && pos < *nvalid)
+    sidx[pos] = spread ? static_cast<const SpreadRec*>(rec)[pos].idx
+                       : static_cast<const PointRec*>(rec)[pos].idx;
 }
 
 // The sort of the weight sets into the scatter's order is two coalesced
@@ -791,23 +794,23 @@ __global__ __launch_bounds__(64 * NW, 1) void k_scatter_wsets(
   }
 }
 
-int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
-                        hipStream_t s) {
-  const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
-  if (nchunks == 0) return DSHT_OK;
+int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                            hipStream_t s) {
   Workspace& ws = pl.ws;
-  if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_part_sets || !ws.wsf ||
+  if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_wsf_sets || !ws.wsf ||
       Np > ws.cap_set_points) {
-    set_error("dsht: the weight-set scatter takes 1..64 sets within the workspace");
+    set_error("dsht: sorting 1..64 weight sets within the workspace");
     return DSHT_EINVAL;
   }
   const int* nvalid = ws.seg_start + pl.nseg;
   k_fill_u32<<<1, 64, 0, s>>>(ws.wmax, nsets, 0u);
   DSHT_LAUNCH_CHECK();
-  k_sorted_idx<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, nvalid, ws.rec, ws.sidx);
+  if (Np == 0) return DSHT_OK;
+  k_sorted_idx<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, nvalid, ws.rec, uses_spread(pl) ? 1 : 0,
+                                                       ws.sidx);
   DSHT_LAUNCH_CHECK();
   // wt (point-major copy) lives in the second half of the wsf allocation
-  float* wt = ws.wsf + (int64_t)ws.cap_part_sets * ws.ldws;
+  float* wt = ws.wsf + (int64_t)ws.cap_wsf_sets * ws.ldws;
   k_transpose_w<<<dim3((unsigned)((Np + kTile - 1) / kTile), (nsets + kTile - 1) / kTile), 256, 0,
                   s>>>(w, ldw, Np, nsets, wt);
   DSHT_LAUNCH_CHECK();
@@ -817,6 +820,20 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
   k_gather_sets<<<(unsigned)((n + per - 1) / per), 256, 0, s>>>(wt, nsets, nvalid, ws.sidx, n,
                                                                 ws.wsf, ws.ldws, ws.wmax);
   DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
+}
+
+int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                        hipStream_t s) {
+  const int nchunks = (int)((Np + pl.chunk - 1) / pl.chunk);
+  if (nchunks == 0) return DSHT_OK;
+  Workspace& ws = pl.ws;
+  if (nsets > ws.cap_part_sets) {
+    set_error("dsht: the weight-set scatter takes 1..64 sets within the workspace");
+    return DSHT_EINVAL;
+  }
+  int rc;
+  if ((rc = launch_sort_weight_sets(pl, Np, w, ldw, nsets, s))) return rc;
   const int nmy = (pl.Nl + kOrders - 1) / kOrders;
   const int64_t nwg = (int64_t)8 * ((nchunks + 7) / 8) * nmy;
   const int64_t stride = part_set_stride(pl);

@@ -351,12 +351,31 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 // pair and branch, whole lines from one workgroup).
 // (One workgroup per node pair and branch measured slower.)  Grids up to
 // n = 4096 (all four in LDS at once, one mode per thread).
-template <int LOGN>
+// kSets: ns.count sets of a batch for the matrix-core contraction instead,
+// set s into slot set0 + s (the 16-value rows rhs[m][P][np][slot of
+// ntot][16] of the batched fold: branch br at 8 br, re then im, each
+// c(2np), c(2np+1), dx d(2np), dx d(2np+1)), with the set's max(|c|, |d|)
+// into rmax[s] (float bits, cleared beforehand).  Set s weighs its points by
+// wsf[s ldws + sorted position] (FP32 weight sets in sorted order, scaled by
+// wmax[s]), or by the records' own weights if wsf is NULL (wmax[0]).  The
+// workgroups of one node pair for all sets run together on one XCD (block
+// b -> XCD b mod 8, the order in which the hardware hands them out): they
+// read the same records (from that XCD's L2) and write the same coefficient
+// rows at about the same time, so the sets' 32-byte pieces of a row meet in
+// the caches and reach HBM as whole lines.
+struct NodeSets {
+  const float* wsf;
+  int64_t ldws;
+  unsigned* rmax;
+  int count, set0, ntot;
+};
+constexpr int kNodeBatch = 2;
+template <int LOGN, bool kSets>
 __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
     const SpreadRec* __restrict__ rec, const int* __restrict__ seg_start, int Nx, int Nl,
     int Npair, double dx, const double* __restrict__ w, int wmode, float beta_log2e,
     const float2* __restrict__ tw, const float* __restrict__ ipsi, float* __restrict__ rhs,
-    const unsigned* __restrict__ wmax) {
+    const unsigned* __restrict__ wmax, NodeSets ns) {
   using Gm = SpreadGeom<LOGN>;
   constexpr int n = Gm::n, G = Gm::G, NCG = Gm::NCG, kT = Gm::kThreads;
   constexpr int kW = kSpreadW;
@@ -364,7 +383,14 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
   __shared__ __attribute__((aligned(16))) unsigned long long sgrid[G * n];
   float2* sz = reinterpret_cast<float2*>(sgrid);
   const int tid = threadIdx.x;
-  const int np = blockIdx.x;
+  int np = blockIdx.x, sl = 0;  // node pair, set of the launch
+  if constexpr (kSets) {  // XCD x: node pairs x, x + 8, ..., each for every set
+    const int k = blockIdx.x >> 3;
+    np = (blockIdx.x & 7) + 8 * (k / ns.count);
+    sl = k % ns.count;
+    if (np >= Npair) return;  // the padding of the last round
+  }
+  const float* wsf = kSets && ns.wsf ? ns.wsf + sl * ns.ldws : nullptr;
   const int m = tid;  // this thread's mode
   // twiddles of the first radix-8 pass in registers (the spectra held across
   // the units leave no room for more), the others from the table
@@ -377,7 +403,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
   }
   int ex = 0;
   {
-    const float mx = __uint_as_float(wmax[0]);
+    const float mx = __uint_as_float(wmax[wsf ? sl : 0]);
     if (mx > 0.f && mx <= 3.0e38f) (void)frexpf(mx, &ex);
     ex = max(-120, min(120, ex));
   }
@@ -393,7 +419,9 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
 
   // the records of branch br at mode m from the pair's spectra
   // a[odd][h][c re, c im, d re, d im], as the fold writes them: per P,
-  // [c | f | g] x (re, im) x (node 0, 1), 48 contiguous bytes
+  // [c | f | g] x (re, im) x (node 0, 1), 48 contiguous bytes (kSets: the
+  // branch's 32 bytes of the set's row)
+  float vmax = 0.f;
   auto emit = [&](int br, const float (&a0)[2][4], const float (&a1)[2][4]) {
 #pragma unroll
     for (int P = 0; P < 2; ++P) {
@@ -406,15 +434,27 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
           const float (&a)[2][4] = odd ? a1 : a0;
           const double cv = (double)a[0][ri] + sg * (double)a[1][ri];
           const double dv = dx * ((double)a[0][2 + ri] + sg * (double)a[1][2 + ri]);
-          const double du = dx * ((double)a[0][2 + ri] - sg * (double)a[1][2 + ri]);  // d_{1-P}
-          c4[2 * ri + odd] = (float)cv;
-          f4[2 * ri + odd] = (float)(xs[odd] * inv[odd] * dv);
-          g4[2 * ri + odd] = (float)(inv[odd] * du);
+          if constexpr (kSets) {
+            c4[2 * ri + odd] = (float)cv;
+            f4[2 * ri + odd] = (float)dv;
+            vmax = fmaxf(vmax, (float)fmax(fabs(cv), fabs(dv / dx)));
+          } else {
+            const double du = dx * ((double)a[0][2 + ri] - sg * (double)a[1][2 + ri]);  // d_{1-P}
+            c4[2 * ri + odd] = (float)cv;
+            f4[2 * ri + odd] = (float)(xs[odd] * inv[odd] * dv);
+            g4[2 * ri + odd] = (float)(inv[odd] * du);
+          }
         }
-      float* o = rhs + plain_rec(np, br, m, P, Nl);
-      *reinterpret_cast<float4*>(o) = make_float4(c4[0], c4[1], c4[2], c4[3]);
-      *reinterpret_cast<float4*>(o + 4) = make_float4(f4[0], f4[1], f4[2], f4[3]);
-      *reinterpret_cast<float4*>(o + 8) = make_float4(g4[0], g4[1], g4[2], g4[3]);
+      if constexpr (kSets) {
+        float* o = rhs + ((((int64_t)m * 2 + P) * Npair + np) * ns.ntot + ns.set0 + sl) * 16 + 8 * br;
+        *reinterpret_cast<float4*>(o) = make_float4(c4[0], c4[1], f4[0], f4[1]);
+        *reinterpret_cast<float4*>(o + 4) = make_float4(c4[2], c4[3], f4[2], f4[3]);
+      } else {
+        float* o = rhs + plain_rec(np, br, m, P, Nl);
+        *reinterpret_cast<float4*>(o) = make_float4(c4[0], c4[1], c4[2], c4[3]);
+        *reinterpret_cast<float4*>(o + 4) = make_float4(f4[0], f4[1], f4[2], f4[3]);
+        *reinterpret_cast<float4*>(o + 8) = make_float4(g4[0], g4[1], g4[2], g4[3]);
+      }
     }
   };
 
@@ -456,22 +496,43 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
         for (int h = 0; h < 2; ++h) {
           unsigned long long* gc = sgrid + (2 * h) * n;  // c^h, then d^h
           unsigned long long* gd = gc + n;
-          for (int q = lo[h] + tid; q < hi[h]; q += kT) {
-            const float4 ah = reinterpret_cast<const float4*>(rec + q)[0];
-            const int4 ri = reinterpret_cast<const int4*>(rec + q)[1];  // seg, idx, j0, z0
-            const double wt = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri.y])) * wf;
-            const bool upper = q >= mid[h];  // bin node: weights a0, a1; bin node - 1: a2, a3
-            const double ac = wt * (double)(upper ? ah.x : ah.z);
-            const double ad = wt * (double)(upper ? ah.y : ah.w);
-            const int j0 = ri.z;
-            const float zs = __int_as_float(ri.w) * (2.0f / kW);  // z0, scaled
+          // kNodeBatch points per thread and step: their record loads, then
+          // their weight loads, are in flight together
+          for (int q0 = lo[h] + tid; q0 < hi[h]; q0 += kNodeBatch * kT) {
+            float4 ah[kNodeBatch];
+            int4 ri[kNodeBatch];  // seg, idx, j0, z0
+            double wt[kNodeBatch];
 #pragma unroll
-            for (int i = 0; i < kW; ++i) {
-              const float z = zs + (float)i * (2.0f / kW);
-              const double psi = (double)es_kernel(z, beta_log2e);
-              const int x = (j0 + i) & (n - 1);
-              atomicAdd(&gc[x], to_fixed(ac, psi));
-              atomicAdd(&gd[x], to_fixed(ad, psi));
+            for (int b = 0; b < kNodeBatch; ++b) {
+              const int q = min(q0 + b * kT, hi[h] - 1);
+              ah[b] = reinterpret_cast<const float4*>(rec + q)[0];
+              ri[b] = reinterpret_cast<const int4*>(rec + q)[1];
+            }
+#pragma unroll
+            for (int b = 0; b < kNodeBatch; ++b) {
+              const int q = min(q0 + b * kT, hi[h] - 1);
+              if (kSets)
+                wt[b] = (wsf ? (double)wsf[q] : 1.0) * wf;
+              else
+                wt[b] = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri[b].y])) * wf;
+            }
+#pragma unroll
+            for (int b = 0; b < kNodeBatch; ++b) {
+              const int q = q0 + b * kT;
+              if (q >= hi[h]) break;
+              const bool upper = q >= mid[h];  // bin node: weights a0, a1; bin node - 1: a2, a3
+              const double ac = wt[b] * (double)(upper ? ah[b].x : ah[b].z);
+              const double ad = wt[b] * (double)(upper ? ah[b].y : ah[b].w);
+              const int j0 = ri[b].z;
+              const float zs = __int_as_float(ri[b].w) * (2.0f / kW);  // z0, scaled
+#pragma unroll
+              for (int i = 0; i < kW; ++i) {
+                const float z = zs + (float)i * (2.0f / kW);
+                const double psi = (double)es_kernel(z, beta_log2e);
+                const int x = (j0 + i) & (n - 1);
+                atomicAdd(&gc[x], to_fixed(ac, psi));
+                atomicAdd(&gd[x], to_fixed(ad, psi));
+              }
             }
           }
         }
@@ -525,6 +586,11 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
       }
     }
   }
+  if constexpr (kSets) {  // the set's bound: wavefront, then one atomic per wavefront
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+    if ((tid & 63) == 0) atomic_max_pos(ns.rmax + sl, vmax);
+  }
 }
 
 __global__ void k_es_values(const float* __restrict__ z, int nq, float beta_log2e,
@@ -552,30 +618,56 @@ int spread_kernel_values(const float* z, int nq, double beta, float* out) {
 
 template <int LOGN>
 static int launch_nodes_n(Plan& pl, const double* w, int wmode, const unsigned* wmax,
-                          hipStream_t s) {
+                          const NodeSets* ns, hipStream_t s) {
   using Gm = SpreadGeom<LOGN>;
-  k_spread_nodes<LOGN><<<pl.Npair, Gm::kThreads, 0, s>>>(
-      reinterpret_cast<const SpreadRec*>(pl.ws.rec), pl.ws.seg_start, pl.Nx, pl.Nl, pl.Npair,
-      pl.dx, w, wmode, (float)(pl.spread_beta * M_LOG2E), pl.spread_tw, pl.spread_ipsi, (float*)pl.ws.rhs,
-      wmax);
+  const float bl = (float)(pl.spread_beta * M_LOG2E);
+  const auto* rec = reinterpret_cast<const SpreadRec*>(pl.ws.rec);
+  if (ns) {
+    const int blocks = 8 * ((pl.Npair + 7) / 8) * ns->count;  // rounds of 8 node pairs x sets
+    k_spread_nodes<LOGN, true><<<blocks, Gm::kThreads, 0, s>>>(
+        rec, pl.ws.seg_start, pl.Nx, pl.Nl, pl.Npair, pl.dx, w, wmode, bl, pl.spread_tw,
+        pl.spread_ipsi, (float*)pl.ws.rhs, wmax, *ns);
+  } else {
+    k_spread_nodes<LOGN, false><<<pl.Npair, Gm::kThreads, 0, s>>>(
+        rec, pl.ws.seg_start, pl.Nx, pl.Nl, pl.Npair, pl.dx, w, wmode, bl, pl.spread_tw,
+        pl.spread_ipsi, (float*)pl.ws.rhs, wmax, NodeSets{});
+  }
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
+}
+
+static int nodes_dispatch(Plan& pl, const double* w, int wmode, const unsigned* wmax,
+                          const NodeSets* ns, hipStream_t s) {
+  switch (pl.spread_logn) {
+    case 8: return launch_nodes_n<8>(pl, w, wmode, wmax, ns, s);
+    case 9: return launch_nodes_n<9>(pl, w, wmode, wmax, ns, s);
+    case 10: return launch_nodes_n<10>(pl, w, wmode, wmax, ns, s);
+    case 11: return launch_nodes_n<11>(pl, w, wmode, wmax, ns, s);
+    case 12: return launch_nodes_n<12>(pl, w, wmode, wmax, ns, s);
+    default:
+      set_error("dsht: the node spreading scatter supports grids up to 4096 cells");
+      return DSHT_EINVAL;
+  }
 }
 
 int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s) {
   int rc;
   if (wmode != 0 && Np > 0 && (rc = launch_weight_max(pl, Np, w, s))) return rc;
-  const unsigned* wmax = pl.ws.wmax;
-  switch (pl.spread_logn) {
-    case 8: return launch_nodes_n<8>(pl, w, wmode, wmax, s);
-    case 9: return launch_nodes_n<9>(pl, w, wmode, wmax, s);
-    case 10: return launch_nodes_n<10>(pl, w, wmode, wmax, s);
-    case 11: return launch_nodes_n<11>(pl, w, wmode, wmax, s);
-    case 12: return launch_nodes_n<12>(pl, w, wmode, wmax, s);
-    default:
-      set_error("dsht: the node spreading scatter supports grids up to 4096 cells");
-      return DSHT_EINVAL;
+  return nodes_dispatch(pl, w, wmode, pl.ws.wmax, nullptr, s);
+}
+
+int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorted_w,
+                             hipStream_t s) {
+  Workspace& ws = pl.ws;
+  if (count < 1 || set0 < 0 || set0 + count > ntot || ntot > ws.cap_sets ||
+      (sorted_w && (count > ws.cap_wsf_sets || !ws.wsf))) {
+    set_error("dsht: node spreading into batch slots beyond the workspace's sets");
+    return DSHT_EINVAL;
   }
+  NodeSets ns{sorted_w ? ws.wsf : nullptr, ws.ldws, ws.rmax + set0, count, set0, ntot};
+  int rc;
+  if ((rc = launch_clear_u32(ws.rmax + set0, nullptr, count, s))) return rc;
+  return nodes_dispatch(pl, nullptr, 0, ws.wmax, &ns, s);
 }
 
 template <int LOGN>

@@ -208,6 +208,82 @@ def test_many_point_sets_dense_and_sparse(torch_cuda):
             assert rel(alm[i], ref) < TOL["float32"], i
 
 
+@pytest.mark.parametrize("S", [3, 5, 21])
+def test_weight_sets_node_spread(torch_cuda, S):
+    """Weight sets on the spreading scatter with 512 node pairs (its node
+    variant): from 4 sets each set is spread and folded in one launch into
+    its slot of the matrix-core contraction (3 sets: into the plain records,
+    contracted set by set).  Sets of other signs and scales (x -1e3, x 1e-6,
+    all zero): each within 1e-6 of its single transform, of the FP64 plan,
+    and two of them of the FP64 oracle; a second call bit for bit equal."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    from oracle import refalg
+    Nl, Nx, n = 96, 1024, 30000
+    th, ph = points(torch, 500 + S, n)
+    g = torch.Generator(device="cuda").manual_seed(7 + S)
+    W = 0.5 + torch.rand((S, n), generator=g, device="cuda", dtype=torch.float64)
+    W[1] = -W[1] * 1e3
+    W[2] = W[2] * 1e-6
+    if S > 3:
+        W[3] = 0.0
+    sht = DirectSHT(Nl, Nx, dflt_type="float32")
+    sht.configure(scatter="spread")
+    alm, status = sht.transform_sets_device(th, ph, W)
+    again, _ = sht.transform_sets_device(th, ph, W)
+    torch.cuda.synchronize()
+    assert int(status.max()) == 0
+    assert torch.equal(alm, again)
+    f64 = DirectSHT(Nl, Nx, dflt_type="float64")
+    for s in range(S):
+        if s == 3:
+            assert float(alm[s].abs().max()) == 0.0
+            continue
+        assert rel(alm[s], sht(th, ph, W[s])) < TOL["float32"], s
+        assert rel(alm[s], f64(th, ph, W[s])) < TOL["float32"], s
+    tables = refalg.make_tables(Nl, Nx, 0.75)
+    t, p = th.cpu().numpy(), ph.cpu().numpy()
+    for s in (1, 2):
+        ref = refalg.direct_sht(tables, t, p, W[s].cpu().numpy())
+        assert rel(alm[s], ref) < TOL["float32"], s
+
+
+def test_many_point_sets_dense_node_spread(torch_cuda):
+    """A point-set batch at 512 node pairs on the automatic engine: the dense
+    sets (256 or more points per segment) are spread and folded in one launch
+    each into their slots (the node variant), the sparse ones go in groups
+    through the matrix engine.  Each set within 1e-6 of the FP64 plan's
+    transform of it, one dense and two sparse ones also of the FP64 oracle."""
+    torch = torch_cuda
+    from directsht_amd.sht import DirectSHT
+    from oracle import refalg
+    Nl, Nx = 40, 1024
+    dense_n = 256 * 4 * (Nx - 1)
+    sizes = [dense_n + 777, 900, 0, 2 * dense_n - 3, 12000, dense_n]
+    sets = []
+    for i, n in enumerate(sizes):
+        th, ph = points(torch, 950 + i, n)
+        w = 0.5 + torch.rand(n, device="cuda", dtype=torch.float64)
+        if i == 3:
+            w = -w * 1e4
+        sets.append((th, ph, w))
+    sht = DirectSHT(Nl, Nx, dflt_type="float32")
+    assert [sht.scatter_engine(n) for n in sizes].count("spread") == 3
+    alm, status = sht.transform_many_device(sets)
+    torch.cuda.synchronize()
+    assert int(status.max()) == 0
+    f64 = DirectSHT(Nl, Nx, dflt_type="float64")
+    tables = refalg.make_tables(Nl, Nx, 0.75)
+    for i, (th, ph, w) in enumerate(sets):
+        if th.numel() == 0:
+            assert float(alm[i].abs().max()) == 0.0, i
+            continue
+        assert rel(alm[i], f64(th, ph, w)) < TOL["float32"], i
+        if i in (0, 1, 4):
+            ref = refalg.direct_sht(tables, *(a.cpu().numpy() for a in (th, ph, w)))
+            assert rel(alm[i], ref) < TOL["float32"], i
+
+
 def test_reserve_kinds_cover_their_transforms(torch_cuda):
     """A point-set reservation holds every dsht_transform_many group of its
     sizes, a weight-set reservation every dsht_transform_sets: the plan's
