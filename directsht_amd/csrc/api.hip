@@ -672,9 +672,10 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
   }
   // the scatter engine: of the shared points for weight sets, the matrix
   // engine for grouped point sets, per set otherwise (set again below)
-  pl.scatter_now = a.shared ? pick_scatter(pl, a.Np[0], a.nsets >= kSetsScatterMin
-                                                            ? kPickWeightSets
-                                                            : kPickPointSet)
+  pl.scatter_now = a.shared ? pick_scatter(pl, a.Np[0],
+                                            a.nsets >= kSetsScatterMin ? kPickWeightSets
+                                                                       : kPickPointSet,
+                                            a.nsets)
                            : pick_scatter(pl, maxNp);
   const bool sets_scatter = a.shared && uses_sets_scatter(pl, a.nsets);
   const bool many = !a.shared && uses_many(pl, a.nsets, mode);
@@ -1187,7 +1188,7 @@ int dsht_plan_reserve_for(dsht_plan* p, int64_t Np, int nsets, int kinds) {
   if (kinds & DSHT_RESERVE_WEIGHT_SETS) {
     // room for either engine a transform of up to Np points may take
     // (the node spreading of weight sets: their sorted rows)
-    pl.scatter_now = pick_scatter(pl, Np, kPickWeightSets);
+    pl.scatter_now = pick_scatter(pl, Np, kPickWeightSets, nsets);
     const bool rows = use_mfma_contract(pl, nsets) && uses_spread(pl) && node_spread_fits(pl);
     pl.scatter_now = pick_scatter(pl, 0);
     const bool sets = uses_sets_scatter(pl, nsets);

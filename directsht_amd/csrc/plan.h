@@ -261,18 +261,24 @@ inline bool node_spread_fits(const Plan& pl) {
 // points per segment for Nl 512-1024, the run-based one (and the fold
 // after it) about 250 at Nl 256.  Weight sets on shared points are
 // scattered together on the matrix engine (its operands built once for all
-// sets) but spread one by one: 0.14 against ~0.25 ms a set at 2e6 points,
-// Nl = 768 (326 points per segment), 0.78 against 0.44 ms at 1e7 points,
-// Nl = 1000 (1221): break-even near 600.
+// sets).  The run-based spreading takes them one by one: 0.14 against ~0.25
+// ms a set at 2e6 points, Nl = 768 (326 points per segment), 0.78 against
+// 0.44 ms at 1e7 points, Nl = 1000 (1221): break-even near 600.  The node
+// variant takes them all in one launch: break-even (whole batches, uniform
+// points, Nl 512-1000) near 100 points per segment for 16 sets and 250-350
+// for 64, about 5 per set (fewer than 4 sets: the plain contraction after
+// each set's spreading, 128).
 constexpr int kSpreadMinPerSeg = 256;
 constexpr int kNodeSpreadMinPerSeg = 64;
 constexpr int kSetsSpreadMinPerSeg = 640;
+inline int node_sets_min_per_seg(int nsets) { return nsets < 4 ? 128 : 5 * nsets + 16; }
 enum PickFor { kPickPlain, kPickPointSet, kPickWeightSets };
-inline int pick_scatter(const Plan& pl, int64_t Np, PickFor use = kPickPointSet) {
+inline int pick_scatter(const Plan& pl, int64_t Np, PickFor use = kPickPointSet, int nsets = 1) {
   if (pl.scatter != DSHT_SCATTER_AUTO) return pl.scatter;
-  const int per = use == kPickWeightSets                        ? kSetsSpreadMinPerSeg
-                  : use == kPickPlain && node_spread_fits(pl) ? kNodeSpreadMinPerSeg
-                                                                : kSpreadMinPerSeg;
+  const bool node = node_spread_fits(pl);
+  const int per = use == kPickWeightSets ? (node ? node_sets_min_per_seg(nsets) : kSetsSpreadMinPerSeg)
+                  : use == kPickPlain && node ? kNodeSpreadMinPerSeg
+                                              : kSpreadMinPerSeg;
   return pl.Nl <= kSpreadMaxNl && Np >= (int64_t)per * pl.nseg ? DSHT_SCATTER_SPREAD
                                                                : DSHT_SCATTER_MATRIX;
 }

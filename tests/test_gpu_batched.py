@@ -284,10 +284,12 @@ def test_many_point_sets_dense_node_spread(torch_cuda):
             assert rel(alm[i], ref) < TOL["float32"], i
 
 
-def test_reserve_kinds_cover_their_transforms(torch_cuda):
+@pytest.mark.parametrize("Nx,engine", [(192, None), (1024, "spread")])
+def test_reserve_kinds_cover_their_transforms(torch_cuda, Nx, engine):
     """A point-set reservation holds every dsht_transform_many group of its
     sizes, a weight-set reservation every dsht_transform_sets: the plan's
-    footprint does not change over transforms within them."""
+    footprint does not change over transforms within them (also on the node
+    spreading variant, whose weight sets need sorted weight rows)."""
     torch = torch_cuda
     from directsht_amd.sht import DirectSHT
     n, S = 50000, 20
@@ -296,7 +298,9 @@ def test_reserve_kinds_cover_their_transforms(torch_cuda):
     for i, m in enumerate(int(x) for x in rng.integers(1, n + 1, S)):
         th, ph = points(torch, 700 + i, m)
         sets.append((th, ph, 0.5 + torch.rand(m, device="cuda", dtype=torch.float64)))
-    base = DirectSHT(96, 192, dflt_type="float32")
+    base = DirectSHT(96, Nx, dflt_type="float32")
+    if engine:
+        base.configure(scatter=engine)
     sp = base.clone()
     sp.reserve(n, S, weight_sets=False)
     f0 = sp.footprint()
