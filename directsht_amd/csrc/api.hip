@@ -42,6 +42,22 @@ void build_layout(int Nl, int Npair, std::vector<GroupDesc>& groups, std::vector
   tab_vec4 = off;
 }
 
+// The matrix-core contraction's work items: each group of 64 rows together
+// with the next group of the same (m, P) (which starts nrows * Npair table
+// entries further on), else alone.
+static std::vector<GroupDesc> pair_groups(const std::vector<GroupDesc>& groups) {
+  std::vector<GroupDesc> out;
+  for (size_t i = 0; i < groups.size(); ++i) {
+    GroupDesc g = groups[i];
+    if (i + 1 < groups.size() && g.nrows == kGroupRows && groups[i + 1].m == g.m &&
+        groups[i + 1].P == g.P) {
+      g.nrows += groups[++i].nrows;
+    }
+    out.push_back(g);
+  }
+  return out;
+}
+
 // Launch order of the contraction's row groups.  Workgroups are dispatched to
 // the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each XCD has its
 // own L2.  Give every XCD a contiguous, row-balanced range of groups, so the
@@ -508,10 +524,14 @@ static int plan_init(Plan& pl, int Nl, int Nx, double xmax, int precision, bool 
   std::vector<GroupDesc> groups;
   std::vector<int64_t> mp_off;
   build_layout(Nl, pl.Npair, groups, mp_off, pl.tab_vec4);
+  std::vector<GroupDesc> mgroups = xcd_schedule(pair_groups(groups), kXcds);
   groups = xcd_schedule(groups, kXcds);
   pl.ngroups = (int)groups.size();
+  pl.nmgroups = (int)mgroups.size();
+  groups.insert(groups.end(), mgroups.begin(), mgroups.end());
   Workspace& ws = pl.ws;  // DSHT_ALLOC counts bytes into ws.bytes; moved below
   DSHT_ALLOC(pl.groups, groups.size() * sizeof(GroupDesc));
+  pl.mgroups = pl.groups + pl.ngroups;
   DSHT_ALLOC(pl.mp_off, mp_off.size() * sizeof(int64_t));
   DSHT_HIP(hipMemcpy(pl.groups, groups.data(), groups.size() * sizeof(GroupDesc), hipMemcpyHostToDevice));
   DSHT_HIP(hipMemcpy(pl.mp_off, mp_off.data(), mp_off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -1153,6 +1173,8 @@ int dsht_plan_clone(const dsht_plan* src, dsht_plan** out) {
   d.contract = s.contract;
   d.ngroups = s.ngroups;
   d.groups = s.groups;
+  d.nmgroups = s.nmgroups;
+  d.mgroups = s.mgroups;
   d.mp_off = s.mp_off;
   d.tab_vec4 = s.tab_vec4;
   d.tab64y = s.tab64y;
@@ -1239,12 +1261,13 @@ int dsht_plan_estimate(int Nl, int Nx, int precision, int64_t Np, int64_t* table
   int64_t vec4 = 0;
   const int Npair = node_pairs(Nx);
   build_layout(Nl, Npair, groups, mp_off, vec4);
+  const size_t nm = xcd_schedule(pair_groups(groups), kXcds).size();
   groups = xcd_schedule(groups, kXcds);
   const int64_t Nlm = (int64_t)Nl * (Nl + 1) / 2;
   // FP32 plans keep only the recurrence's coefficients (no Y table)
   const int64_t tab = precision == DSHT_FP32 ? 16 * otf_offset(Nl, Nl) : vec4 * 2 * 8;
   if (table_bytes)
-    *table_bytes = tab + Nlm * (3 * 8 + 4) + (int64_t)groups.size() * sizeof(GroupDesc) +
+    *table_bytes = tab + Nlm * (3 * 8 + 4) + (int64_t)(groups.size() + nm) * sizeof(GroupDesc) +
                    (int64_t)mp_off.size() * 8;
   if (work_bytes) *work_bytes = workspace_bytes(Nl, Nx, precision, Np, 2048);
   return DSHT_OK;

@@ -89,11 +89,12 @@ struct Workspace {
   unsigned* rmax = nullptr;   // [cap_sets]: max(|c|, |d/dx|) per set (float bits)
   unsigned* wmax = nullptr;   // [cap_sets]: max |w| per weight set (float bits)
   int* sidx = nullptr;        // [cap_points]: input index of each sorted point
-                              // (weight-set scatter)
+                              // (weight-set scatter); sorted position of each input
+                              // point (node spreading of weight sets)
   float* wsf = nullptr;       // [cap_wsf_sets][ldws]: weight sets in sorted order,
-                              // FP32, zero past the valid points (weight-set scatter,
-                              // node spreading of weight sets); followed by the same
-                              // size of scratch for the sort
+                              // FP32, zero past the valid points (weight-set scatter;
+                              // the node spreading of weight sets: [pos][nsets]);
+                              // followed by the same size of scratch for the sort
   int64_t ldws = 0;           // row stride of wsf: cap_set_points rounded up to 64, + 64
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
   double* a_equ = nullptr;    // [cap_sets][Nlm][2]
@@ -149,6 +150,11 @@ struct Plan {
   // contraction layout
   int ngroups = 0;
   GroupDesc* groups = nullptr;  // device [ngroups]
+  // the matrix-core contraction's work: pairs of consecutive groups of one
+  // (m, P) (nrows up to 128), in XCD order; device, after groups in the same
+  // allocation
+  int nmgroups = 0;
+  GroupDesc* mgroups = nullptr;
   int64_t* mp_off = nullptr;    // device [2*Nl]
   int64_t tab_vec4 = 0;         // entries (row, node pair) of one table copy
   // Y-only tables {Y(x_2np), Y(x_2np+1)} per entry: FP64 plans tab64y, FP32
@@ -294,9 +300,6 @@ inline int chunk_cap(const Plan& pl) {
 // (float bits)
 int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s,
                       unsigned* out = nullptr);
-// out[k] = the same of weight set k, w + k ldw, for k < nsets (one launch)
-int launch_weight_max_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
-                           unsigned* out, hipStream_t s);
 // p[0 .. n) = 0 and, if q, q[0 .. n) = 0 (a kernel: capturable)
 int launch_clear_u32(unsigned* p, unsigned* q, int n, hipStream_t s);
 // gather: w holds a weight set in the caller's input order (read through the
@@ -319,11 +322,12 @@ inline int work_chunk(int chunk_max, int64_t Np) {
                                 std::min<int64_t>(c, chunk_max));
 }
 constexpr int kMfmaMinSets = 4;  // FP32 plans: matrix-core contraction from this many sets
+constexpr int kMfmaMinPairs = 16;  // and node pairs (one chunk of its K dimension)
 constexpr int kMaxSets = 64;     // sets per batched transform
 // the matrix-core contraction handles a transform of nsets sets on this plan
 inline bool use_mfma_contract(const Plan& pl, int nsets) {
   return pl.precision == DSHT_FP32 && !pl.caller_tables && nsets >= kMfmaMinSets &&
-         pl.contract == DSHT_CONTRACT_MATRIX;
+         pl.contract == DSHT_CONTRACT_MATRIX && pl.Npair >= kMfmaMinPairs;
 }
 // The node variant for transforms that write the plain coefficient layout
 // (no matrix-core contraction after them).  rhs: the plain layout
@@ -341,8 +345,9 @@ int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStr
 // else one set of the records' own weights (count 1, ws.wmax[0]).
 int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorted_w,
                              hipStream_t s);
-// The nsets weight sets w + k ldw (k < nsets) of the sorted points in sorted
-// order, FP32, into the rows of ws.wsf, and max |w| per set into ws.wmax.
+// The nsets weight sets w + k ldw (k < nsets) of the spreading scatter's
+// sorted points, FP32, point-major in sorted order into ws.wsf
+// ([pos][nsets]), and max |w| per set into ws.wmax.
 int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
                             hipStream_t s);
 // the spreading kernel's values at the nq points z (host arrays), as the

@@ -556,12 +556,11 @@ int launch_scatter_mfma_many(Plan& pl, const ManySets& ms, const unsigned* wmax,
   return launch_many_m1<32, 128>(pl, ms, wmax, s);
 }
 
-// wmax[y] = max |w[y ldw + i]| over the finite entries of weight set y
-// (float bits; blockIdx.y = set).  Four independent loads per thread and step.
+// *wmax = max |w[i]| over the finite entries of one weight set (float bits).
+// Four independent loads per thread and step.
 __global__ __launch_bounds__(256) void k_absmax(int64_t n, const double* __restrict__ w,
-                                                int64_t ldw, unsigned* __restrict__ wmax) {
+                                                unsigned* __restrict__ wmax) {
   __shared__ float red[4];
-  w += blockIdx.y * ldw;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
   float v[4] = {0.f, 0.f, 0.f, 0.f};
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -585,7 +584,7 @@ __global__ __launch_bounds__(256) void k_absmax(int64_t n, const double* __restr
   __syncthreads();
   if (threadIdx.x == 0) {
     const float b = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    atomic_max_pos(wmax + blockIdx.y, b);
+    atomic_max_pos(wmax, b);
   }
 }
 
@@ -609,19 +608,13 @@ int launch_clear_u32(unsigned* p, unsigned* q, int n, hipStream_t s) {
 // The operand scale comes from ws.wmax[0]: the points pass's max |w| for the
 // records' own weights; for a gathered weight set, its own max, reduced here.
 int launch_weight_max(Plan& pl, int64_t Np, const double* w, hipStream_t s, unsigned* out) {
-  return launch_weight_max_sets(pl, Np, w, 0, 1, out ? out : pl.ws.wmax, s);
-}
-
-int launch_weight_max_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
-                           unsigned* out, hipStream_t s) {
-  (void)pl;
+  if (!out) out = pl.ws.wmax;
   int rc;
-  if ((rc = launch_clear_u32(out, nullptr, nsets, s))) return rc;
+  if ((rc = launch_clear_u32(out, nullptr, 1, s))) return rc;
   if (Np <= 0) return DSHT_OK;
-  // about 2048 blocks in all, four loads in flight per thread
-  const int per = std::max(16, 2048 / nsets);
-  const int blocks = (int)std::min<int64_t>((Np + 1023) / 1024, per);
-  k_absmax<<<dim3(blocks, nsets), 256, 0, s>>>(Np, w, ldw, out);
+  // up to 2048 blocks, four loads in flight per thread
+  const int blocks = (int)std::min<int64_t>((Np + 1023) / 1024, 2048);
+  k_absmax<<<blocks, 256, 0, s>>>(Np, w, out);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

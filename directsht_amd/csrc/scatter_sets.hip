@@ -69,15 +69,12 @@ __global__ void k_fill_u32(unsigned* __restrict__ p, int n, unsigned v) {
   if (i < n) p[i] = v;
 }
 
-// sidx[pos] = input index of sorted point pos (valid points only), from the
-// matrix engine's records or (spread) the spreading scatter's
+// sidx[pos] = input index of sorted point pos (valid points only)
 __global__ __launch_bounds__(256) void k_sorted_idx(int64_t Np, const int* __restrict__ nvalid,
-                                                    const void* __restrict__ rec, int spread,
+                                                    const PointRec* __restrict__ rec,
                                                     int* __restrict__ sidx) {
   const int64_t pos = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (pos < Np && pos < *nvalid)
-    sidx[pos] = spread ? static_cast<const SpreadRec*>(rec)[pos].idx
-                       : static_cast<const PointRec*>(rec)[pos].idx;
+  if (pos < Np && pos < *nvalid) sidx[pos] = rec[pos].idx;
 }
 
 // The sort of the weight sets into the scatter's order is two coalesced
@@ -155,6 +152,94 @@ __global__ __launch_bounds__(256) void k_gather_sets(const float* __restrict__ w
     const int sl = hi + 4 * r;
     if (lo == 0 && sl < nsets) atomic_max_pos(smax + sl, m);
   }
+}
+
+// The node spreading's weight sets, point-major in sorted order:
+//  k_rank_sorted: rank[idx] = sorted position of input point idx (-1 for the
+//                 points the binning dropped, filled first);
+//  k_place_w:     wts[rank[i] nsets + s] = (float) w_s[i], one 64 x 64 tile
+//                 (points x sets) through LDS per step: coalesced reads of
+//                 each set's weights, each point's row of nsets floats
+//                 written whole at its sorted position.  wmax[s] = max |w_s|
+//                 over the finite weights (of every input point: an upper
+//                 bound for the valid ones), one atomic per set per block.
+__global__ __launch_bounds__(256) void k_rank_sorted(int64_t Np, const int* __restrict__ nvalid,
+                                                     const SpreadRec* __restrict__ rec,
+                                                     int* __restrict__ rank) {
+  const int64_t pos = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (pos < Np && pos < *nvalid) rank[rec[pos].idx] = (int)pos;
+}
+
+__global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_place_w(const double* __restrict__ w, int64_t ldw,
+                                                 int64_t Np, int nsets,
+                                                 const int* __restrict__ rank,
+                                                 float* __restrict__ wts,
+                                                 unsigned* __restrict__ wmax) {
+  __shared__ float tile[kTile][kTile + 1];  // [point][set]
+  const int t = threadIdx.x, lo = t & 63, hi = t >> 6;
+  float mx[kTile / 4];
+#pragma unroll
+  for (int r = 0; r < kTile / 4; ++r) mx[r] = 0.f;
+  for (int tt = 0; tt < kGatherTiles; ++tt) {
+    const int64_t i0 = ((int64_t)blockIdx.x * kGatherTiles + tt) * kTile;
+    if (i0 >= Np) break;  // uniform
+#pragma unroll
+    for (int r = 0; r < kTile / 4; ++r) {
+      const int sl = hi + 4 * r;
+      const int64_t i = i0 + lo;
+      float v = 0.f;
+      if (sl < nsets && i < Np) v = (float)w[(int64_t)sl * ldw + i];
+      tile[lo][sl] = v;
+      const float a = fabsf(v);
+      if (a <= 3.0e38f) mx[r] = fmaxf(mx[r], a);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kTile / 4; ++r) {
+      const int il = hi + 4 * r;
+      const int64_t i = i0 + il;
+      const int q = i < Np ? rank[i] : -1;
+      if (q >= 0 && lo < nsets) wts[(int64_t)q * nsets + lo] = tile[il][lo];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < kTile / 4; ++r) {
+    float m = mx[r];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    const int sl = hi + 4 * r;
+    if (lo == 0 && sl < nsets) atomic_max_pos(wmax + sl, m);
+  }
+}
+
+int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+                            hipStream_t s) {
+  Workspace& ws = pl.ws;
+  if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_wsf_sets || !ws.wsf ||
+      Np > ws.cap_set_points || !uses_spread(pl)) {
+    set_error("dsht: sorting 1..64 weight sets within the workspace");
+    return DSHT_EINVAL;
+  }
+  k_fill_u32<<<1, 64, 0, s>>>(ws.wmax, nsets, 0u);
+  DSHT_LAUNCH_CHECK();
+  if (Np == 0) return DSHT_OK;
+  const unsigned nb = (unsigned)((Np + 255) / 256);
+  k_fill_i32<<<nb, 256, 0, s>>>(ws.sidx, Np, -1);  // sidx holds the ranks here
+  DSHT_LAUNCH_CHECK();
+  k_rank_sorted<<<nb, 256, 0, s>>>(Np, ws.seg_start + pl.nseg,
+                                   reinterpret_cast<const SpreadRec*>(ws.rec), ws.sidx);
+  DSHT_LAUNCH_CHECK();
+  const int64_t per = (int64_t)kTile * kGatherTiles;
+  k_place_w<<<(unsigned)((Np + per - 1) / per), 256, 0, s>>>(w, ldw, Np, nsets, ws.sidx, ws.wsf,
+                                                             ws.wmax);
+  DSHT_LAUNCH_CHECK();
+  return DSHT_OK;
 }
 
 // Weight-set scatter, RT row tiles of 32 sets (RT = 2: up to 64 sets).
@@ -794,7 +879,8 @@ __global__ __launch_bounds__(64 * NW, 1) void k_scatter_wsets(
   }
 }
 
-int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+// The matrix engine's sorted weight rows: ws.wsf[s][pos] = w_s[sidx[pos]].
+static int sort_weight_rows(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
                             hipStream_t s) {
   Workspace& ws = pl.ws;
   if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_wsf_sets || !ws.wsf ||
@@ -806,8 +892,7 @@ int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, 
   k_fill_u32<<<1, 64, 0, s>>>(ws.wmax, nsets, 0u);
   DSHT_LAUNCH_CHECK();
   if (Np == 0) return DSHT_OK;
-  k_sorted_idx<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, nvalid, ws.rec, uses_spread(pl) ? 1 : 0,
-                                                       ws.sidx);
+  k_sorted_idx<<<(int)((Np + 255) / 256), 256, 0, s>>>(Np, nvalid, ws.rec, ws.sidx);
   DSHT_LAUNCH_CHECK();
   // wt (point-major copy) lives in the second half of the wsf allocation
   float* wt = ws.wsf + (int64_t)ws.cap_wsf_sets * ws.ldws;
@@ -833,7 +918,7 @@ int launch_scatter_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int 
     return DSHT_EINVAL;
   }
   int rc;
-  if ((rc = launch_sort_weight_sets(pl, Np, w, ldw, nsets, s))) return rc;
+  if ((rc = sort_weight_rows(pl, Np, w, ldw, nsets, s))) return rc;
   const int nmy = (pl.Nl + kOrders - 1) / kOrders;
   const int64_t nwg = (int64_t)8 * ((nchunks + 7) / 8) * nmy;
   const int64_t stride = part_set_stride(pl);

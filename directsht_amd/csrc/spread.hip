@@ -356,16 +356,15 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 // ntot][16] of the batched fold: branch br at 8 br, re then im, each
 // c(2np), c(2np+1), dx d(2np), dx d(2np+1)), with the set's max(|c|, |d|)
 // into rmax[s] (float bits, cleared beforehand).  Set s weighs its points by
-// wsf[s ldws + sorted position] (FP32 weight sets in sorted order, scaled by
-// wmax[s]), or by the records' own weights if wsf is NULL (wmax[0]).  The
-// workgroups of one node pair for all sets run together on one XCD (block
-// b -> XCD b mod 8, the order in which the hardware hands them out): they
-// read the same records (from that XCD's L2) and write the same coefficient
-// rows at about the same time, so the sets' 32-byte pieces of a row meet in
-// the caches and reach HBM as whole lines.
+// wts[sorted position * count + s] (FP32 weight sets, point-major in sorted
+// order, scaled by wmax[s]), or by the records' own weights if wts is NULL
+// (wmax[0]).  The workgroups of one node pair for all sets run together on
+// one XCD (block b -> XCD b mod 8, the order in which the hardware hands them
+// out): they read the same records and weight rows (from that XCD's L2) and
+// write the same coefficient rows at about the same time, so the sets'
+// 32-byte pieces of a row meet in the caches and reach HBM as whole lines.
 struct NodeSets {
-  const float* wsf;
-  int64_t ldws;
+  const float* wts;
   unsigned* rmax;
   int count, set0, ntot;
 };
@@ -390,7 +389,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
     sl = k % ns.count;
     if (np >= Npair) return;  // the padding of the last round
   }
-  const float* wsf = kSets && ns.wsf ? ns.wsf + sl * ns.ldws : nullptr;
+  const float* wsf = kSets && ns.wts ? ns.wts + sl : nullptr;  // set sl's weights, stride count
   const int m = tid;  // this thread's mode
   // twiddles of the first radix-8 pass in registers (the spectra held across
   // the units leave no room for more), the others from the table
@@ -512,7 +511,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
             for (int b = 0; b < kNodeBatch; ++b) {
               const int q = min(q0 + b * kT, hi[h] - 1);
               if (kSets)
-                wt[b] = (wsf ? (double)wsf[q] : 1.0) * wf;
+                wt[b] = (wsf ? (double)wsf[(int64_t)q * ns.count] : 1.0) * wf;
               else
                 wt[b] = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri[b].y])) * wf;
             }
@@ -664,7 +663,7 @@ int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorte
     set_error("dsht: node spreading into batch slots beyond the workspace's sets");
     return DSHT_EINVAL;
   }
-  NodeSets ns{sorted_w ? ws.wsf : nullptr, ws.ldws, ws.rmax + set0, count, set0, ntot};
+  NodeSets ns{sorted_w ? ws.wsf : nullptr, ws.rmax + set0, count, set0, ntot};
   int rc;
   if ((rc = launch_clear_u32(ws.rmax + set0, nullptr, count, s))) return rc;
   return nodes_dispatch(pl, nullptr, 0, ws.wmax, &ns, s);
