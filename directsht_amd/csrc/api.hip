@@ -801,8 +801,8 @@ static int run_batch(Plan& pl, const BatchArgs& a, double* alm, int* status, int
     // one slot per set: fold every set in one launch
     const bool fold_all = sets_scatter && mfma;
     if (fold_all && (rc = launch_fold_sets(pl, a.nsets, s, ws.part, part_set_stride(pl)))) return rc;
-    if (node_sets) {  // sorted FP32 weight rows, then every set in one launch
-      if ((rc = launch_sort_weight_sets(pl, Np, a.wsets, a.ldw, a.nsets, s))) return rc;
+    if (node_sets) {  // point-major FP32 weight rows, then every set in one launch
+      if ((rc = launch_place_weight_sets(pl, Np, a.wsets, a.ldw, a.nsets, s))) return rc;
       if ((rc = launch_spread_nodes_sets(pl, 0, a.nsets, a.nsets, true, s))) return rc;
     }
     for (int g0 = 0; spread_group && g0 < a.nsets; g0 += spread_group) {
@@ -1209,7 +1209,7 @@ int dsht_plan_reserve_for(dsht_plan* p, int64_t Np, int nsets, int kinds) {
   int rc = DSHT_OK;
   if (kinds & DSHT_RESERVE_WEIGHT_SETS) {
     // room for either engine a transform of up to Np points may take
-    // (the node spreading of weight sets: their sorted rows)
+    // (the node spreading of weight sets: their point-major rows)
     pl.scatter_now = pick_scatter(pl, Np, kPickWeightSets, nsets);
     const bool rows = use_mfma_contract(pl, nsets) && uses_spread(pl) && node_spread_fits(pl);
     pl.scatter_now = pick_scatter(pl, 0);

@@ -89,11 +89,10 @@ struct Workspace {
   unsigned* rmax = nullptr;   // [cap_sets]: max(|c|, |d/dx|) per set (float bits)
   unsigned* wmax = nullptr;   // [cap_sets]: max |w| per weight set (float bits)
   int* sidx = nullptr;        // [cap_points]: input index of each sorted point
-                              // (weight-set scatter); sorted position of each input
-                              // point (node spreading of weight sets)
+                              // (weight-set scatter)
   float* wsf = nullptr;       // [cap_wsf_sets][ldws]: weight sets in sorted order,
                               // FP32, zero past the valid points (weight-set scatter;
-                              // the node spreading of weight sets: [pos][nsets]);
+                              // the node spreading of weight sets: [point][nsets]);
                               // followed by the same size of scratch for the sort
   int64_t ldws = 0;           // row stride of wsf: cap_set_points rounded up to 64, + 64
   double* wsort = nullptr;    // [cap_points]: one weight set in sorted order
@@ -341,14 +340,13 @@ int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStr
 // The node variant for sets set0 .. set0 + count - 1 of a batch of ntot sets
 // (matrix-core contraction), one launch, into their 16-value row slots, with
 // each set's bound into ws.rmax (cleared here): sorted_w, the weight sets of
-// launch_sort_weight_sets (count of them, maxima in ws.wmax[0 .. count));
+// launch_place_weight_sets (count of them, maxima in ws.wmax[0 .. count));
 // else one set of the records' own weights (count 1, ws.wmax[0]).
 int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorted_w,
                              hipStream_t s);
-// The nsets weight sets w + k ldw (k < nsets) of the spreading scatter's
-// sorted points, FP32, point-major in sorted order into ws.wsf
-// ([pos][nsets]), and max |w| per set into ws.wmax.
-int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+// The nsets weight sets w + k ldw (k < nsets), FP32, point-major into
+// ws.wsf ([input index][nsets]), and max |w| per set into ws.wmax.
+int launch_place_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
                             hipStream_t s);
 // the spreading kernel's values at the nq points z (host arrays), as the
 // device evaluates them in the scatters (synchronous; at plan creation)

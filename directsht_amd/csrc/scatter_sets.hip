@@ -154,31 +154,17 @@ __global__ __launch_bounds__(256) void k_gather_sets(const float* __restrict__ w
   }
 }
 
-// The node spreading's weight sets, point-major in sorted order:
-//  k_rank_sorted: rank[idx] = sorted position of input point idx (-1 for the
-//                 points the binning dropped, filled first);
-//  k_place_w:     wts[rank[i] nsets + s] = (float) w_s[i], one 64 x 64 tile
-//                 (points x sets) through LDS per step: coalesced reads of
-//                 each set's weights, each point's row of nsets floats
-//                 written whole at its sorted position.  wmax[s] = max |w_s|
-//                 over the finite weights (of every input point: an upper
-//                 bound for the valid ones), one atomic per set per block.
-__global__ __launch_bounds__(256) void k_rank_sorted(int64_t Np, const int* __restrict__ nvalid,
-                                                     const SpreadRec* __restrict__ rec,
-                                                     int* __restrict__ rank) {
-  const int64_t pos = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (pos < Np && pos < *nvalid) rank[rec[pos].idx] = (int)pos;
-}
-
-__global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, int64_t n, int v) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
+// The node spreading's weight sets, point-major: k_place_w writes
+// wt[i nsets + s] = (float) w_s[i], one 64 x 64 tile (points x sets) through
+// LDS per step (coalesced reads of each set's weights, each point's row of
+// nsets floats written whole), and wmax[s] = max |w_s| over the finite
+// weights (of every input point: an upper bound for the points the binning
+// keeps), one atomic per set per block.  The node kernel reads a point's
+// weights through its record's input index: the workgroups of all sets of a
+// node pair run together on one XCD, so each row comes from HBM once.
 __global__ __launch_bounds__(256) void k_place_w(const double* __restrict__ w, int64_t ldw,
                                                  int64_t Np, int nsets,
-                                                 const int* __restrict__ rank,
-                                                 float* __restrict__ wts,
+                                                 float* __restrict__ wt,
                                                  unsigned* __restrict__ wmax) {
   __shared__ float tile[kTile][kTile + 1];  // [point][set]
   const int t = threadIdx.x, lo = t & 63, hi = t >> 6;
@@ -203,8 +189,7 @@ __global__ __launch_bounds__(256) void k_place_w(const double* __restrict__ w, i
     for (int r = 0; r < kTile / 4; ++r) {
       const int il = hi + 4 * r;
       const int64_t i = i0 + il;
-      const int q = i < Np ? rank[i] : -1;
-      if (q >= 0 && lo < nsets) wts[(int64_t)q * nsets + lo] = tile[il][lo];
+      if (i < Np && lo < nsets) wt[i * nsets + lo] = tile[il][lo];
     }
     __syncthreads();
   }
@@ -218,26 +203,19 @@ __global__ __launch_bounds__(256) void k_place_w(const double* __restrict__ w, i
   }
 }
 
-int launch_sort_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
+int launch_place_weight_sets(Plan& pl, int64_t Np, const double* w, int64_t ldw, int nsets,
                             hipStream_t s) {
   Workspace& ws = pl.ws;
   if (nsets < 1 || nsets > 64 || nsets > ws.cap_sets || nsets > ws.cap_wsf_sets || !ws.wsf ||
-      Np > ws.cap_set_points || !uses_spread(pl)) {
-    set_error("dsht: sorting 1..64 weight sets within the workspace");
+      Np > ws.cap_set_points) {
+    set_error("dsht: placing 1..64 weight sets within the workspace");
     return DSHT_EINVAL;
   }
   k_fill_u32<<<1, 64, 0, s>>>(ws.wmax, nsets, 0u);
   DSHT_LAUNCH_CHECK();
   if (Np == 0) return DSHT_OK;
-  const unsigned nb = (unsigned)((Np + 255) / 256);
-  k_fill_i32<<<nb, 256, 0, s>>>(ws.sidx, Np, -1);  // sidx holds the ranks here
-  DSHT_LAUNCH_CHECK();
-  k_rank_sorted<<<nb, 256, 0, s>>>(Np, ws.seg_start + pl.nseg,
-                                   reinterpret_cast<const SpreadRec*>(ws.rec), ws.sidx);
-  DSHT_LAUNCH_CHECK();
   const int64_t per = (int64_t)kTile * kGatherTiles;
-  k_place_w<<<(unsigned)((Np + per - 1) / per), 256, 0, s>>>(w, ldw, Np, nsets, ws.sidx, ws.wsf,
-                                                             ws.wmax);
+  k_place_w<<<(unsigned)((Np + per - 1) / per), 256, 0, s>>>(w, ldw, Np, nsets, ws.wsf, ws.wmax);
   DSHT_LAUNCH_CHECK();
   return DSHT_OK;
 }

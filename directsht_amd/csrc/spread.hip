@@ -356,9 +356,8 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_scatter_spread(
 // ntot][16] of the batched fold: branch br at 8 br, re then im, each
 // c(2np), c(2np+1), dx d(2np), dx d(2np+1)), with the set's max(|c|, |d|)
 // into rmax[s] (float bits, cleared beforehand).  Set s weighs its points by
-// wts[sorted position * count + s] (FP32 weight sets, point-major in sorted
-// order, scaled by wmax[s]), or by the records' own weights if wts is NULL
-// (wmax[0]).  The workgroups of one node pair for all sets run together on
+// wts[input index * count + s] (FP32 weight sets, point-major, scaled by
+// wmax[s]), or by the records' own weights if wts is NULL (wmax[0]).  The workgroups of one node pair for all sets run together on
 // one XCD (block b -> XCD b mod 8, the order in which the hardware hands them
 // out): they read the same records and weight rows (from that XCD's L2) and
 // write the same coefficient rows at about the same time, so the sets'
@@ -511,7 +510,7 @@ __global__ __launch_bounds__(SpreadGeom<LOGN>::kThreads) void k_spread_nodes(
             for (int b = 0; b < kNodeBatch; ++b) {
               const int q = min(q0 + b * kT, hi[h] - 1);
               if (kSets)
-                wt[b] = (wsf ? (double)wsf[(int64_t)q * ns.count] : 1.0) * wf;
+                wt[b] = (wsf ? (double)wsf[(int64_t)ri[b].y * ns.count] : 1.0) * wf;
               else
                 wt[b] = (wmode == 0 ? 1.0 : (wmode == 1 ? w[q] : w[ri[b].y])) * wf;
             }
