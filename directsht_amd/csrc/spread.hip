@@ -16,10 +16,12 @@
 // ~2e-8 relative error, under the FP32 rounding of the transform.  The work
 // per point is 4 W grid updates instead of 4 Nl products.
 //
-// Two kernels: k_spread_nodes (single transforms on grids of up to 4096
-// cells with 512 or more node pairs) spreads per node and includes the
-// node/parity fold (below, before its launcher); k_scatter_spread (the
-// rest) spreads per run and leaves the fold to fold.hip.  Common to both:
+// Two kernels: k_spread_nodes (grids of up to 4096 cells with 512 or more
+// node pairs: single transforms, and the weight sets and dense point sets
+// of batches, straight into their contraction slots) spreads per node and
+// includes the node/parity fold (below, before its launcher);
+// k_scatter_spread (the rest) spreads per run and leaves the fold to
+// fold.hip.  Common to both:
 //   * the kernel values 2^(beta log2(e) (sqrt(1 - z^2) - 1)) come from the
 //     hardware sqrt and exp2 (es_kernel), and 1/psihat is the transform of
 //     that very function (spread_kernel_values at plan creation).
