@@ -339,10 +339,10 @@ inline bool uses_node_spread(const Plan& pl, int nsets) {
 int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStream_t s);
 // The node variant for sets set0 .. set0 + count - 1 of a batch of ntot sets
 // (matrix-core contraction), one launch, into their 16-value row slots, with
-// each set's bound into ws.rmax (cleared here): sorted_w, the weight sets of
+// each set's bound into ws.rmax (cleared here): weight_rows, the weight sets of
 // launch_place_weight_sets (count of them, maxima in ws.wmax[0 .. count));
 // else one set of the records' own weights (count 1, ws.wmax[0]).
-int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorted_w,
+int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool weight_rows,
                              hipStream_t s);
 // The nsets weight sets w + k ldw (k < nsets), FP32, point-major into
 // ws.wsf ([input index][nsets]), and max |w| per set into ws.wmax.

@@ -656,15 +656,16 @@ int launch_spread_nodes(Plan& pl, int64_t Np, const double* w, int wmode, hipStr
   return nodes_dispatch(pl, w, wmode, pl.ws.wmax, nullptr, s);
 }
 
-int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool sorted_w,
+int launch_spread_nodes_sets(Plan& pl, int set0, int count, int ntot, bool weight_rows,
                              hipStream_t s) {
   Workspace& ws = pl.ws;
   if (count < 1 || set0 < 0 || set0 + count > ntot || ntot > ws.cap_sets ||
-      (sorted_w && (count > ws.cap_wsf_sets || !ws.wsf))) {
+      (int64_t)pl.Nl * 2 * pl.Npair * 16 * ntot > ws.rhs_elems ||
+      (weight_rows && (count > ws.cap_wsf_sets || !ws.wsf))) {
     set_error("dsht: node spreading into batch slots beyond the workspace's sets");
     return DSHT_EINVAL;
   }
-  NodeSets ns{sorted_w ? ws.wsf : nullptr, ws.rmax + set0, count, set0, ntot};
+  NodeSets ns{weight_rows ? ws.wsf : nullptr, ws.rmax + set0, count, set0, ntot};
   int rc;
   if ((rc = launch_clear_u32(ws.rmax + set0, nullptr, count, s))) return rc;
   return nodes_dispatch(pl, nullptr, 0, ws.wmax, &ns, s);
